@@ -1,0 +1,75 @@
+"""Build recipe of the native libraries (in-tree, no JIT cache).
+
+    python -m edus_amd.build            # engine (gfx950) + host library + runner
+
+Outputs (git-ignored, shipped with the tree):
+    edus_amd/lib/libedus_hip.so    HIP engine, C-ABI of include/edus.h
+    edus_amd/lib/libedus_host.so   C++ host (input, setup, time loop), include/edus_host.h
+    edus_amd/lib/edus_run          drop-in executable:  edus_run input.txt [outdir]
+"""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "edus_amd")
+LIB = os.path.join(PKG, "lib")
+CSRC = os.path.join(PKG, "csrc")
+ARCH = "gfx950"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# Packed FP32 instructions are disabled for every HIP object of this project.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-variable"]
+
+HOST_SOURCES = ["input.cpp", "lattice.cpp", "linalg.cpp", "models.cpp", "setup.cpp", "runner.cpp", "capi.cpp"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_engine():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libedus_hip.so")
+    src = os.path.join(CSRC, "hip", "engine.hip")
+    _run([HIPCC, *HIP_FLAGS, src, "-o", out])
+    return out
+
+
+def build_host():
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    for s in HOST_SOURCES:
+        o = os.path.join(LIB, "_obj_" + s.replace(".cpp", ".o"))
+        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "host", s), "-o", o])
+        objs.append(o)
+    out = os.path.join(LIB, "libedus_host.so")
+    _run(["g++", "-shared", "-fopenmp", *objs, "-o", out, f"-L{LIB}", "-ledus_hip", "-Wl,-rpath,$ORIGIN",
+          "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"])
+    exe = os.path.join(LIB, "edus_run")
+    _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "edus_run.cpp"), "-o", exe, f"-L{LIB}", "-ledus_host", "-ledus_hip",
+          "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"])
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_all():
+    build_engine()
+    build_host()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    if "engine" in what:
+        build_engine()
+    if "host" in what:
+        build_host()
+    if "all" in what:
+        build_all()

@@ -1,0 +1,1312 @@
+// libedus_hip.so -- the MI355X engine of the EDUS time-stepping path.
+//
+// Device layout (all FP64):
+//   * The density matrix is exactly Hermitian with a real diagonal at every step
+//     of the reference's three steppers (the right-hand side is hermitised,
+//     RungeKutta.h:283-294, and every update is a real-linear combination), so a
+//     k point is stored packed: nb real diagonal entries, then (re, im) of the
+//     strict upper triangle in row-major pair order -> nb*nb doubles.  This is
+//     lossless and halves the HBM traffic of the reference's full complex arrays.
+//   * nb <= 4: "SoA" -- element e of k at e*nkp + k (one thread per k, coalesced).
+//     nb >= 5: "AoS" -- k-major, nb*nb contiguous doubles (one wavefront per k).
+//   * H is packed like the state (upper triangle, as the RHS reads it);
+//     D components are packed symmetric (D_ji = D_ij); U is full complex.
+//
+// Kernels: fused RHS + stage update per RK4 stage / Adams-Bashforth step /
+// Taylor step (RungeKutta.h:7-355, RungeKutta.cpp, Linear_Multistep_ODE_solvers.cpp,
+// Taylor_DE_solver_deriv_Polynom.cpp), wave-per-k observable reductions with
+// Kahan-compensated partials (Observables_MPI.h:217-775) and the Wannier phase
+// contraction (Wannier.h:278-471).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/edus.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string &m) {
+    g_err = m;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return set_err(e_ == hipErrorOutOfMemory ? EDUS_ERR_NOMEM : EDUS_ERR_HIP,                \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+    } while (0)
+
+enum Mode { RK1 = 0, RK2 = 1, RK3 = 2, RK4S = 3, ABM = 4, TAY = 5, RHSO = 6 };
+
+constexpr int kMaxNbr = 64;
+constexpr int kMaxTinv = (EDUS_MAX_ORDER - 1) * (EDUS_MAX_ORDER - 1);
+
+// Per-context constants of the right-hand side.
+struct ConstArgs {
+    const double *H;     // packed (layout of the state)
+    const double *D;     // 3 packed symmetric components
+    const double *U;     // full complex AoS [k][nb][nb] (dissipation)
+    const double *T;     // [nb][nb]
+    const int *nbr;      // [n_nbr][nkp]
+    int nk, nkp, nb, nch, ncc, n_nbr, diss, pad;
+    double wx2;
+};
+
+// Per-launch stage data.  `tab` points at the stage's row of the field table:
+// [0..2] pump sum, [3..5] half probe, [6..6+n_nbr) gradient coefficients
+// c_n = sum_x E_pump[x] * w_shell b_n[x], [6+n_nbr] gradient flag.
+struct StageArgs {
+    const double *Y;     // stage input (neighbours gathered from it)
+    const double *P0;    // RK: state at t_n
+    double *ACC;         // RK: accumulator
+    double *OUT;         // stage output
+    double *HIST;        // AB/Taylor history slots
+    const double *tab;
+    int slot_out, order, pad0, pad1;
+    int slots[EDUS_MAX_ORDER];
+    double coef[EDUS_MAX_ORDER];
+    double tinv[kMaxTinv];
+    double dtp[EDUS_MAX_ORDER];
+    double ca, cb, dt;
+};
+
+// ---------------------------------------------------------------- complex helpers
+struct cd {
+    double x, y;
+};
+__device__ __forceinline__ cd mk(double a, double b) { return cd{a, b}; }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cd cconj(cd a) { return cd{a.x, -a.y}; }
+__device__ __forceinline__ cd cscale(cd a, double s) { return cd{a.x * s, a.y * s}; }
+
+// Block index remapped so that each of the 8 XCDs (which receive workgroups
+// round-robin) works on one contiguous range of k: the 12-point gradient stencil
+// then finds its neighbours in the XCD's own L2.
+__device__ __forceinline__ int xcd_block() {
+    const int G = gridDim.x, b = blockIdx.x;
+    const int x = b & 7, idx = b >> 3, per = G >> 3, rem = G & 7;
+    return x * per + min(x, rem) + idx;
+}
+
+// Pair index of (i<j) in row-major upper-triangle order.
+__host__ __device__ __forceinline__ int pair_index(int i, int j, int nb) { return i * nb - i * (i + 1) / 2 + (j - i - 1); }
+
+// ---------------------------------------------------------------- stage update
+// The elementwise tail shared by all kernels.  `kv` is one packed double of the
+// right-hand side, `y` the same double of the stage input, `o` its offset in the
+// state arrays and `h` the stride between history slots.
+template <int MODE>
+__device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_t h, double kv, double y) {
+    if constexpr (MODE == RK1) {
+        a.ACC[o] = y + kv * a.ca;
+        a.OUT[o] = y + kv * a.cb;
+    } else if constexpr (MODE == RK2 || MODE == RK3) {
+        a.ACC[o] = a.ACC[o] + kv * a.ca;
+        a.OUT[o] = a.P0[o] + kv * a.cb;
+    } else if constexpr (MODE == RK4S) {
+        a.OUT[o] = a.ACC[o] + kv * a.ca;
+    } else if constexpr (MODE == ABM) {
+        a.HIST[(size_t)a.slot_out * h + o] = kv;
+        double delta = a.coef[0] * kv;
+        for (int s = 1; s < a.order; s++) delta += a.coef[s] * a.HIST[(size_t)a.slots[s] * h + o];
+        a.OUT[o] = y + a.dt * delta;
+    } else if constexpr (MODE == RHSO) {
+        a.OUT[o] = kv;
+    } else {  // TAY
+        a.HIST[(size_t)a.slot_out * h + o] = kv;
+        const int n = a.order - 1;
+        double dp[EDUS_MAX_ORDER];
+        for (int i = 0; i < n; i++) dp[i] = kv - a.HIST[(size_t)a.slots[i + 1] * h + o];
+        double delta = kv * a.dt;
+        for (int i = 1; i <= n; i++) {
+            double al = 0.0;
+            for (int p = 0; p < n; p++) al += a.tinv[(i - 1) * n + p] * dp[p];
+            delta += (a.dtp[i] * al) / (double)(i + 1);
+        }
+        a.OUT[o] = y + delta;
+    }
+}
+
+// ================================================================ nb <= 4: thread per k
+template <int NB, int MODE>
+__global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
+    constexpr int NP = NB * NB;
+    const int k = xcd_block() * 256 + threadIdx.x;
+    if (k >= c.nk) return;
+    const size_t nkp = c.nkp;
+    const double *tab = a.tab;
+    const double Ep[3] = {tab[0], tab[1], tab[2]}, Eq[3] = {tab[3], tab[4], tab[5]};
+
+    double y[NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) y[e] = a.Y[e * nkp + k];
+    cd P[NB][NB], M[NB][NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        P[i][i] = mk(y[i], 0.0);
+#pragma unroll
+        for (int j = i + 1; j < NB; j++) {
+            const int p = pair_index(i, j, NB);
+            P[i][j] = mk(y[NB + 2 * p], y[NB + 2 * p + 1]);
+            P[j][i] = cconj(P[i][j]);
+        }
+    }
+    // M = H + E.D with the probe on core<->non-core elements (RungeKutta.h:165-168).
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+#pragma unroll
+        for (int j = i; j < NB; j++) {
+            const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
+            const double *E = w ? Eq : Ep;
+            cd h, dx[3];
+            if (i == j) {
+                h = mk(c.H[i * nkp + k], 0.0);
+#pragma unroll
+                for (int x = 0; x < 3; x++) dx[x] = mk(c.D[(x * NP + i) * nkp + k], 0.0);
+            } else {
+                const int p = pair_index(i, j, NB);
+                h = mk(c.H[(NB + 2 * p) * nkp + k], c.H[(NB + 2 * p + 1) * nkp + k]);
+#pragma unroll
+                for (int x = 0; x < 3; x++) dx[x] = mk(c.D[(x * NP + NB + 2 * p) * nkp + k], c.D[(x * NP + NB + 2 * p + 1) * nkp + k]);
+            }
+            cd mij = h, mji = cconj(h);
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                mij = cadd(mij, cscale(dx[x], E[x]));
+                mji = cadd(mji, cscale(dx[x], E[x]));
+            }
+            M[i][j] = mij;
+            M[j][i] = mji;
+        }
+    }
+    // -i[M, rho] in the transposed storage (RungeKutta.h:158-212).
+    double pv[NP];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        cd beta = mk(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < NB; j++) beta = cadd(beta, cmul(M[i][j], P[i][j]));
+        pv[i] = 2. * beta.y;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = i + 1; j < NB; j++) {
+            cd beta = mk(0.0, 0.0);
+#pragma unroll
+            for (int l = 0; l < NB; l++) beta = cadd(beta, csub(cmul(M[j][l], P[i][l]), cconj(cmul(M[i][l], P[j][l]))));
+            cd v = mk(beta.y, -beta.x);                      // -i * beta
+            if (i < c.nch && j >= c.nch) v = cadd(v, mk(-c.wx2 * P[i][j].y, c.wx2 * P[i][j].x));  // RWA term
+            const int p = pair_index(i, j, NB);
+            pv[NB + 2 * p] = v.x;
+            pv[NB + 2 * p + 1] = v.y;
+        }
+    // Dephasing in the eigenbasis (RungeKutta.h:219-254) as
+    // Pv -= conj(U) (T o (conj(U) P U^T)) conj(U) plus the hole source.
+    if (c.diss) {
+        cd U[NB][NB];
+        const double *u = c.U + (size_t)k * NB * NB * 2;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) U[i][j] = mk(u[(i * NB + j) * 2], u[(i * NB + j) * 2 + 1]);
+        cd G[NB][NB];
+#pragma unroll
+        for (int ia = 0; ia < NB; ia++)
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++) {
+                cd g = mk(0.0, 0.0);
+#pragma unroll
+                for (int i = 0; i < NB; i++) {
+                    cd t = mk(0.0, 0.0);
+#pragma unroll
+                    for (int j = 0; j < NB; j++) t = cadd(t, cmul(U[ib][j], P[i][j]));
+                    g = cadd(g, cmul(cconj(U[ia][i]), t));
+                }
+                G[ia][ib] = cscale(g, c.T[ia * NB + ib]);
+            }
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = i; j < NB; j++) {
+                cd w = mk(0.0, 0.0);
+#pragma unroll
+                for (int ia = 0; ia < NB; ia++) {
+                    cd t = mk(0.0, 0.0);
+#pragma unroll
+                    for (int ib = 0; ib < NB; ib++) t = cadd(t, cmul(G[ia][ib], cconj(U[ib][j])));
+                    w = cadd(w, cmul(cconj(U[i][ia]), t));
+                }
+                cd hole = mk(0.0, 0.0);
+                for (int v = 0; v < c.ncc; v++) hole = cadd(hole, cscale(cmul(cconj(U[v][j]), U[v][i]), c.T[v * NB + v]));
+                if (i == j) {
+                    pv[i] = pv[i] - w.x + hole.x;
+                } else {
+                    const int p = pair_index(i, j, NB);
+                    pv[NB + 2 * p] = pv[NB + 2 * p] - w.x + hole.x;
+                    pv[NB + 2 * p + 1] = pv[NB + 2 * p + 1] - w.y + hole.y;
+                }
+            }
+    }
+    // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).
+    const int nn = c.n_nbr;
+    if (tab[6 + nn] != 0.0) {
+        double g[NP];
+#pragma unroll
+        for (int e = 0; e < NP; e++) g[e] = 0.0;
+        for (int n = 0; n < nn; n++) {
+            const int kn = c.nbr[(size_t)n * nkp + k];
+            const double cn = tab[6 + n];
+#pragma unroll
+            for (int e = 0; e < NP; e++) g[e] += cn * a.Y[e * nkp + kn];
+        }
+#pragma unroll
+        for (int e = 0; e < NP; e++) pv[e] += g[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NP; e++) stage_update<MODE>(a, e * nkp + k, (size_t)NP * nkp, pv[e], y[e]);
+}
+
+// ================================================================ nb in 5..32: wavefront per k
+// LDS GEMM of one wavefront: lanes form an 8x8 grid, each owning an RB x RB
+// block of the NBT x NBT result.  OPA: conj(A) if 1.  OPB: 0 -> B[j][l]
+// (result A B^T), 1 -> B[l][j], 2 -> conj(B[l][j]).
+template <int NBT, int OPA, int OPB>
+__device__ __forceinline__ void gemm_lds(const cd *A, const cd *B, int nb, int lane, cd (&acc)[NBT / 8][NBT / 8]) {
+    constexpr int RB = NBT / 8, LD = NBT + 1;
+    const int r0 = (lane >> 3) * RB, c0 = (lane & 7) * RB;
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < RB; j++) acc[i][j] = mk(0.0, 0.0);
+    for (int l = 0; l < nb; l++) {
+        cd av[RB], bv[RB];
+#pragma unroll
+        for (int i = 0; i < RB; i++) {
+            cd t = A[(r0 + i) * LD + l];
+            av[i] = OPA ? cconj(t) : t;
+        }
+#pragma unroll
+        for (int j = 0; j < RB; j++) {
+            cd t = (OPB == 0) ? B[(c0 + j) * LD + l] : B[l * LD + c0 + j];
+            bv[j] = (OPB == 2) ? cconj(t) : t;
+        }
+#pragma unroll
+        for (int i = 0; i < RB; i++)
+#pragma unroll
+            for (int j = 0; j < RB; j++) {
+                acc[i][j].x += av[i].x * bv[j].x - av[i].y * bv[j].y;
+                acc[i][j].y += av[i].x * bv[j].y + av[i].y * bv[j].x;
+            }
+    }
+}
+
+template <int NBT>
+__device__ __forceinline__ void store_block(cd *C, int lane, const cd (&acc)[NBT / 8][NBT / 8]) {
+    constexpr int RB = NBT / 8, LD = NBT + 1;
+    const int r0 = (lane >> 3) * RB, c0 = (lane & 7) * RB;
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < RB; j++) C[(r0 + i) * LD + c0 + j] = acc[i][j];
+}
+
+// (i, j, component) of packed double e: diag -> (e, e, 0); pair -> (i<j, re/im).
+__device__ __forceinline__ void unpack_index(int e, int nb, int &i, int &j, int &comp) {
+    if (e < nb) { i = j = e; comp = 0; return; }
+    int q = e - nb, p = q >> 1;
+    comp = q & 1;
+    int r = 0, rowlen = nb - 1;
+    while (p >= rowlen) { p -= rowlen; r++; rowlen--; }
+    i = r;
+    j = r + 1 + p;
+}
+
+template <int NBT>
+constexpr int waves_per_block() { return NBT >= 32 ? 2 : 4; }
+
+template <int NBT, int MODE>
+__global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(ConstArgs c, StageArgs a) {
+    constexpr int WPB = waves_per_block<NBT>();
+    constexpr int LD = NBT + 1, MAT = NBT * LD, RB = NBT / 8;
+    constexpr int PER = (NBT * NBT + 63) / 64;   // packed doubles per lane (upper bound)
+    extern __shared__ cd smem[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = xcd_block() * WPB + wave;
+    const bool valid = k < c.nk;
+    const int nb = c.nb, NP = nb * nb;
+    cd *A = smem + (size_t)wave * 3 * MAT, *B = A + MAT, *C = B + MAT;
+    const double *tab = a.tab;
+    const double Ep[3] = {tab[0], tab[1], tab[2]}, Eq[3] = {tab[3], tab[4], tab[5]};
+    const size_t base = (size_t)k * NP;
+
+    if (nb < NBT)
+        for (int t = lane; t < 3 * MAT; t += 64) A[t] = mk(0.0, 0.0);
+    double y[PER], pv[PER];
+    int ei[PER], ej[PER], ec[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = lane + 64 * m;
+        y[m] = 0.0;
+        pv[m] = 0.0;
+        ei[m] = ej[m] = ec[m] = 0;
+        if (valid && e < NP) {
+            unpack_index(e, nb, ei[m], ej[m], ec[m]);
+            y[m] = a.Y[base + e];
+        }
+    }
+    __syncthreads();
+    // A <- P (full Hermitian), B <- M = H + E.D
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = lane + 64 * m;
+        if (!valid || e >= NP) continue;
+        const int i = ei[m], j = ej[m], comp = ec[m];
+        const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
+        const double *E = w ? Eq : Ep;
+        const double hv = c.H[base + e];
+        double dsum = hv;
+#pragma unroll
+        for (int x = 0; x < 3; x++) dsum += E[x] * c.D[(size_t)x * c.nk * NP + base + e];
+        double *Ad = reinterpret_cast<double *>(A), *Bd = reinterpret_cast<double *>(B);
+        if (i == j) {
+            A[i * LD + i] = mk(y[m], 0.0);
+            B[i * LD + i] = mk(dsum, 0.0);
+        } else if (comp == 0) {
+            Ad[2 * (i * LD + j)] = y[m];
+            Ad[2 * (j * LD + i)] = y[m];
+            Bd[2 * (i * LD + j)] = dsum;
+            Bd[2 * (j * LD + i)] = dsum;
+        } else {
+            double dd = 0.0;
+#pragma unroll
+            for (int x = 0; x < 3; x++) dd += E[x] * c.D[(size_t)x * c.nk * NP + base + e];
+            Ad[2 * (i * LD + j) + 1] = y[m];
+            Ad[2 * (j * LD + i) + 1] = -y[m];
+            Bd[2 * (i * LD + j) + 1] = dsum;
+            Bd[2 * (j * LD + i) + 1] = -hv + dd;
+        }
+    }
+    __syncthreads();
+    // X = P M^T; -i[M,rho] (transposed storage) = -i (X - X^+)
+    {
+        cd acc[RB][RB];
+        gemm_lds<NBT, 0, 0>(A, B, nb, lane, acc);
+        store_block<NBT>(C, lane, acc);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = lane + 64 * m;
+        if (!valid || e >= NP) continue;
+        const int i = ei[m], j = ej[m];
+        if (i == j) {
+            pv[m] = 2. * C[i * LD + i].y;
+        } else {
+            cd beta = csub(C[i * LD + j], cconj(C[j * LD + i]));
+            cd v = mk(beta.y, -beta.x);
+            if (i < c.nch && j >= c.nch) {
+                cd pij = A[i * LD + j];
+                v = cadd(v, mk(-c.wx2 * pij.y, c.wx2 * pij.x));
+            }
+            pv[m] = ec[m] ? v.y : v.x;
+        }
+    }
+    if (c.diss) {
+        __syncthreads();
+        // C <- U
+        const double *u = c.U + (size_t)k * nb * nb * 2;
+        for (int t = lane; t < nb * nb; t += 64)
+            if (valid) C[(t / nb) * LD + (t % nb)] = mk(u[2 * t], u[2 * t + 1]);
+        __syncthreads();
+        cd acc[RB][RB];
+        gemm_lds<NBT, 0, 0>(A, C, nb, lane, acc);     // T1 = P U^T
+        __syncthreads();
+        store_block<NBT>(B, lane, acc);
+        __syncthreads();
+        gemm_lds<NBT, 1, 1>(C, B, nb, lane, acc);     // G = conj(U) T1
+        {
+            const int r0 = (lane >> 3) * RB, c0 = (lane & 7) * RB;
+#pragma unroll
+            for (int i = 0; i < RB; i++)
+#pragma unroll
+                for (int j = 0; j < RB; j++) {
+                    const int ia = r0 + i, ib = c0 + j;
+                    const double t = (ia < nb && ib < nb) ? c.T[ia * nb + ib] : 0.0;
+                    acc[i][j] = cscale(acc[i][j], t);
+                }
+        }
+        __syncthreads();
+        store_block<NBT>(A, lane, acc);               // A <- K = T o G
+        __syncthreads();
+        gemm_lds<NBT, 0, 2>(A, C, nb, lane, acc);     // T2 = K conj(U)
+        __syncthreads();
+        store_block<NBT>(B, lane, acc);
+        __syncthreads();
+        gemm_lds<NBT, 1, 1>(C, B, nb, lane, acc);     // W = conj(U) T2
+        __syncthreads();
+        store_block<NBT>(A, lane, acc);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = lane + 64 * m;
+            if (!valid || e >= NP) continue;
+            const int i = ei[m], j = ej[m];
+            cd w = A[i * LD + j];
+            cd hole = mk(0.0, 0.0);
+            for (int v = 0; v < c.ncc; v++) hole = cadd(hole, cscale(cmul(cconj(C[v * LD + j]), C[v * LD + i]), c.T[v * nb + v]));
+            if (i == j || ec[m] == 0) pv[m] = pv[m] - w.x + hole.x;
+            else pv[m] = pv[m] - w.y + hole.y;
+        }
+    }
+    if (!valid) return;
+    const int nn = c.n_nbr;
+    if (tab[6 + nn] != 0.0) {
+        double g[PER];
+#pragma unroll
+        for (int m = 0; m < PER; m++) g[m] = 0.0;
+        for (int n = 0; n < nn; n++) {
+            const size_t kb = (size_t)c.nbr[(size_t)n * c.nkp + k] * NP;
+            const double cn = tab[6 + n];
+#pragma unroll
+            for (int m = 0; m < PER; m++) {
+                const int e = lane + 64 * m;
+                if (e < NP) g[m] += cn * a.Y[kb + e];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < PER; m++) pv[m] += g[m];
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = lane + 64 * m;
+        if (e < NP) stage_update<MODE>(a, base + e, (size_t)c.nk * NP, pv[m], y[m]);
+    }
+}
+
+// ================================================================ observables
+// Values per partial: pop[nb], j1 re x3, j2 (re,im) x3, tabs (re,im) x3 -> nb + 15,
+// plus min and max of the population diagnostics.
+struct ObsArgs {
+    const double *P;     // state
+    const double *U;     // full complex AoS
+    const double *GE;    // [k][3][NP] packed Hermitian gradient of H (may be null)
+    const double *Q;     // [k][3][nb][nb] complex: H D^+ (Current2)
+    const double *DT;    // [k][3][nch][nb-nch] complex: D on the core<->non-core block
+    double *part;        // [nblocks][nv + 2] (sums with Kahan compensation collapsed)
+    int nk, nb, nch, ncc;
+    size_t sk, se;       // state strides: element e of k at k*sk + e*se
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+struct Kahan {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double v) {
+        double yv = v - c;
+        double t = s + yv;
+        c = (t - s) - yv;
+        s = t;
+    }
+};
+
+__device__ __forceinline__ cd load_p(const ObsArgs &o, int k, int i, int j) {
+    const int nb = o.nb;
+    if (i == j) return mk(o.P[k * o.sk + i * o.se], 0.0);
+    const int a = min(i, j), b = max(i, j);
+    const int p = pair_index(a, b, nb);
+    cd v = mk(o.P[k * o.sk + (nb + 2 * p) * o.se], o.P[k * o.sk + (nb + 2 * p + 1) * o.se]);
+    return (i < j) ? v : cconj(v);
+}
+
+constexpr int kObsWaves = 4;
+
+__global__ __launch_bounds__(64 * kObsWaves) void k_observables(ObsArgs o) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb = o.nb, NP = nb * nb, nch = o.nch;
+    const int gw = blockIdx.x * kObsWaves + wave, nw = gridDim.x * kObsWaves;
+    Kahan pop, j1[3], j2[6], ta[6];
+    double pmin = 1.0, pmax = 0.0;
+    for (int k = gw; k < o.nk; k += nw) {
+        // population of band c = Re (conj(U) P_sym U^T)_cc (InitParameters.h:264-335)
+        if (lane < nb) {
+            const double *u = o.U + ((size_t)k * nb + lane) * nb * 2;
+            cd s = mk(0.0, 0.0);
+            for (int ii = 0; ii < nb; ii++) {
+                cd t = mk(0.0, 0.0);
+                for (int jj = 0; jj < nb; jj++) t = cadd(t, cmul(load_p(o, k, ii, jj), mk(u[2 * jj], u[2 * jj + 1])));
+                s = cadd(s, cmul(mk(u[2 * ii], -u[2 * ii + 1]), t));
+            }
+            const double pik = s.x;
+            pop.add(pik);
+            const double hv = (lane < o.ncc) ? 1. - pik : pik;
+            pmax = fmax(pmax, fabs(hv));
+            pmin = fmin(pmin, hv);
+        }
+        // Current1: sum_ij P_ij gradH_ij (real: P and gradH are Hermitian)
+        if (o.GE) {
+            for (int e = lane; e < NP; e += 64) {
+                const double pe = o.P[k * o.sk + e * o.se];
+                const double wgt = (e < nb) ? 1.0 : 2.0;   // a pair carries (i,j) and (j,i)
+                for (int x = 0; x < 3; x++) {
+                    const double g = o.GE[((size_t)k * 3 + x) * NP + e];
+                    // Re(P_ij G_ij): re*re - im*im over the (re, im) doubles of the pair
+                    const double sgn = (e >= nb && ((e - nb) & 1)) ? -1.0 : 1.0;
+                    j1[x].add(wgt * sgn * pe * g);
+                }
+            }
+        }
+        // Current2: sum_ij P_ij (H D^+)_ij
+        for (int t = lane; t < nb * nb; t += 64) {
+            const int i = t / nb, j = t % nb;
+            const cd pij = load_p(o, k, i, j);
+            for (int x = 0; x < 3; x++) {
+                const double *q = o.Q + (((size_t)k * 3 + x) * nb * nb + t) * 2;
+                cd v = cmul(pij, mk(q[0], q[1]));
+                j2[2 * x].add(v.x);
+                j2[2 * x + 1].add(v.y);
+            }
+        }
+        // Transient absorption: -2 sum_{i<nch<=j} D_ij P_ij (Observables_MPI.h:559-569)
+        const int nblk = nch * (nb - nch);
+        for (int t = lane; t < nblk; t += 64) {
+            const int i = t / (nb - nch), j = nch + t % (nb - nch);
+            const cd pij = load_p(o, k, i, j);
+            for (int x = 0; x < 3; x++) {
+                const double *d = o.DT + (((size_t)k * 3 + x) * nblk + t) * 2;
+                cd v = cmul(mk(d[0], d[1]), pij);
+                ta[2 * x].add(-2. * v.x);
+                ta[2 * x + 1].add(-2. * v.y);
+            }
+        }
+    }
+    // Combine lanes and waves deterministically: lane sums by shuffle, waves through LDS.
+    __shared__ double red[kObsWaves][EDUS_MAX_NB + 17];
+    const int nv = nb + 15;
+    for (int c = 0; c < nb; c++) {
+        double v = (lane == c) ? pop.s : 0.0;
+        v = wave_sum(v);
+        if (lane == 0) red[wave][c] = v;
+    }
+    for (int x = 0; x < 3; x++) {
+        double v = wave_sum(j1[x].s);
+        if (lane == 0) red[wave][nb + x] = v;
+    }
+    for (int x = 0; x < 6; x++) {
+        double v = wave_sum(j2[x].s);
+        if (lane == 0) red[wave][nb + 3 + x] = v;
+        v = wave_sum(ta[x].s);
+        if (lane == 0) red[wave][nb + 9 + x] = v;
+    }
+    double mn = wave_min(pmin), mx = wave_max(pmax);
+    if (lane == 0) { red[wave][nv] = mn; red[wave][nv + 1] = mx; }
+    __syncthreads();
+    if (threadIdx.x < nv + 2) {
+        const int c = threadIdx.x;
+        double v;
+        if (c < nv) { v = 0.0; for (int w = 0; w < kObsWaves; w++) v += red[w][c]; }
+        else if (c == nv) { v = 1.0; for (int w = 0; w < kObsWaves; w++) v = fmin(v, red[w][c]); }
+        else { v = 0.0; for (int w = 0; w < kObsWaves; w++) v = fmax(v, red[w][c]); }
+        o.part[(size_t)blockIdx.x * (nv + 2) + c] = v;
+    }
+}
+
+// Sum of the block partials in a fixed order with Kahan compensation.
+__global__ void k_obs_final(const double *part, int nparts, int nv, double *out) {
+    const int c = threadIdx.x;
+    if (c >= nv + 2) return;
+    if (c < nv) {
+        double s = 0.0, comp = 0.0;
+        for (int b = 0; b < nparts; b++) {
+            double yv = part[(size_t)b * (nv + 2) + c] - comp;
+            double t = s + yv;
+            comp = (t - s) - yv;
+            s = t;
+        }
+        out[c] = s;
+    } else if (c == nv) {
+        double v = 1.0;
+        for (int b = 0; b < nparts; b++) v = fmin(v, part[(size_t)b * (nv + 2) + c]);
+        out[c] = v;
+    } else {
+        double v = 0.0;
+        for (int b = 0; b < nparts; b++) v = fmax(v, part[(size_t)b * (nv + 2) + c]);
+        out[c] = v;
+    }
+}
+
+// ================================================================ Wannier contraction
+// Out[k][m] = sum_R phase(k,R) Val[R][m], Val = [H | r_x | r_y | r_z] per R.
+// phase(k,R) = prod_l (-1)^{R_l} w_l[(i_l R_l) mod N_l] for N_l > 1 (k_l = -1/2 + i_l/N_l).
+// Tiled complex GEMM: 64 k x 64 m per block, 16x16 threads with 4x4 outputs.
+constexpr int WT = 64, WK = 8;
+
+__global__ __launch_bounds__(256) void k_wannier(const int *__restrict__ R, const cd *__restrict__ Val, const cd *__restrict__ tw,
+                                                 int N0, int N1, int N2, int off1, int off2, int nR, int nm, int nk, int kbase, cd *__restrict__ Out) {
+    __shared__ cd Ph[WT][WK + 1];
+    __shared__ cd Vs[WK][WT + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int k0 = blockIdx.y * WT, m0 = blockIdx.x * WT;
+    cd acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = mk(0.0, 0.0);
+    for (int r0 = 0; r0 < nR; r0 += WK) {
+        for (int t = threadIdx.x; t < WT * WK; t += 256) {
+            const int kk = t / WK, rr = t % WK, k = k0 + kk, r = r0 + rr;
+            cd ph = mk(0.0, 0.0);
+            if (k < nk && r < nR) {
+                const int ka = kbase + k;
+                const int i0 = ka / (N1 * N2), i1 = (ka / N2) % N1, i2 = ka % N2;
+                const int Ii[3] = {i0, i1, i2}, Nn[3] = {N0, N1, N2}, Off[3] = {0, off1, off2};
+                ph = mk(1.0, 0.0);
+                for (int l = 0; l < 3; l++) {
+                    if (Nn[l] == 1) continue;
+                    const long long Rl = R[r * 3 + l];
+                    long long q = ((long long)Ii[l] * Rl) % Nn[l];
+                    if (q < 0) q += Nn[l];
+                    cd w = tw[Off[l] + q];
+                    if (Rl & 1) w = mk(-w.x, -w.y);
+                    ph = cmul(ph, w);
+                }
+            }
+            Ph[kk][rr] = ph;
+        }
+        for (int t = threadIdx.x; t < WK * WT; t += 256) {
+            const int rr = t / WT, mm = t % WT, r = r0 + rr, m = m0 + mm;
+            Vs[rr][mm] = (r < nR && m < nm) ? Val[(size_t)r * nm + m] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < WK; rr++) {
+            cd pv[4], vv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) pv[i] = Ph[ty * 4 + i][rr];
+#pragma unroll
+            for (int j = 0; j < 4; j++) vv[j] = Vs[rr][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    acc[i][j].x += pv[i].x * vv[j].x - pv[i].y * vv[j].y;
+                    acc[i][j].y += pv[i].x * vv[j].y + pv[i].y * vv[j].x;
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = k0 + ty * 4 + i, m = m0 + tx * 4 + j;
+            if (k < nk && m < nm) Out[(size_t)k * nm + m] = acc[i][j];
+        }
+}
+
+}  // namespace
+
+// ==================================================================== context
+struct edus_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0;
+    bool soa = true;
+    double dt = 0, wx2 = 0, j_norm = 0, tabs_norm = 0, nktot = 0;
+    std::vector<double> tinv, coef;
+    std::vector<double> wb;  // [n_nbr][3]
+    // device buffers
+    double *P0 = nullptr, *ACC = nullptr, *YA = nullptr, *YB = nullptr, *HIST = nullptr;
+    double *H = nullptr, *D = nullptr, *U = nullptr, *T = nullptr, *GE = nullptr, *Q = nullptr, *DT = nullptr;
+    int *nbr = nullptr;
+    double *tab = nullptr;
+    size_t tab_cap = 0;
+    double *part = nullptr, *obs = nullptr;
+    int obs_blocks = 0;
+    int cur = 0;          // AB/Taylor: state in P0 (0) or YA (1)
+    long long nstep = 0;  // steps done (history ring position)
+    float last_ms = 0;
+    int last_launches = 0;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+template <class T>
+int dalloc(edus_ctx *c, T **p, size_t n) {
+    void *q = nullptr;
+    HIP_TRY(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    c->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return 0;
+}
+
+// Packs reference [k][nb][nb] complex (upper triangle) into the state layout.
+void pack_state(const edus_ctx *c, const double *full, std::vector<double> &out) {
+    const int nb = c->nb, np = c->np;
+    out.assign(c->soa ? (size_t)np * c->nkp : (size_t)np * c->nk, 0.0);
+    for (int k = 0; k < c->nk; k++) {
+        const double *m = full + (size_t)k * nb * nb * 2;
+        for (int i = 0; i < nb; i++) {
+            const size_t e = i;
+            out[c->soa ? e * c->nkp + k : (size_t)k * np + e] = m[(i * nb + i) * 2];
+            for (int j = i + 1; j < nb; j++) {
+                const int p = pair_index(i, j, nb);
+                const size_t er = nb + 2 * p;
+                out[c->soa ? er * c->nkp + k : (size_t)k * np + er] = m[(i * nb + j) * 2];
+                out[c->soa ? (er + 1) * c->nkp + k : (size_t)k * np + er + 1] = m[(i * nb + j) * 2 + 1];
+            }
+        }
+    }
+}
+
+void unpack_state(const edus_ctx *c, const std::vector<double> &in, double *full) {
+    const int nb = c->nb, np = c->np;
+    for (int k = 0; k < c->nk; k++) {
+        double *m = full + (size_t)k * nb * nb * 2;
+        auto at = [&](size_t e) { return in[c->soa ? e * c->nkp + k : (size_t)k * np + e]; };
+        for (int i = 0; i < nb; i++) {
+            m[(i * nb + i) * 2] = at(i);
+            m[(i * nb + i) * 2 + 1] = 0.0;
+            for (int j = i + 1; j < nb; j++) {
+                const int p = pair_index(i, j, nb);
+                const double re = at(nb + 2 * p), im = at(nb + 2 * p + 1);
+                m[(i * nb + j) * 2] = re;
+                m[(i * nb + j) * 2 + 1] = im;
+                m[(j * nb + i) * 2] = re;
+                m[(j * nb + i) * 2 + 1] = -im;
+            }
+        }
+    }
+}
+
+int upload(edus_ctx *c, double *dst, const std::vector<double> &v) {
+    HIP_TRY(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+int state_elems(const edus_ctx *c) { return c->soa ? c->np * c->nkp : c->np * c->nk; }
+
+// ---------------------------------------------------------------- launches
+template <int MODE>
+int launch_stage(edus_ctx *c, const StageArgs &a) {
+    ConstArgs ca;
+    ca.H = c->H; ca.D = c->D; ca.U = c->U; ca.T = c->T; ca.nbr = c->nbr;
+    ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
+    ca.diss = c->diss; ca.pad = 0; ca.wx2 = c->wx2;
+    if (c->soa) {
+        const int blocks = (c->nk + 255) / 256;
+        switch (c->nb) {
+            case 1: hipLaunchKernelGGL((k_stage_small<1, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+            case 2: hipLaunchKernelGGL((k_stage_small<2, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+            case 3: hipLaunchKernelGGL((k_stage_small<3, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+            case 4: hipLaunchKernelGGL((k_stage_small<4, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+            default: return set_err(EDUS_ERR_UNSUPPORTED, "no small kernel for this band count");
+        }
+    } else {
+        auto go = [&](auto tag) {
+            constexpr int NBT = decltype(tag)::value;
+            constexpr int WPB = waves_per_block<NBT>();
+            const int blocks = (c->nk + WPB - 1) / WPB;
+            const size_t lds = (size_t)WPB * 3 * NBT * (NBT + 1) * sizeof(cd);
+            hipLaunchKernelGGL((k_stage_wave<NBT, MODE>), dim3(blocks), dim3(64 * WPB), lds, c->stream, ca, a);
+        };
+        if (c->nb <= 8) go(std::integral_constant<int, 8>{});
+        else if (c->nb <= 16) go(std::integral_constant<int, 16>{});
+        else go(std::integral_constant<int, 32>{});
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Field table rows for n stages: E (6), gradient coefficients (n_nbr), flag.
+int upload_table(edus_ctx *c, const std::vector<double> &rows) {
+    if (rows.size() > c->tab_cap) {
+        if (c->tab) {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            HIP_TRY(hipFree(c->tab));
+            c->tab = nullptr;
+        }
+        size_t cap = std::max<size_t>(rows.size(), 4096);
+        HIP_TRY(hipMalloc(&c->tab, cap * sizeof(double)));
+        c->tab_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(c->tab, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+void table_row(const edus_ctx *c, const double *E, uint8_t on, double *row) {
+    for (int i = 0; i < 6; i++) row[i] = E[i];
+    for (int n = 0; n < c->n_nbr; n++)
+        row[6 + n] = E[0] * c->wb[n * 3] + E[1] * c->wb[n * 3 + 1] + E[2] * c->wb[n * 3 + 2];
+    row[6 + c->n_nbr] = on ? 1.0 : 0.0;
+}
+
+int check_ctx(edus_ctx *c) {
+    if (!c) return set_err(EDUS_ERR_INVALID, "null context");
+    hipError_t e = hipSetDevice(c->dev);
+    if (e != hipSuccess) return set_err(EDUS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+// ==================================================================== C-ABI
+extern "C" {
+
+const char *edus_last_error(void) { return g_err.c_str(); }
+
+int edus_create(const edus_problem *p, edus_ctx **out) {
+    if (!p || !out) return set_err(EDUS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const int nb = p->nb;
+    if (p->nk < 1 || nb < 1 || nb > EDUS_MAX_NB) return set_err(EDUS_ERR_INVALID, "nk/nb out of range");
+    if (p->nch < 0 || p->nch > nb || p->ncc < p->nch || p->ncc > nb) return set_err(EDUS_ERR_INVALID, "core/filled band counts out of range");
+    if (p->n_nbr < 0 || p->n_nbr > kMaxNbr) return set_err(EDUS_ERR_INVALID, "neighbour count out of range (0..64)");
+    if (!p->H || !p->D || !p->U || !p->T || !p->rho0 || (p->n_nbr > 0 && (!p->nbr || !p->nbr_wb)))
+        return set_err(EDUS_ERR_INVALID, "missing table");
+    if (p->stepper < 0 || p->stepper > 2) return set_err(EDUS_ERR_INVALID, "unknown stepper");
+    if (p->stepper != EDUS_STEPPER_RK4 && (p->order < 1 || p->order > EDUS_MAX_ORDER))
+        return set_err(EDUS_ERR_INVALID, "order out of range");
+    if (p->stepper == EDUS_STEPPER_TAYLOR && (p->order < 2 || !p->taylor_tau_inv))
+        return set_err(EDUS_ERR_INVALID, "Taylor needs order >= 2 and tau^-1");
+    // Neighbour indices must address the grid: the gradient gather trusts them.
+    for (size_t i = 0; i < (size_t)p->nk * p->n_nbr; i++)
+        if (p->nbr[i] < 0 || p->nbr[i] >= p->nk) return set_err(EDUS_ERR_INVALID, "neighbour index outside the grid");
+    // The packed operator layout needs D symmetric with a real diagonal.
+    for (int k = 0; k < p->nk; k++)
+        for (int x = 0; x < 3; x++)
+            for (int i = 0; i < nb; i++) {
+                const double *dii = p->D + (((size_t)k * nb + i) * nb + i) * 6 + 2 * x;
+                if (dii[1] != 0.0) return set_err(EDUS_ERR_UNSUPPORTED, "dipole diagonal must be real");
+                for (int j = i + 1; j < nb; j++) {
+                    const double *a = p->D + (((size_t)k * nb + i) * nb + j) * 6 + 2 * x;
+                    const double *b = p->D + (((size_t)k * nb + j) * nb + i) * 6 + 2 * x;
+                    if (a[0] != b[0] || a[1] != b[1]) return set_err(EDUS_ERR_UNSUPPORTED, "dipole table must be symmetric (D_ij == D_ji)");
+                }
+            }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(EDUS_ERR_NODEVICE, "no HIP device visible");
+
+    auto *c = new edus_ctx();
+    auto fail = [&](int rc) { edus_destroy(c); return rc; };
+    if (hipGetDevice(&c->dev) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "hipGetDevice failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "stream create failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "event create failed"));
+    c->nk = p->nk; c->nb = nb; c->np = nb * nb; c->nch = p->nch; c->ncc = p->ncc; c->n_nbr = p->n_nbr;
+    c->stepper = p->stepper; c->order = (p->stepper == EDUS_STEPPER_RK4) ? 4 : p->order;
+    c->soa = nb <= 4;
+    c->nkp = c->soa ? ((p->nk + 255) / 256) * 256 : p->nk;
+    c->dt = p->dt; c->wx2 = p->wx2; c->j_norm = p->j_norm; c->tabs_norm = p->tabs_norm; c->nktot = p->nktot;
+    c->wb.assign(p->nbr_wb, p->nbr_wb + (size_t)p->n_nbr * 3);
+    for (int i = 0; i < nb * nb; i++)
+        if (std::fabs(p->T[i]) > 1e-24) c->diss = 1;
+    if (p->stepper == EDUS_STEPPER_TAYLOR) c->tinv.assign(p->taylor_tau_inv, p->taylor_tau_inv + (size_t)(p->order - 1) * (p->order - 1));
+    // Adams-Bashforth coefficients (ODE_solver.h:152-179).
+    if (p->stepper == EDUS_STEPPER_AB) {
+        static const double ab[5][5] = {{1., 0, 0, 0, 0},
+                                        {3.0 / 2.0, -1.0 / 2.0, 0, 0, 0},
+                                        {23.0 / 12.0, -16.0 / 12.0, 5.0 / 12.0, 0, 0},
+                                        {55.0 / 24.0, -59.0 / 24.0, 37.0 / 24.0, -9.0 / 24.0, 0},
+                                        {1901.0 / 720.0, -2774.0 / 720.0, 2616.0 / 720.0, -1274.0 / 720.0, 251.0 / 720.0}};
+        if (p->order > 5) return fail(set_err(EDUS_ERR_UNSUPPORTED, "Adams-Bashforth order > 5"));
+        c->coef.assign(ab[p->order - 1], ab[p->order - 1] + p->order);
+    }
+
+    const size_t se = state_elems(c);
+    int rc = 0;
+    if ((rc = dalloc(c, &c->P0, se)) || (rc = dalloc(c, &c->ACC, se)) || (rc = dalloc(c, &c->YA, se)) || (rc = dalloc(c, &c->YB, se)))
+        return fail(rc);
+    if (c->stepper != EDUS_STEPPER_RK4) {
+        if ((rc = dalloc(c, &c->HIST, se * c->order))) return fail(rc);
+        if (hipMemsetAsync(c->HIST, 0, se * c->order * sizeof(double), c->stream) != hipSuccess)
+            return fail(set_err(EDUS_ERR_HIP, "memset history"));
+    }
+    // Operators in the state layout.
+    std::vector<double> hp;
+    pack_state(c, p->H, hp);
+    if ((rc = dalloc(c, &c->H, hp.size())) || (rc = upload(c, c->H, hp))) return fail(rc);
+    {
+        std::vector<double> dall, one, tmp((size_t)p->nk * nb * nb * 2);
+        for (int x = 0; x < 3; x++) {
+            for (size_t t = 0; t < (size_t)p->nk * nb * nb; t++) {
+                tmp[2 * t] = p->D[t * 6 + 2 * x];
+                tmp[2 * t + 1] = p->D[t * 6 + 2 * x + 1];
+            }
+            pack_state(c, tmp.data(), one);
+            dall.insert(dall.end(), one.begin(), one.end());
+        }
+        if ((rc = dalloc(c, &c->D, dall.size())) || (rc = upload(c, c->D, dall))) return fail(rc);
+        // D on the core <-> non-core block for the transient absorption.
+        const int nblk = c->nch * (nb - c->nch);
+        std::vector<double> dt((size_t)p->nk * 3 * std::max(nblk, 1) * 2, 0.0);
+        for (int k = 0; k < p->nk; k++)
+            for (int x = 0; x < 3; x++)
+                for (int t = 0; t < nblk; t++) {
+                    const int i = t / (nb - c->nch), j = c->nch + t % (nb - c->nch);
+                    const double *d = p->D + (((size_t)k * nb + i) * nb + j) * 6 + 2 * x;
+                    dt[(((size_t)k * 3 + x) * nblk + t) * 2] = d[0];
+                    dt[(((size_t)k * 3 + x) * nblk + t) * 2 + 1] = d[1];
+                }
+        if ((rc = dalloc(c, &c->DT, dt.size())) || (rc = upload(c, c->DT, dt))) return fail(rc);
+        // Q_x = H D_x^+ for Current2 (sum_kc H_ik conj(D_jk)), full complex.
+        std::vector<double> q((size_t)p->nk * 3 * nb * nb * 2);
+        for (int k = 0; k < p->nk; k++)
+            for (int x = 0; x < 3; x++)
+                for (int i = 0; i < nb; i++)
+                    for (int j = 0; j < nb; j++) {
+                        double re = 0, im = 0;
+                        for (int kc = 0; kc < nb; kc++) {
+                            const double *h = p->H + (((size_t)k * nb + i) * nb + kc) * 2;
+                            const double *d = p->D + (((size_t)k * nb + j) * nb + kc) * 6 + 2 * x;
+                            re += h[0] * d[0] + h[1] * d[1];
+                            im += h[1] * d[0] - h[0] * d[1];
+                        }
+                        q[((((size_t)k * 3 + x) * nb + i) * nb + j) * 2] = re;
+                        q[((((size_t)k * 3 + x) * nb + i) * nb + j) * 2 + 1] = im;
+                    }
+        if ((rc = dalloc(c, &c->Q, q.size())) || (rc = upload(c, c->Q, q))) return fail(rc);
+    }
+    {
+        std::vector<double> u(p->U, p->U + (size_t)p->nk * nb * nb * 2), t(p->T, p->T + (size_t)nb * nb);
+        if ((rc = dalloc(c, &c->U, u.size())) || (rc = upload(c, c->U, u))) return fail(rc);
+        if ((rc = dalloc(c, &c->T, t.size())) || (rc = upload(c, c->T, t))) return fail(rc);
+    }
+    if (p->gradH) {
+        // packed Hermitian gradient of H, [k][3][NP] (always k-major)
+        std::vector<double> ge((size_t)p->nk * 3 * c->np);
+        for (int k = 0; k < p->nk; k++)
+            for (int x = 0; x < 3; x++) {
+                double *g = &ge[((size_t)k * 3 + x) * c->np];
+                for (int i = 0; i < nb; i++) {
+                    g[i] = p->gradH[(((size_t)k * nb + i) * nb + i) * 6 + 2 * x];
+                    for (int j = i + 1; j < nb; j++) {
+                        const int pp = pair_index(i, j, nb);
+                        const double *v = p->gradH + (((size_t)k * nb + i) * nb + j) * 6 + 2 * x;
+                        g[nb + 2 * pp] = v[0];
+                        g[nb + 2 * pp + 1] = v[1];
+                    }
+                }
+            }
+        if ((rc = dalloc(c, &c->GE, ge.size())) || (rc = upload(c, c->GE, ge))) return fail(rc);
+    }
+    {
+        std::vector<int> nbr((size_t)std::max(p->n_nbr, 1) * c->nkp, 0);
+        for (int k = 0; k < p->nk; k++)
+            for (int n = 0; n < p->n_nbr; n++) nbr[(size_t)n * c->nkp + k] = p->nbr[(size_t)k * p->n_nbr + n];
+        if ((rc = dalloc(c, &c->nbr, nbr.size()))) return fail(rc);
+        if (hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return fail(set_err(EDUS_ERR_HIP, "upload nbr"));
+    }
+    c->obs_blocks = std::max(1, std::min(2048, (p->nk + kObsWaves - 1) / kObsWaves));
+    if ((rc = dalloc(c, &c->part, (size_t)c->obs_blocks * (nb + 17))) || (rc = dalloc(c, &c->obs, nb + 17))) return fail(rc);
+    {
+        std::vector<double> st;
+        pack_state(c, p->rho0, st);
+        if ((rc = upload(c, c->P0, st))) return fail(rc);
+        if (c->soa) {  // padding columns stay zero in every state buffer
+            if (hipMemsetAsync(c->ACC, 0, se * sizeof(double), c->stream) != hipSuccess ||
+                hipMemsetAsync(c->YA, 0, se * sizeof(double), c->stream) != hipSuccess ||
+                hipMemsetAsync(c->YB, 0, se * sizeof(double), c->stream) != hipSuccess)
+                return fail(set_err(EDUS_ERR_HIP, "memset state"));
+        }
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "sync after upload"));
+    *out = c;
+    g_err.clear();
+    return 0;
+}
+
+int edus_destroy(edus_ctx *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->dev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->tab) (void)hipFree(c->tab);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+static int finish_step(edus_ctx *c, int launches) {
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = launches;
+    return 0;
+}
+
+int edus_step_rk4(edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->stepper != EDUS_STEPPER_RK4) return set_err(EDUS_ERR_INVALID, "context was created for another stepper");
+    if (nsteps < 0 || (nsteps > 0 && (!fields || !grad_on))) return set_err(EDUS_ERR_INVALID, "bad step arguments");
+    const int rs = 7 + c->n_nbr;
+    std::vector<double> rows((size_t)nsteps * 4 * rs);
+    static const int evalof[4] = {0, 1, 1, 2};
+    for (int s = 0; s < nsteps; s++)
+        for (int st = 0; st < 4; st++)
+            table_row(c, fields + ((size_t)s * 3 + evalof[st]) * 6, grad_on[s], &rows[((size_t)s * 4 + st) * rs]);
+    if ((rc = upload_table(c, rows))) return rc;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    const double dt = c->dt, dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
+    for (int s = 0; s < nsteps; s++) {
+        StageArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.dt = dt;
+        a.tab = c->tab + (size_t)s * 4 * rs;
+        a.Y = c->P0; a.P0 = c->P0; a.ACC = c->ACC; a.OUT = c->YA; a.ca = dt6; a.cb = dt2;
+        if ((rc = launch_stage<RK1>(c, a))) return rc;
+        a.tab += rs; a.Y = c->YA; a.OUT = c->YB; a.ca = dt3; a.cb = dt2;
+        if ((rc = launch_stage<RK2>(c, a))) return rc;
+        a.tab += rs; a.Y = c->YB; a.OUT = c->YA; a.ca = dt3; a.cb = dt;
+        if ((rc = launch_stage<RK3>(c, a))) return rc;
+        a.tab += rs; a.Y = c->YA; a.OUT = c->P0; a.ca = dt6;
+        if ((rc = launch_stage<RK4S>(c, a))) return rc;
+    }
+    c->nstep += nsteps;
+    return finish_step(c, 4 * nsteps);
+}
+
+static int step_multistep(edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on, bool taylor) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->stepper != (taylor ? EDUS_STEPPER_TAYLOR : EDUS_STEPPER_AB)) return set_err(EDUS_ERR_INVALID, "context was created for another stepper");
+    if (nsteps < 0 || (nsteps > 0 && (!fields || !grad_on))) return set_err(EDUS_ERR_INVALID, "bad step arguments");
+    const int rs = 7 + c->n_nbr, o = c->order;
+    std::vector<double> rows((size_t)nsteps * rs);
+    for (int s = 0; s < nsteps; s++) table_row(c, fields + (size_t)s * 6, grad_on[s], &rows[(size_t)s * rs]);
+    if ((rc = upload_table(c, rows))) return rc;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < nsteps; s++) {
+        StageArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.dt = c->dt;
+        a.order = o;
+        a.tab = c->tab + (size_t)s * rs;
+        a.HIST = c->HIST;
+        // The derivative of step n lives in slot (-n-1) mod order (ODE_solver.h:66-75
+        // shifts the ring before each step); slots[s] holds f_{n-s}.
+        const long long n = c->nstep;
+        for (int q = 0; q < o; q++) a.slots[q] = (int)((((q - n - 1) % o) + o) % o);
+        a.slot_out = a.slots[0];
+        if (taylor) {
+            std::copy(c->tinv.begin(), c->tinv.end(), a.tinv);
+            for (int i = 0; i < o; i++) a.dtp[i] = std::pow(c->dt, i + 1);
+        } else {
+            std::copy(c->coef.begin(), c->coef.end(), a.coef);
+        }
+        double *in = c->cur ? c->YA : c->P0, *out = c->cur ? c->P0 : c->YA;
+        a.Y = in; a.P0 = in; a.OUT = out;
+        rc = taylor ? launch_stage<TAY>(c, a) : launch_stage<ABM>(c, a);
+        if (rc) return rc;
+        c->cur ^= 1;
+        c->nstep++;
+    }
+    return finish_step(c, nsteps);
+}
+
+int edus_step_ab(edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on) {
+    return step_multistep(c, nsteps, fields, grad_on, false);
+}
+
+int edus_step_taylor(edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on) {
+    return step_multistep(c, nsteps, fields, grad_on, true);
+}
+
+static const double *current_state(const edus_ctx *c) {
+    if (c->stepper == EDUS_STEPPER_RK4) return c->P0;
+    return c->cur ? c->YA : c->P0;
+}
+
+int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *tabs, double *pminmax) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    ObsArgs o;
+    o.P = current_state(c); o.U = c->U; o.GE = c->GE; o.Q = c->Q; o.DT = c->DT; o.part = c->part;
+    o.nk = c->nk; o.nb = c->nb; o.nch = c->nch; o.ncc = c->ncc;
+    o.sk = c->soa ? 1 : c->np;
+    o.se = c->soa ? c->nkp : 1;
+    hipLaunchKernelGGL(k_observables, dim3(c->obs_blocks), dim3(64 * kObsWaves), 0, c->stream, o);
+    HIP_TRY(hipGetLastError());
+    const int nv = c->nb + 15;
+    hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->part, c->obs_blocks, nv, c->obs);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(nv + 2);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->obs, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int nb = c->nb;
+    const double inv = 1. / c->nktot;
+    if (pop)
+        for (int i = 0; i < nb; i++) pop[i] = h[i] * inv;
+    if (j1)
+        for (int x = 0; x < 3; x++) { j1[2 * x] = h[nb + x] * c->j_norm; j1[2 * x + 1] = 0.0; }
+    if (j2)
+        for (int x = 0; x < 3; x++) { j2[2 * x] = h[nb + 3 + 2 * x] * c->j_norm; j2[2 * x + 1] = h[nb + 4 + 2 * x] * c->j_norm; }
+    if (tabs)
+        for (int x = 0; x < 3; x++) { tabs[2 * x] = h[nb + 9 + 2 * x] * c->tabs_norm; tabs[2 * x + 1] = h[nb + 10 + 2 * x] * c->tabs_norm; }
+    if (pminmax) { pminmax[0] = h[nv]; pminmax[1] = h[nv + 1]; }
+    return 0;
+}
+
+int edus_rhs(edus_ctx *c, const double *field6, uint8_t grad_on, double *pv) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!field6 || !pv) return set_err(EDUS_ERR_INVALID, "null argument");
+    std::vector<double> row(7 + c->n_nbr);
+    table_row(c, field6, grad_on, row.data());
+    if ((rc = upload_table(c, row))) return rc;
+    StageArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.tab = c->tab;
+    a.Y = current_state(c);
+    a.OUT = c->YB;
+    if ((rc = launch_stage<RHSO>(c, a))) return rc;
+    std::vector<double> st(state_elems(c));
+    HIP_TRY(hipMemcpyAsync(st.data(), c->YB, st.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unpack_state(c, st, pv);
+    return 0;
+}
+
+int edus_get_state(edus_ctx *c, double *rho) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!rho) return set_err(EDUS_ERR_INVALID, "null output");
+    std::vector<double> st(state_elems(c));
+    HIP_TRY(hipMemcpyAsync(st.data(), current_state(c), st.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unpack_state(c, st, rho);
+    return 0;
+}
+
+int edus_set_state(edus_ctx *c, const double *rho) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!rho) return set_err(EDUS_ERR_INVALID, "null input");
+    std::vector<double> st;
+    pack_state(c, rho, st);
+    HIP_TRY(hipMemcpyAsync(const_cast<double *>(current_state(c)), st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int edus_last_step_time(edus_ctx *c, double *ms, int32_t *nlaunch) {
+    if (!c) return set_err(EDUS_ERR_INVALID, "null context");
+    if (ms) *ms = c->last_ms;
+    if (nlaunch) *nlaunch = c->last_launches;
+    return 0;
+}
+
+int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR, double *H_out,
+                    double *D_out) {
+    if (!nk3 || !R || !HR || !rR || !H_out || nR < 1 || nw < 1) return set_err(EDUS_ERR_INVALID, "bad Wannier arguments");
+    for (int l = 0; l < 3; l++)
+        if (nk3[l] < 1) return set_err(EDUS_ERR_INVALID, "bad grid");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(EDUS_ERR_NODEVICE, "no HIP device visible");
+    const long long nk = (long long)nk3[0] * nk3[1] * nk3[2];
+    const int nw2 = nw * nw, nm = 4 * nw2;
+    // Val[R][c][i][j]: c = 0 H, 1..3 r_x, r_y, r_z
+    std::vector<double> val((size_t)nR * nm * 2);
+    for (int r = 0; r < nR; r++)
+        for (int t = 0; t < nw2; t++) {
+            val[((size_t)r * nm + t) * 2] = HR[((size_t)r * nw2 + t) * 2];
+            val[((size_t)r * nm + t) * 2 + 1] = HR[((size_t)r * nw2 + t) * 2 + 1];
+            for (int x = 0; x < 3; x++) {
+                val[((size_t)r * nm + (x + 1) * nw2 + t) * 2] = rR[((size_t)r * nw2 + t) * 6 + 2 * x];
+                val[((size_t)r * nm + (x + 1) * nw2 + t) * 2 + 1] = rR[((size_t)r * nw2 + t) * 6 + 2 * x + 1];
+            }
+        }
+    // twiddles e^{2 pi i m / N_l}, evaluated on the host in long double
+    std::vector<double> tw;
+    int offs[3];
+    for (int l = 0; l < 3; l++) {
+        offs[l] = (int)(tw.size() / 2);
+        for (int m = 0; m < nk3[l]; m++) {
+            long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)nk3[l];
+            tw.push_back((double)cosl(ang));
+            tw.push_back((double)sinl(ang));
+        }
+    }
+    int *dR = nullptr;
+    double *dVal = nullptr, *dTw = nullptr, *dOut = nullptr;
+    auto cleanup = [&]() { (void)hipFree(dR); (void)hipFree(dVal); (void)hipFree(dTw); (void)hipFree(dOut); };
+    const size_t chunk = std::min<long long>(nk, std::max<long long>(64, (1LL << 28) / ((long long)nm * 16)));
+    if (hipMalloc(&dR, (size_t)nR * 3 * sizeof(int)) != hipSuccess || hipMalloc(&dVal, val.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dTw, tw.size() * sizeof(double)) != hipSuccess || hipMalloc(&dOut, chunk * nm * 2 * sizeof(double)) != hipSuccess) {
+        cleanup();
+        return set_err(EDUS_ERR_NOMEM, "Wannier buffers");
+    }
+    if (hipMemcpy(dR, R, (size_t)nR * 3 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dVal, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dTw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return set_err(EDUS_ERR_HIP, "Wannier upload");
+    }
+    std::vector<double> all(chunk * nm * 2);
+    for (long long k0 = 0; k0 < nk; k0 += chunk) {
+        const int kc = (int)std::min<long long>(chunk, nk - k0);
+        dim3 grid((nm + WT - 1) / WT, (unsigned)((kc + WT - 1) / WT));
+        hipLaunchKernelGGL(k_wannier, grid, dim3(256), 0, 0, (const int *)dR, (const cd *)dVal, (const cd *)dTw, nk3[0], nk3[1], nk3[2],
+                           offs[1], offs[2], nR, nm, kc, (int)k0, (cd *)dOut);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(all.data(), dOut, (size_t)kc * nm * 16, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            cleanup();
+            return set_err(EDUS_ERR_HIP, std::string("k_wannier: ") + hipGetErrorString(e));
+        }
+        for (int kk = 0; kk < kc; kk++) {
+            const long long k = k0 + kk;
+            const double *src = &all[(size_t)kk * nm * 2];
+            double *h = H_out + (size_t)k * nw2 * 2;
+            for (int t = 0; t < nw2; t++) { h[2 * t] = src[2 * t]; h[2 * t + 1] = src[2 * t + 1]; }
+            for (int i = 0; i < nw; i++) h[(i * nw + i) * 2 + 1] = 0.0;   // Wannier.h:489-492
+            if (D_out) {
+                double *d = D_out + (size_t)k * nw2 * 6;
+                for (int i = 0; i < nw; i++)
+                    for (int j = 0; j < nw; j++)
+                        for (int x = 0; x < 3; x++) {
+                            const double *a = &src[((x + 1) * nw2 + i * nw + j) * 2];
+                            const double *b = &src[((x + 1) * nw2 + j * nw + i) * 2];
+                            const double ai = (i == j) ? 0.0 : a[1], bi = (i == j) ? 0.0 : b[1];
+                            // (D + D^T)/2 with zero diagonal imaginary parts (Wannier.h:577-578, save_HUD.cpp:187-191)
+                            d[((i * nw + j) * 3 + x) * 2] = 0.5 * (a[0] + b[0]);
+                            d[((i * nw + j) * 3 + x) * 2 + 1] = 0.5 * (ai + bi);
+                        }
+            }
+        }
+    }
+    cleanup();
+    g_err.clear();
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// Builds every table the time-stepping path reads, from an input.txt.
+//
+// Order and arithmetic follow the reference's setup: Source_Main/input.cpp
+// (parse, BZ_parameters, model init, det_a, CalculateWeigths), save_HUD.cpp
+// (H/U/D per k, scissor, Wannier dipole symmetrisation, GradientEnergy),
+// allocate_arr.cpp + InitParameters.h:203-224 (T), main_MPI.cpp:66-79
+// (nonzero-T list), runge_kutta_init.cpp + InitialPopulation.h:377-414 (rho0),
+// const_dt_evolution.cpp:164-169 (start time, last step).
+#include "host.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <omp.h>
+
+#include "../../../include/edus.h"
+
+namespace edus {
+
+static inline size_t mi(int nb, int k, int i, int j) { return ((size_t)k * nb + i) * nb + j; }
+
+void Setup::build(const std::string &input_path, int flags) {
+    in = parse_input(input_path);
+    if (in.dyn_dt) throw Error("dynamical_dt_evolution is not supported by this build (constant dt only)");
+    if (in.coulomb) throw Error("the Coulomb (Hartree-Fock) term is not supported by this build");
+    if (in.iMode != "TB" && in.iMode != "W") throw Error("iMode = " + in.iMode + " is incorrect");
+
+    L.set_from_b(in.b);
+    // BZ_parameters re-applies the polarisation of every pulse (InitParameters.h:4-11).
+    for (auto &p : in.pumps) p.set_pol(p.pol1[0], p.pol1[1], p.pol1[2], p.pol2[0], p.pol2[1], p.pol2[2]);
+    in.probe.set_pol(in.probe.pol1[0], in.probe.pol1[1], in.probe.pol1[2], in.probe.pol2[0], in.probe.pol2[1], in.probe.pol2[2]);
+
+    for (int c = 0; c < 3; c++)
+        if (in.Nk[c] < 1) throw Error("nkPT section missing or invalid");
+    kpt = make_kgrid(in.Nk);
+    nk = in.Nk[0] * in.Nk[1] * in.Nk[2];
+    nb = in.Nb[0] + in.Nb[1] + in.Nb[2];
+    nch = in.Nb[0];
+    ncc = in.Nb[0] + in.Nb[1];
+    if (nb < 1 || nb > EDUS_MAX_NB) throw Error("band count out of range (1.." + std::to_string(EDUS_MAX_NB) + ")");
+    kcart.resize((size_t)nk * 3);
+    for (int ik = 0; ik < nk; ik++) L.to_cart(&kpt[(size_t)ik * 3], &kcart[(size_t)ik * 3]);
+
+    // In-plane cell areas (input.cpp:65-73).
+    if (in.Nk[2] < in.Nk[1]) {  // sample in xy
+        det_b = std::fabs(in.b[0][0] * in.b[1][1] - in.b[0][1] * in.b[1][0]);
+        det_a = std::fabs(in.a[0][0] * in.a[1][1] - in.a[0][1] * in.a[1][0]);
+    } else {                     // sample in yz (default "011")
+        det_b = std::fabs(in.b[1][1] * in.b[2][2] - in.b[1][2] * in.b[2][1]);
+        det_a = std::fabs(in.a[1][1] * in.a[2][2] - in.a[1][2] * in.a[2][1]);
+    }
+
+    sh = find_shells(kpt, in.Nk, L);
+    nbr = neighbours_closed_form(in.Nk, sh);
+    const int nn = (int)sh.offsets.size();
+    if (flags & 1) {
+        std::vector<int32_t> ref = neighbours_bruteforce(kpt, sh, L);
+        if (ref != nbr) throw Error("closed-form neighbour table differs from the brute-force search");
+    }
+    nbr_wb.assign((size_t)nn * 3, 0.);
+    bvec_flat.assign((size_t)nn * 3, 0.);
+    {
+        int n = 0;
+        for (size_t s = 0; s < sh.bvec.size(); s++)
+            for (auto &v : sh.bvec[s]) {
+                for (int c = 0; c < 3; c++) {
+                    nbr_wb[(size_t)n * 3 + c] = sh.weights[s] * v[c];
+                    bvec_flat[(size_t)n * 3 + c] = v[c];
+                }
+                n++;
+            }
+    }
+
+    // ---- model tables (save_HUD.cpp:142-210)
+    H.assign((size_t)nk * nb * nb, 0.);
+    U.assign((size_t)nk * nb * nb, 0.);
+    D.assign((size_t)nk * nb * nb * 3, 0.);
+    if (in.iMode == "TB") {
+        TBModel tb;
+        tb.init(in.TBtype, nch, nb, in.hopping);
+#pragma omp parallel for schedule(static)
+        for (int ik = 0; ik < nk; ik++) {
+            const double *k = &kpt[(size_t)ik * 3];
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) {
+                    H[mi(nb, ik, i, j)] = tb.energy(i, j, k);
+                    U[mi(nb, ik, i, j)] = tb.unitary(i, j, k);
+                    for (int x = 0; x < 3; x++) D[mi(nb, ik, i, j) * 3 + x] = tb.dipole(i, j, x, k);
+                }
+        }
+    } else {
+        WannierModel w;
+        std::string seed = in.ref_coord;
+        std::string path = (seed.size() && seed[0] == '/') ? seed + "_tb.dat" : in.base_dir + "/" + seed + "_tb.dat";
+        w.read_tb(path);
+        if (w.nw != nb) throw Error("bands in input (" + std::to_string(nb) + ") differ from wannier90 nwannier (" + std::to_string(w.nw) + ")");
+        std::vector<double> HR((size_t)w.nR * nb * nb * 2), rR((size_t)w.nR * nb * nb * 6);
+        std::memcpy(HR.data(), w.HR.data(), HR.size() * sizeof(double));
+        std::memcpy(rR.data(), w.rR.data(), rR.size() * sizeof(double));
+        int rc = edus_wannier_hk(in.Nk, w.nR, nb, w.R.data(), HR.data(), rR.data(), reinterpret_cast<double *>(H.data()),
+                                 reinterpret_cast<double *>(D.data()));
+        if (rc != 0) throw Error(std::string("Wannier contraction on the GPU failed: ") + edus_last_error());
+        // U from the eigenvectors of H(k) (Wannier.h:496-542): rows are conj(v_a).
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int ik = 0; ik < nk; ik++) {
+            std::vector<double> ev(nb);
+            std::vector<cplx> V((size_t)nb * nb);
+            eigh_jacobi(nb, &H[mi(nb, ik, 0, 0)], ev.data(), V.data());
+            for (int a = 0; a < nb; a++)
+                for (int j = 0; j < nb; j++) U[mi(nb, ik, a, j)] = std::conj(V[(size_t)j * nb + a]);
+        }
+    }
+    // Scissor: conduction_shift on bands ic >= Nb[1] in the eigenbasis (save_HUD.cpp:147-181).
+    if (in.change_gap_constant > 1e-16) {
+#pragma omp parallel for schedule(static)
+        for (int ik = 0; ik < nk; ik++) {
+            std::vector<cplx> add((size_t)nb * nb, 0.);
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) {
+                    cplx v = 0.;
+                    for (int m = 0; m < nb; m++) {
+                        double s = (m >= in.Nb[1]) ? in.change_gap_constant : 0.0;
+                        v += std::conj(U[mi(nb, ik, m, i)]) * s * U[mi(nb, ik, m, j)];
+                    }
+                    add[(size_t)i * nb + j] = v;
+                }
+            for (int i = 0; i < nb * nb; i++) H[(size_t)ik * nb * nb + i] += add[i];
+        }
+    }
+    // GradientEnergy for Current1 (save_HUD.cpp:236-254): sum_shell,b w*b_x*(H(k+b)-H(k)).
+    if (in.iCurrent) {
+        gradH.assign((size_t)nk * nb * nb * 3, 0.);
+#pragma omp parallel for schedule(static)
+        for (int ik = 0; ik < nk; ik++)
+            for (int x = 0; x < 3; x++)
+                for (int i = 0; i < nb; i++)
+                    for (int j = 0; j < nb; j++) {
+                        cplx acc = 0.;
+                        int n = 0;
+                        for (size_t s = 0; s < sh.bvec.size(); s++)
+                            for (auto &bv : sh.bvec[s]) {
+                                int jk = nbr[(size_t)ik * nn + n];
+                                acc += sh.weights[s] * bv[x] * (H[mi(nb, jk, i, j)] - H[mi(nb, ik, i, j)]);
+                                n++;
+                            }
+                        gradH[mi(nb, ik, i, j) * 3 + x] = acc;
+                    }
+    }
+
+    // ---- decoherence matrix (InitParameters.h:203-224) and its nonzero count
+    T.assign((size_t)nb * nb, 0.);
+    for (int i = nch; i < nb; i++)
+        for (int j = nch; j < nb; j++) T[(size_t)i * nb + j] = (i == j) ? in.T1 : in.T2;
+    for (int i = 0; i < nch; i++) {
+        for (int j = 0; j < nch; j++) T[(size_t)i * nb + j] = in.Tch;
+        for (int j = nch; j < nb; j++) T[(size_t)i * nb + j] = in.Tch / 2.;
+    }
+    for (int i = 0; i < nb; i++)
+        for (int j = i + 1; j < nb; j++) T[(size_t)j * nb + i] = T[(size_t)i * nb + j];
+    n_diss = 0;
+    for (double t : T)
+        if (std::fabs(t) > 1e-24) n_diss++;
+
+    // ---- initial density matrix, stored transposed: P0[a][b] = sum_{i<ncc} U[i][a] conj(U[i][b])
+    rho0.assign((size_t)nk * nb * nb, 0.);
+#pragma omp parallel for schedule(static)
+    for (int ik = 0; ik < nk; ik++)
+        for (int a = 0; a < nb; a++)
+            for (int b = 0; b < nb; b++) {
+                cplx acc = 0.;
+                for (int i = 0; i < ncc; i++) acc += U[mi(nb, ik, i, a)] * std::conj(U[mi(nb, ik, i, b)]);
+                rho0[mi(nb, ik, a, b)] = acc;
+            }
+
+    // ---- stepper and time axis (const_dt_evolution.cpp:164-172, runge_kutta_init.cpp:356-370)
+    if (in.taylor) { stepper = EDUS_STEPPER_TAYLOR; order = in.solver_order; }
+    else if (in.adams_bashforth) { stepper = EDUS_STEPPER_AB; order = in.solver_order; }
+    else { stepper = EDUS_STEPPER_RK4; order = 4; }
+    if (stepper != EDUS_STEPPER_RK4 && (order < 1 || order > EDUS_MAX_ORDER))
+        throw Error("solver order out of range 1.." + std::to_string(EDUS_MAX_ORDER));
+    if (stepper == EDUS_STEPPER_TAYLOR) {
+        if (order < 2) throw Error("TaylorOrder must be >= 2");
+        // tau_ip = 1e7 * t_i^(p+1) with t_i = i*dt, inverted and rescaled by 1e7
+        // (Taylor_DE_solver_deriv_Polynom.cpp:100-123); constant for constant dt.
+        int n = order - 1;
+        std::vector<double> ti(n), tau((size_t)n * n);
+        ti[0] = in.dt;
+        for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + in.dt;
+        double scale = std::pow(10, 7);
+        for (int i = 0; i < n; i++)
+            for (int p = 0; p < n; p++) tau[(size_t)i * n + p] = scale * std::pow(ti[i], (p + 1));
+        tau_inv = inverse_gauss_jordan(tau, n);
+        for (double &v : tau_inv) v *= scale;
+    }
+    double t_fin = in.t_fin;
+    if (!(t_fin > 0)) {
+        // Without t_fin the run lasts 8 sigma (gaussian pump) or the pump's cycles.
+        const Pulse &p0 = in.pumps[0];
+        double tf = p0.gaussian ? 8 * p0.sigma : p0.ncycle * p0.Period;
+        int nstep = int(tf / in.dt);
+        t_fin = nstep * in.dt;
+        in.t_fin = t_fin;
+    }
+    t_start = -0.21 / Units::time_au_fs;
+    itfi = (int)((t_fin - t_start) / in.dt);
+    times.resize((size_t)itfi + 2);
+    double t = t_start;
+    for (int64_t it = 0; it <= itfi + 1; it++) { times[it] = t; t += in.dt; }
+}
+
+// Pump sum and half probe at time t (RungeKutta.cpp:24-30).
+void Setup::fields_at(double t, double ef[6]) const {
+    ef[0] = ef[1] = ef[2] = 0.;
+    for (const Pulse &p : in.pumps) {
+        double e[3];
+        p.field(t, e);
+        for (int i = 0; i < 3; i++) ef[i] += e[i];
+    }
+    double e2[3];
+    in.probe.field(t, e2);
+    for (int i = 0; i < 3; i++) ef[3 + i] = .5 * e2[i];
+}
+
+static bool field_on(const double ef[6]) {
+    double s = 0;
+    for (int i = 0; i < 6; i++) s += ef[i] * ef[i];
+    return std::sqrt(s) > 1e-20;
+}
+
+// Fields of step `it` and its gradient flag: RK4 evaluates at t, t+dt/2 and
+// t+dt and gates the gradient with the previous step's last field
+// (RungeKutta.cpp:20-30,44-61,113-128,151-162; initially true, main_MPI.cpp:274);
+// the multistep methods evaluate at t and gate with it.
+void Setup::step_fields(int64_t it, double *f, uint8_t *grad_on) const {
+    const double tl = times[it];
+    if (stepper == EDUS_STEPPER_RK4) {
+        const double dt2 = in.dt / 2.0;
+        fields_at(tl, f);
+        fields_at(tl + dt2, f + 6);
+        fields_at(tl + in.dt, f + 12);
+        if (it == 0) *grad_on = 1;
+        else {
+            double prev[6];
+            fields_at(times[it - 1] + in.dt, prev);
+            *grad_on = field_on(prev) ? 1 : 0;
+        }
+    } else {
+        fields_at(tl, f);
+        *grad_on = field_on(f) ? 1 : 0;
+    }
+}
+
+}  // namespace edus

@@ -1,0 +1,291 @@
+"""ctypes bindings of the two native libraries.
+
+libedus_host.so (include/edus_host.h): input.txt -> setup tables, time loop, writers.
+libedus_hip.so  (include/edus.h):      the GPU engine behind the C-ABI.
+
+The libraries are built in-tree by ``python -m edus_amd.build``; importing this
+module raises if they are missing -- there is no fallback implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+STEPPER_RK4, STEPPER_AB, STEPPER_TAYLOR = 0, 1, 2
+FIELDS_PER_EVAL = 6
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("nk", C.c_int32), ("nb", C.c_int32), ("nch", C.c_int32), ("ncc", C.c_int32),
+        ("n_nbr", C.c_int32), ("stepper", C.c_int32), ("order", C.c_int32), ("reserved0", C.c_int32),
+        ("nbr", _ip), ("nbr_wb", _dp), ("H", _dp), ("D", _dp), ("U", _dp), ("T", _dp),
+        ("rho0", _dp), ("gradH", _dp),
+        ("wx2", C.c_double), ("dt", C.c_double), ("j_norm", C.c_double), ("tabs_norm", C.c_double),
+        ("nktot", C.c_double), ("taylor_tau_inv", _dp),
+    ]
+
+
+class SetupInfo(C.Structure):
+    _fields_ = [
+        ("nk0", C.c_int32), ("nk1", C.c_int32), ("nk2", C.c_int32), ("nk", C.c_int32),
+        ("nb", C.c_int32), ("nch", C.c_int32), ("ncc", C.c_int32), ("nbands", C.c_int32 * 3),
+        ("n_nbr", C.c_int32), ("n_shells", C.c_int32), ("stepper", C.c_int32), ("order", C.c_int32),
+        ("it_resolution", C.c_int32), ("i_current", C.c_int32), ("i_tabs", C.c_int32), ("n_pumps", C.c_int32),
+        ("model", C.c_int32), ("n_diss", C.c_int32), ("itfi", C.c_int64),
+        ("dt", C.c_double), ("t_fin", C.c_double), ("t_start", C.c_double),
+        ("det_a", C.c_double), ("det_b", C.c_double), ("jacobian", C.c_double),
+        ("j_norm", C.c_double), ("tabs_norm", C.c_double), ("nktot", C.c_double), ("wx2", C.c_double),
+        ("T1", C.c_double), ("T2", C.c_double), ("Tch", C.c_double), ("shell_residual", C.c_double),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name == "nbands" else v
+        return d
+
+
+class Pulse(C.Structure):
+    _fields_ = [
+        ("gaussian", C.c_int32), ("sin2", C.c_int32), ("pw", C.c_int32), ("envelope", C.c_int32),
+        ("wl", C.c_double), ("period", C.c_double), ("intensity", C.c_double), ("A0", C.c_double),
+        ("E0", C.c_double), ("t0", C.c_double), ("tf", C.c_double), ("w1", C.c_double), ("w2", C.c_double),
+        ("ncycle", C.c_double), ("sigma", C.c_double), ("phase", C.c_double),
+        ("pol1", C.c_double * 3), ("pol2", C.c_double * 3),
+    ]
+
+
+def _load(name):
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m edus_amd.build`")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+hip = _load("libedus_hip.so")
+host = _load("libedus_host.so")
+
+# ---- engine
+hip.edus_create.argtypes = [C.POINTER(Problem), C.POINTER(C.c_void_p)]
+hip.edus_destroy.argtypes = [C.c_void_p]
+for _f in (hip.edus_step_rk4, hip.edus_step_ab, hip.edus_step_taylor):
+    _f.argtypes = [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_uint8)]
+hip.edus_observables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+hip.edus_rhs.argtypes = [C.c_void_p, _dp, C.c_uint8, _dp]
+hip.edus_get_state.argtypes = [C.c_void_p, _dp]
+hip.edus_set_state.argtypes = [C.c_void_p, _dp]
+hip.edus_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
+hip.edus_wannier_hk.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
+hip.edus_last_error.restype = C.c_char_p
+
+# ---- host
+host.edus_setup_from_input.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]
+host.edus_setup_free.argtypes = [C.c_void_p]
+host.edus_setup_get_info.argtypes = [C.c_void_p, C.POINTER(SetupInfo)]
+host.edus_setup_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]
+host.edus_setup_array.restype = _dp
+host.edus_setup_nbr.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+host.edus_setup_nbr.restype = _ip
+host.edus_setup_pulse.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Pulse)]
+host.edus_setup_model.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, _dp]
+host.edus_setup_eval_tb.argtypes = [C.c_void_p, _dp, C.c_int32, _dp, _dp, _dp]
+host.edus_setup_problem.argtypes = [C.c_void_p, C.POINTER(Problem)]
+host.edus_setup_fields.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _dp, C.POINTER(C.c_uint8)]
+host.edus_setup_time.argtypes = [C.c_void_p, C.c_int64]
+host.edus_setup_time.restype = C.c_double
+host.edus_run.argtypes = [C.c_void_p, C.c_char_p, C.c_int64, C.c_int32]
+host.edus_run.restype = C.c_int64
+host.edus_write_synthetic_tb.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_uint64]
+host.edus_host_last_error.restype = C.c_char_p
+
+
+class EdusError(RuntimeError):
+    pass
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check_hip(rc, what):
+    if rc != 0:
+        raise EdusError(f"{what} failed ({rc}): {hip.edus_last_error().decode()}")
+
+
+class Setup:
+    """Setup tables built by the C++ host from a reference input.txt."""
+
+    def __init__(self, input_path, verify_neighbours=False):
+        h = C.c_void_p()
+        rc = host.edus_setup_from_input(os.fsencode(input_path), 1 if verify_neighbours else 0, C.byref(h))
+        if rc != 0:
+            raise EdusError(host.edus_host_last_error().decode())
+        self._h = h
+        inf = SetupInfo()
+        host.edus_setup_get_info(h, C.byref(inf))
+        self.info = inf.as_dict()
+
+    def close(self):
+        if self._h:
+            host.edus_setup_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def array(self, name, complex_=False):
+        n = C.c_int64()
+        p = host.edus_setup_array(self._h, name.encode(), C.byref(n))
+        if not p or n.value == 0:
+            return np.zeros(0)
+        a = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+        return a.view(np.complex128) if complex_ else a
+
+    def nbr(self):
+        n = C.c_int64()
+        p = host.edus_setup_nbr(self._h, C.byref(n))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy().reshape(self.info["nk"], -1)
+
+    def pulse(self, index):
+        p = Pulse()
+        if host.edus_setup_pulse(self._h, index, C.byref(p)) != 0:
+            raise EdusError("bad pulse index")
+        return {n: (list(getattr(p, n)) if n.startswith("pol") else getattr(p, n)) for n, _ in Pulse._fields_}
+
+    def model(self):
+        """(name, params dict or None) of the setup's band model."""
+        buf = C.create_string_buffer(256)
+        par = np.zeros(8)
+        rc = host.edus_setup_model(self._h, buf, 256, _dptr(par))
+        if rc < 0:
+            raise EdusError(host.edus_host_last_error().decode())
+        if rc == 1:
+            return buf.value.decode(), None
+        keys = ["a", "g", "b", "t2", "phi0", "core", "rchv", "rchc"]
+        return buf.value.decode(), dict(zip(keys, par.tolist()))
+
+    def eval_tb(self, kcrys):
+        kcrys = np.ascontiguousarray(kcrys, dtype=np.float64).reshape(-1, 3)
+        n, nb = len(kcrys), self.info["nb"]
+        H = np.zeros((n, nb, nb), dtype=np.complex128)
+        U = np.zeros_like(H)
+        D = np.zeros((n, nb, nb, 3), dtype=np.complex128)
+        rc = host.edus_setup_eval_tb(self._h, _dptr(kcrys), n, _dptr(H.view(np.float64)), _dptr(U.view(np.float64)),
+                                     _dptr(D.view(np.float64)))
+        if rc != 0:
+            raise EdusError(host.edus_host_last_error().decode())
+        return H, U, D
+
+    def problem(self):
+        p = Problem()
+        host.edus_setup_problem(self._h, C.byref(p))
+        return p
+
+    def fields_per_step(self):
+        return 3 if self.info["stepper"] == STEPPER_RK4 else 1
+
+    def fields(self, it0, n):
+        nfe = self.fields_per_step()
+        f = np.zeros((n, nfe, FIELDS_PER_EVAL))
+        g = np.zeros(n, dtype=np.uint8)
+        rc = host.edus_setup_fields(self._h, it0, n, _dptr(f), g.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            raise EdusError(host.edus_host_last_error().decode())
+        return f, g
+
+    def time(self, it):
+        return host.edus_setup_time(self._h, it)
+
+    def run(self, outdir, max_steps=-1, verbose=False):
+        n = host.edus_run(self._h, os.fsencode(outdir), max_steps, 1 if verbose else 0)
+        if n < 0:
+            raise EdusError(host.edus_host_last_error().decode())
+        return n
+
+
+class Engine:
+    """One GPU context; the arrays passed to create() are copied to HBM."""
+
+    def __init__(self, problem, stepper=None):
+        self.stepper = problem.stepper if stepper is None else stepper
+        self.nk, self.nb = problem.nk, problem.nb
+        h = C.c_void_p()
+        _check_hip(hip.edus_create(C.byref(problem), C.byref(h)), "edus_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            hip.edus_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, fields, grad_on):
+        fields = np.ascontiguousarray(fields, dtype=np.float64)
+        grad_on = np.ascontiguousarray(grad_on, dtype=np.uint8)
+        n = len(grad_on)
+        fn = {STEPPER_RK4: hip.edus_step_rk4, STEPPER_AB: hip.edus_step_ab, STEPPER_TAYLOR: hip.edus_step_taylor}[self.stepper]
+        _check_hip(fn(self._h, n, _dptr(fields), grad_on.ctypes.data_as(C.POINTER(C.c_uint8))), "edus_step")
+
+    def observables(self):
+        pop = np.zeros(self.nb)
+        j1, j2, ta, pm = np.zeros(6), np.zeros(6), np.zeros(6), np.zeros(2)
+        _check_hip(hip.edus_observables(self._h, _dptr(pop), _dptr(j1), _dptr(j2), _dptr(ta), _dptr(pm)), "edus_observables")
+        return {"pop": pop, "j1": j1, "j2": j2, "tabs": ta, "pminmax": pm}
+
+    def rhs(self, field6, grad_on=True):
+        field6 = np.ascontiguousarray(field6, dtype=np.float64)
+        pv = np.zeros((self.nk, self.nb, self.nb), dtype=np.complex128)
+        _check_hip(hip.edus_rhs(self._h, _dptr(field6), 1 if grad_on else 0, _dptr(pv.view(np.float64))), "edus_rhs")
+        return pv
+
+    def get_state(self):
+        rho = np.zeros((self.nk, self.nb, self.nb), dtype=np.complex128)
+        _check_hip(hip.edus_get_state(self._h, _dptr(rho.view(np.float64))), "edus_get_state")
+        return rho
+
+    def set_state(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=np.complex128)
+        _check_hip(hip.edus_set_state(self._h, _dptr(rho.view(np.float64))), "edus_set_state")
+
+    def last_step_time(self):
+        ms = C.c_double()
+        n = C.c_int32()
+        hip.edus_last_step_time(self._h, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+
+def wannier_hk(nk3, R, HR, rR, nw):
+    """GPU Wannier contraction: returns H [nk][nw][nw], D [nk][nw][nw][3] (complex)."""
+    nk3 = np.ascontiguousarray(nk3, dtype=np.int32)
+    R = np.ascontiguousarray(R, dtype=np.int32)
+    HR = np.ascontiguousarray(HR, dtype=np.complex128)
+    rR = np.ascontiguousarray(rR, dtype=np.complex128)
+    nk = int(np.prod(nk3))
+    H = np.zeros((nk, nw, nw), dtype=np.complex128)
+    D = np.zeros((nk, nw, nw, 3), dtype=np.complex128)
+    rc = hip.edus_wannier_hk(nk3.ctypes.data_as(_ip), len(R), nw, R.ctypes.data_as(_ip), _dptr(HR.view(np.float64)),
+                             _dptr(rR.view(np.float64)), _dptr(H.view(np.float64)), _dptr(D.view(np.float64)))
+    _check_hip(rc, "edus_wannier_hk")
+    return H, D
+
+
+def write_synthetic_tb(path, nw, rmax, a_ang=2.5, sigma_h_ev=1.5, sigma_r_ang=0.3, decay_ang=2.0, seed=1001):
+    rc = host.edus_write_synthetic_tb(os.fsencode(path), nw, rmax, a_ang, sigma_h_ev, sigma_r_ang, decay_ang, seed)
+    if rc != 0:
+        raise EdusError(host.edus_host_last_error().decode())

@@ -1,0 +1,154 @@
+/*
+ * edus.h -- C-ABI of libedus_hip.so, the MI355X (gfx950) engine that advances the
+ * semiconductor Bloch equations of EDUS on one GPU with the whole k-grid resident
+ * in HBM.
+ *
+ * The reference has no FFI: its hot path is a set of C++ free functions called by
+ * every OpenMP thread inside one parallel region.  Each entry point below names the
+ * reference function(s) it replaces (paths relative to the reference checkout).
+ *
+ * Conventions (all entry points):
+ *   - plain pointers and sizes; complex numbers are interleaved (re, im) doubles;
+ *   - host arrays use the reference layouts: density matrix P[k][i][j] = rho_ji
+ *     ("transposed", headers/InitialPopulation.h:18-35), Hamiltonian/Unitary
+ *     [k][i][j], Dipole and GradientEnergy [k][i][j][xyz];
+ *   - one host thread per context; every call is stream-ordered on the context's
+ *     own HIP stream and returns when its results are visible to the host;
+ *   - return 0 on success, a negative EDUS_ERR_* code otherwise; the message of
+ *     the last failure on the calling thread is returned by edus_last_error().
+ *   - no exceptions cross the ABI.
+ */
+#ifndef EDUS_H
+#define EDUS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDUS_OK               0
+#define EDUS_ERR_INVALID     -1  /* bad argument, shape or table */
+#define EDUS_ERR_HIP         -2  /* HIP runtime failure (message has the hipError_t) */
+#define EDUS_ERR_NOMEM       -3  /* device allocation failed */
+#define EDUS_ERR_UNSUPPORTED -4  /* configuration this build does not implement */
+#define EDUS_ERR_NODEVICE    -5  /* no gfx950 device visible */
+
+#define EDUS_STEPPER_RK4     0
+#define EDUS_STEPPER_AB      1
+#define EDUS_STEPPER_TAYLOR  2
+
+#define EDUS_MAX_NB          32  /* largest band count with a kernel in this build */
+#define EDUS_MAX_ORDER        8  /* largest Adams-Bashforth / Taylor order */
+
+typedef struct edus_ctx edus_ctx;
+
+/*
+ * Everything the time-stepping path reads, in reference layout.  Built on the host
+ * by the setup code (headers/InitParameters.h, CalculateWeigths.h, TightBinding.h,
+ * Wannier.h, Source_Main/save_HUD.cpp, runge_kutta_init.cpp) and uploaded once.
+ * Pointers are only read during edus_create().
+ */
+typedef struct edus_problem {
+    int32_t nk;          /* k points (the whole grid, nk = Nk0*Nk1*Nk2)               */
+    int32_t nb;          /* bands Ncv = Nb[0]+Nb[1]+Nb[2]                               */
+    int32_t nch;         /* core bands Nb[0] (probe field / RWA selection)              */
+    int32_t ncc;         /* filled bands Nb[0]+Nb[1] (hole source, Losses holes)        */
+    int32_t n_nbr;       /* gradient neighbours per k, all shells (12 for 2D hexagonal) */
+    int32_t stepper;     /* EDUS_STEPPER_*                                               */
+    int32_t order;       /* Adams-Bashforth / Taylor order (ignored for RK4)             */
+    int32_t reserved0;
+    const int32_t *nbr;  /* [nk][n_nbr]  GradientIndex[ik][shell][b], shells in order    */
+    const double  *nbr_wb; /* [n_nbr][3] Weigths[shell] * Bvector[shell][b][xyz]        */
+    const double  *H;    /* [nk][nb][nb] complex  Hamiltonian                            */
+    const double  *D;    /* [nk][nb][nb][3] complex Dipole (must be symmetric D_ij=D_ji) */
+    const double  *U;    /* [nk][nb][nb] complex  Unitary (eigenbasis rows)              */
+    const double  *T;    /* [nb][nb] real decoherence rates (Initialize_T)               */
+    const double  *rho0; /* [nk][nb][nb] complex  initial P0                             */
+    const double  *gradH;/* [nk][nb][nb][3] complex GradientEnergy, or NULL (no Current) */
+    double wx2;          /* probe carrier (pulse2.wl), RWA term RungeKutta.h:209          */
+    double dt;           /* time step (a.u.)                                              */
+    double j_norm;       /* dk0*dk1*dk2/det_a        (Observables_MPI.h:698,757)          */
+    double tabs_norm;    /* |det M|*dk0*dk1*dk2       (Observables_MPI.h:568)             */
+    double nktot;        /* Nk0*Nk1*Nk2               (Losses and TAbs normalisation)     */
+    const double *taylor_tau_inv; /* [(order-1)^2] row-major tau^{-1} for EDUS_STEPPER_TAYLOR */
+} edus_problem;
+
+/* Field schedule entry of one step: for every field evaluation of the step the
+ * pump sum E_pump[xyz] followed by 0.5*E_probe[xyz] (RungeKutta.cpp:24-30).
+ * RK4 uses 3 evaluations per step (t, t+dt/2, t+dt; stage 3 reuses the second),
+ * Adams-Bashforth and Taylor use 1 (t_n).  `grad_on` is the E_non0 flag of the
+ * step (RungeKutta.cpp:151-162; Linear_Multistep_ODE_solvers.cpp:14-26). */
+#define EDUS_FIELDS_PER_EVAL 6
+
+/* Creates a context on the current HIP device and uploads the problem.
+ * Replaces: allocate_arr.cpp, the per-thread copies main_MPI.cpp:143-227,
+ * CalculateIndicesMPI/Organize_kspace (one device holds the whole grid). */
+int edus_create(const edus_problem *p, edus_ctx **out);
+
+/* Frees every device buffer of the context. */
+int edus_destroy(edus_ctx *ctx);
+
+/* Advances `nsteps` RK4 steps.  fields: [nsteps][3][EDUS_FIELDS_PER_EVAL],
+ * grad_on: [nsteps].  Replaces Source_Main/RungeKutta.cpp with
+ * get_derivative_Df (headers/RungeKutta.h:7-297), Runge_Kutta_Ad/Ac
+ * (RungeKutta.h:301-355) and Mpi_communication (MPIcommunication.h:1-20). */
+int edus_step_rk4(edus_ctx *ctx, int32_t nsteps, const double *fields, const uint8_t *grad_on);
+
+/* Advances `nsteps` Adams-Bashforth steps of the context's order.
+ * fields: [nsteps][1][EDUS_FIELDS_PER_EVAL].  Replaces
+ * Source_Main/Linear_Multistep_ODE_solvers.cpp with ODE_solver.h:66-98,152-179. */
+int edus_step_ab(edus_ctx *ctx, int32_t nsteps, const double *fields, const uint8_t *grad_on);
+
+/* Advances `nsteps` Taylor multistep steps of the context's order (constant dt,
+ * tau^{-1} given at create).  Replaces Source_Main/Taylor_DE_solver_deriv_Polynom.cpp
+ * with ODE_solver.h:3-33,107-123. */
+int edus_step_taylor(edus_ctx *ctx, int32_t nsteps, const double *fields, const uint8_t *grad_on);
+
+/* Observable reductions of the current state, FP64 with Kahan-compensated
+ * block sums.  Replaces PrintLossesMPI_exciton (Observables_MPI.h:217-484, without
+ * Coulomb), Current1 (668-718), Current2 (726-775) and CalculateTransientAbs (559-569).
+ *   pop    [nb]  sum_k Re diag(U P U^+) / nktot   (before the hole transform 1-x)
+ *   j1     [6]   (re,im) x,y,z of sum_k sum_ij P_ij gradH_ij * j_norm
+ *   j2     [6]   (re,im) x,y,z of sum_k P_ij H_ik conj(D_jk) * j_norm
+ *   tabs   [6]   (re,im) x,y,z of -2 sum_k sum_{i<nch<=j} D_ij P_ij * tabs_norm
+ *   pminmax[2]   min over k,bands of the (hole) population, max |population|
+ */
+int edus_observables(edus_ctx *ctx, double *pop, double *j1, double *j2, double *tabs, double *pminmax);
+
+/* Right-hand side dP/dt of the current state for one field evaluation
+ * (field6 = pump sum xyz, half probe xyz) and gradient flag, written hermitised
+ * in reference layout [nk][nb][nb].  Replaces get_derivative_Df
+ * (headers/RungeKutta.h:7-297) as a standalone call. */
+int edus_rhs(edus_ctx *ctx, const double *field6, uint8_t grad_on, double *pv);
+
+/* Copies the current density matrix to/from the host, reference layout [nk][nb][nb]. */
+int edus_get_state(edus_ctx *ctx, double *rho);
+int edus_set_state(edus_ctx *ctx, const double *rho);
+
+/* Device time of the last edus_step_* call's kernels, in milliseconds, measured
+ * with HIP events on the context's stream; *nlaunch receives the kernel count. */
+int edus_last_step_time(edus_ctx *ctx, double *ms, int32_t *nlaunch);
+
+/* Wannier phase contraction on the GPU: H(k) = sum_R e^{2 pi i k.R} H(R) and
+ * r(k) likewise for the three position components, for every k of a grid
+ * k = -0.5 + i/N (crystal units).  Replaces Wannier::energy and dipolex/y/z
+ * (headers/Wannier.h:278-304, 336-471) as driven by save_HUD.cpp:19-68.
+ *   nk3[3]   grid sizes (Nk0, Nk1, Nk2)
+ *   R        [nR][3] integer lattice vectors (crystal)
+ *   HR       [nR][nw][nw] complex   H_mn(R) in a.u. (already divided by ndegen)
+ *   rR       [nR][nw][nw][3] complex r_mn(R) in a.u. (already divided by ndegen)
+ *   H_out    [nk][nw][nw] complex, diagonal imaginary part zeroed (Wannier.h:489-492)
+ *   D_out    [nk][nw][nw][3] complex, (D + D^T)/2 with zero diagonal imaginary part
+ *            (Wannier.h:577-578, save_HUD.cpp:47-51); may be NULL
+ * Phases are exact: k.R = -R/2 + (i R mod N)/N is reduced in integers. */
+int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R,
+                    const double *HR, const double *rR, double *H_out, double *D_out);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *edus_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDUS_H */

@@ -1,0 +1,793 @@
+/*
+ * edus_oracle.c -- CPU restatement (FP64, C99) of the EDUS k-space time-stepping
+ * path, used ONLY as the checker of this project's GPU engine: by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  Nothing in the
+ * product links or calls it.
+ *
+ * Every function follows the loop and operation order of the reference file it
+ * cites (paths relative to the reference checkout).  Where the reference calls a
+ * library (Armadillo pinv / inv), a textbook method is used instead and named.
+ *
+ * Pinning of each function (what checks it, see tests/):
+ *   orc_field            Laser.h:56-144            EF.txt + TransientAbs.txt probe columns of the AB5 run
+ *   orc_kgrid            InitParameters.h:13-40    k_grid.txt of the AB5 run (4900 rows)
+ *   orc_shells           CalculateWeigths.h:1-138  b-vectors and residuals printed in the AB5 output.log;
+ *                                                  B1 condition sum w b_a b_b = delta_ab
+ *   orc_neighbours       CalculateWeigths.h:145-226 closed form k+b on small grids (tests), invariants
+ *   orc_tb_tables        TightBinding.h            Test_matrices.txt of the AB5 run; U H U^+ diagonal, U U^+ = 1
+ *   orc_wannier_tables   Wannier.h:278-471         brute-force DFT on tiny NR, Hermiticity, periodicity
+ *   orc_rhs / orc_run    RungeKutta.h:7-355, RungeKutta.cpp, Linear_Multistep_ODE_solvers.cpp,
+ *                        Taylor_DE_solver_deriv_Polynom.cpp, const_dt_evolution.cpp,
+ *                        Observables_MPI.h          AB5 J1/J2/Losses/TransientAbs (AB5 + core-hole decay);
+ *                                                  RK4 / Taylor / dephasing: free precession, T2 decay,
+ *                                                  trace preservation, scalar-ODE order tests
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+#define ORC_PI (4.0 * atan(1.0))
+#define ORC_MAXNB 32
+#define ORC_MAXNBR 64
+
+/* ------------------------------------------------------------------ laser
+ * Laser::E (Laser.h:128-144): envelope (gaussian cut at 40 sigma^2, sin^2,
+ * trapezoid) times sin/cos carrier on the two polarisation vectors. */
+typedef struct orc_pulse {
+    int gaussian, sin2, pw, envelope;
+    double wl, E0, t0, tf, w1, w2, sigma, phase;
+    double pol1[3], pol2[3];
+} orc_pulse;
+
+void orc_field(const orc_pulse *p, double t, double out[3]) {
+    double env;
+    if (p->gaussian) {
+        env = p->E0 * ((pow(t - p->t0, 2) < 40 * p->sigma * p->sigma) ? exp(-pow(t - p->t0, 2) / (2 * p->sigma * p->sigma)) : 0.);
+    } else if (p->sin2) {
+        env = p->E0 * ((t >= p->t0 && t <= p->tf) ? pow(sin(ORC_PI * (t - p->t0) / (p->tf - p->t0)), 2) : 0.);
+    } else if (p->pw) {
+        double tr = 0.;
+        if (t >= p->t0 && t < p->w1) tr = (t - p->t0) / (p->w1 - p->t0);
+        else if (t >= p->w1 && t < p->w2) tr = 1.;
+        else if (t >= p->w2 && t <= p->tf) tr = (t - p->tf) / (p->w2 - p->tf);
+        env = p->E0 * tr;
+    } else {
+        env = 0.;
+    }
+    double p1 = p->envelope ? 1. : sin(p->wl * (t - p->t0) + p->phase);
+    double p2 = p->envelope ? 1. : cos(p->wl * (t - p->t0) + p->phase);
+    for (int i = 0; i < 3; i++) out[i] = env * (p1 * p->pol1[i] + p2 * p->pol2[i]);
+}
+
+/* ------------------------------------------------------------------ k-grid (InitParameters.h:24-38) */
+void orc_kgrid(int n0, int n1, int n2, double *kpt) {
+    int N[3] = {n0, n1, n2};
+    long nk = (long)n0 * n1 * n2;
+    for (long ik = 0; ik < nk; ik++) {
+        long ii[3] = {(ik / (N[2] * N[1])) % N[0], (ik / N[2]) % N[1], ik % N[2]};
+        for (int c = 0; c < 3; c++) {
+            double dk = 1. / N[c];
+            kpt[ik * 3 + c] = (dk != 1.) ? -0.5 + dk * ii[c] : 0.;
+        }
+    }
+}
+
+/* crystal -> cartesian with M[i][j] = b[j][i] (Coordinate.h:68-149) */
+static void crys_to_cart(const double *b, const double *cr, double *ca) {
+    for (int i = 0; i < 3; i++) {
+        ca[i] = 0.;
+        for (int j = 0; j < 3; j++) ca[i] += b[j * 3 + i] * cr[j];
+    }
+}
+
+/* Inverse of M by Cramer's rule (the reference uses Armadillo inv). */
+static void inv_M(const double *b, double MI[3][3]) {
+    double M[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = b[j * 3 + i];
+    double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                 M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            int i1 = (j + 1) % 3, i2 = (j + 2) % 3, j1 = (i + 1) % 3, j2 = (i + 2) % 3;
+            MI[i][j] = (M[i1][j1] * M[i2][j2] - M[i1][j2] * M[i2][j1]) / det;
+        }
+}
+
+/* ------------------------------------------------------------------ shells (CalculateWeigths.h:1-138)
+ * Returns the number of shells; bvec [n_nbr][3] cartesian (shell-major),
+ * shell_size[s], weights[s] = pinv(A) q by a one-sided Jacobi SVD (the reference
+ * calls Armadillo pinv, also SVD based). */
+int orc_shells(const double *kpt, const int N[3], const double *b, double *bvec, int *shell_size, double *weights, double *residual,
+               int max_nbr) {
+    long nk = (long)N[0] * N[1] * N[2];
+    int plane = -1;
+    for (int i = 0; i < 3; i++)
+        if (N[i] == 1) plane = i;
+    double *norms = malloc(sizeof(double) * nk), *cart = malloc(sizeof(double) * nk * 3);
+    for (long ik = 0; ik < nk; ik++) {
+        crys_to_cart(b, &kpt[ik * 3], &cart[ik * 3]);
+        norms[ik] = sqrt(cart[ik * 3] * cart[ik * 3] + cart[ik * 3 + 1] * cart[ik * 3 + 1] + cart[ik * 3 + 2] * cart[ik * 3 + 2]);
+    }
+    double mins[40];
+    int nmin = 1;
+    mins[0] = 1.e-08;
+    int nshell = 0, nb_tot = 0;
+    double w[40];
+    for (;;) {
+        double m = 1000.;
+        for (long ik = 0; ik < nk; ik++) {
+            int inprev = 0;
+            for (int i = 0; i < nmin; i++) inprev = inprev || (norms[ik] <= mins[i] + 1.e-11);
+            if (!inprev && norms[ik] < m) m = norms[ik];
+        }
+        mins[nmin++] = m;
+        int cnt = 0;
+        for (long ik = 0; ik < nk; ik++)
+            if (fabs(norms[ik] - m) < 1.e-11) {
+                if (nb_tot >= max_nbr) { free(norms); free(cart); return -1; }
+                memcpy(&bvec[nb_tot * 3], &cart[ik * 3], 3 * sizeof(double));
+                nb_tot++;
+                cnt++;
+            }
+        shell_size[nshell++] = cnt;
+        double A[21][40];
+        memset(A, 0, sizeof A);
+        int off = 0;
+        for (int s = 0; s < nshell; s++) {
+            int row = 0;
+            for (int d = 0; d < 3; d++)
+                if (d != plane) {
+                    for (int q = 0; q < shell_size[s]; q++) A[row][s] += bvec[(off + q) * 3 + d] * bvec[(off + q) * 3 + d];
+                    row++;
+                }
+            for (int d = 0; d < 3; d++)
+                for (int e = d + 1; e < 3; e++)
+                    if (d != plane && e != plane) {
+                        for (int q = 0; q < shell_size[s]; q++) A[row][s] += bvec[(off + q) * 3 + d] * bvec[(off + q) * 3 + e];
+                        row++;
+                    }
+            row = 5;
+            for (int i = 0; i < 3; i++)
+                for (int j = i; j < 3; j++)
+                    for (int l = j; l < 3; l++)
+                        for (int mm = l; mm < 3; mm++) {
+                            for (int q = 0; q < shell_size[s]; q++) {
+                                const double *v = &bvec[(off + q) * 3];
+                                A[row][s] += v[i] * v[j] * v[l] * v[mm];
+                            }
+                            row++;
+                        }
+            off += shell_size[s];
+        }
+        double q[21] = {0};
+        q[0] = 1; q[1] = 1; q[2] = !(plane + 1);
+        /* w = pinv(A) q through a one-sided Jacobi (Hestenes) SVD A V = U S:
+         * columns of B = A V are rotated pairwise until orthogonal, then
+         * w = V S^-1 U^T q (singular values below 1e-16 * max dropped, as pinv(A,1e-16)). */
+        double B[21][40], V[40][40];
+        for (int r = 0; r < 21; r++)
+            for (int c = 0; c < nshell; c++) B[r][c] = A[r][c];
+        for (int i = 0; i < nshell; i++)
+            for (int j = 0; j < nshell; j++) V[i][j] = (i == j);
+        for (int sweep = 0; sweep < 60; sweep++) {
+            double off = 0;
+            for (int p = 0; p < nshell; p++)
+                for (int qq = p + 1; qq < nshell; qq++) {
+                    double al = 0, be = 0, ga = 0;
+                    for (int r = 0; r < 21; r++) { al += B[r][p] * B[r][p]; be += B[r][qq] * B[r][qq]; ga += B[r][p] * B[r][qq]; }
+                    if (ga == 0.0) continue;
+                    off = fmax(off, fabs(ga) / sqrt(al * be));
+                    double zeta = (be - al) / (2 * ga);
+                    double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+                    double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+                    for (int r = 0; r < 21; r++) { double x = B[r][p], y = B[r][qq]; B[r][p] = cs * x - sn * y; B[r][qq] = sn * x + cs * y; }
+                    for (int r = 0; r < nshell; r++) { double x = V[r][p], y = V[r][qq]; V[r][p] = cs * x - sn * y; V[r][qq] = sn * x + cs * y; }
+                }
+            if (off < 1e-15) break;
+        }
+        double sv[40], smax = 0;
+        for (int c = 0; c < nshell; c++) {
+            double n2 = 0;
+            for (int r = 0; r < 21; r++) n2 += B[r][c] * B[r][c];
+            sv[c] = sqrt(n2);
+            if (sv[c] > smax) smax = sv[c];
+        }
+        for (int i = 0; i < nshell; i++) w[i] = 0;
+        for (int c = 0; c < nshell; c++) {
+            if (sv[c] <= 1e-16 * smax || sv[c] == 0) continue;
+            double uq = 0;
+            for (int r = 0; r < 21; r++) uq += (B[r][c] / sv[c]) * q[r];
+            for (int i = 0; i < nshell; i++) w[i] += V[i][c] * uq / sv[c];
+        }
+        double r2 = 0;
+        for (int r = 0; r < 21; r++) {
+            double s = 0;
+            for (int i = 0; i < nshell; i++) s += A[r][i] * w[i];
+            r2 += (s - q[r]) * (s - q[r]);
+        }
+        *residual = sqrt(r2);
+        if (*residual < 1.e-07 || nshell >= 32) break;
+    }
+    for (int i = 0; i < nshell; i++) weights[i] = w[i];
+    free(norms);
+    free(cart);
+    return nshell;
+}
+
+/* ------------------------------------------------------------------ neighbours (CalculateWeigths.h:145-226)
+ * Brute force: k+b in cartesian, to crystal, fmod wrap, first grid point within
+ * 1e-8 modulo 1.  O(Nk^2 n_nbr) -- small grids only. nbr [nk][n_nbr]. */
+void orc_neighbours(const double *kpt, long nk, const double *b, const double *bvec, int n_nbr, int32_t *nbr) {
+    double MI[3][3];
+    inv_M(b, MI);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long ik = 0; ik < nk; ik++) {
+        double kc[3];
+        crys_to_cart(b, &kpt[ik * 3], kc);
+        for (int n = 0; n < n_nbr; n++) {
+            double pc[3] = {kc[0] + bvec[n * 3], kc[1] + bvec[n * 3 + 1], kc[2] + bvec[n * 3 + 2]}, cr[3];
+            for (int i = 0; i < 3; i++) cr[i] = MI[i][0] * pc[0] + MI[i][1] * pc[1] + MI[i][2] * pc[2];
+            for (int c = 0; c < 3; c++) cr[c] = fmod(cr[c] + 0.5, 1.) - 0.5;
+            int32_t found = -1;
+            for (long j = 0; j < nk; j++) {
+                int ok = 1;
+                for (int c = 0; c < 3 && ok; c++) {
+                    double d = cr[c] - kpt[j * 3 + c];
+                    ok = fabs(d) < 1.e-08 || fabs(d - 1.) < 1.e-08 || fabs(d + 1.) < 1.e-08;
+                }
+                if (ok) { found = (int32_t)j; break; }
+            }
+            nbr[ik * n_nbr + n] = found;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ tight binding (TightBinding.h)
+ * model: 0 BoronNitride, 1 CoreBN_Nedge, 2 CoreBN_Bedge, 3 Haldane_CoreBN_Nedge,
+ * 4 Haldane_CoreBN_Bedge, 5 gen2d_hexagonal.
+ * par: a, g, b, t2, phi0, core, rchv, rchc (a.u.) */
+typedef struct orc_tb { int model; double a, g, b, t2, phi0, core, rchv, rchc; } orc_tb;
+
+static cplx tb_energy(const orc_tb *m, int ic, int jc, const double *k) {
+    const double pi = ORC_PI;
+    if (m->model == 1 || m->model == 2) {
+        cplx f = 1. + cexp(2. * pi * I * k[1]) + cexp(2. * pi * I * k[2]);
+        if (ic == 0) return jc == 0 ? (m->model == 1 ? -15.06 : -6.91) : 0;
+        if (ic == 1) return jc == 1 ? -m->g / 2. : (jc == 2 ? m->b * conj(f) : 0.);
+        if (ic == 2) return jc == 1 ? m->b * f : (jc == 2 ? m->g / 2. : 0.);
+        return 0;
+    }
+    if (m->model == 0) {
+        if (ic == jc) return ic == 0 ? -m->g / 2 : m->g / 2;
+        cplx f = cexp(I * 2. * pi * k[1]) + 1. + cexp(I * 2. * pi * k[2]);
+        return ic == 0 ? conj(f) * m->b : f * m->b;
+    }
+    if (ic == 0) return jc == 0 ? (m->model == 3 ? -15.06 : (m->model == 4 ? -6.91 : m->core)) : 0;
+    cplx f2 = cexp(-2. * pi * I * k[2] + 2. * pi * I * k[1]) + cexp(2. * pi * I * k[2]) + cexp(-2. * pi * I * k[1]);
+    cplx f = 1. + cexp(-2. * pi * I * k[1]) + cexp(-2. * pi * I * k[2]);
+    if (ic == 1) {
+        if (jc == 1) return -m->g / 2. + 2. * creal(m->t2 * cexp(I * m->phi0) * f2);
+        if (jc == 2) return m->b * f;
+        return 0;
+    }
+    if (ic == 2) {
+        if (jc == 1) return m->b * conj(f);
+        if (jc == 2) return m->g / 2. + 2. * creal(m->t2 * cexp(-I * m->phi0) * f2);
+    }
+    return 0;
+}
+
+static cplx tb_unitary(const orc_tb *m, int ic, int jc, const double *k) {
+    const double pi = ORC_PI;
+    if (m->model == 1 || m->model == 2) {
+        if (ic == 0) return jc == 0 ? 1. : 0.;
+        if (jc == 0) return 0.;
+        cplx f = 1. + cexp(2. * pi * I * k[1]) + cexp(2. * pi * I * k[2]);
+        double E = creal(csqrt(m->g * m->g + 4. * m->b * m->b * f * conj(f))) / 2.;
+        double a = sqrt((m->g + 2. * E) / (4. * E)), h = sqrt((2. * E - m->g) / (4. * E));
+        if (ic == 1) return jc == 1 ? a : -h * conj(f) / cabs(f);
+        return jc == 1 ? h : a * conj(f) / cabs(f);
+    }
+    int o = 0;
+    if (m->model != 0) {
+        if (ic == 0) return jc == 0 ? 1. : 0.;
+        if (jc == 0) return 0.;
+        o = 1;
+    }
+    double Bx = m->b * (1. + cos(2. * pi * k[1]) + cos(2. * pi * k[2]));
+    double By = m->b * (sin(2. * pi * k[1]) + sin(2. * pi * k[2]));
+    double Bz = -m->g / 2.;
+    if (m->model != 0) Bz = -m->g / 2. - 2. * m->t2 * sin(m->phi0) * (sin(-2. * pi * k[2] + 2. * pi * k[1]) + sin(+2. * pi * k[2]) + sin(-2. * pi * k[1]));
+    double B = sqrt(Bx * Bx + By * By + Bz * Bz), phi = atan2(By, Bx) / 2.;
+    double c2 = sqrt((B + Bz) / 2. / B), s2 = sqrt((B - Bz) / 2. / B);
+    if (ic == o) return jc == o ? cexp(I * phi) * s2 : -cexp(-I * phi) * c2;
+    return jc == o ? cexp(I * phi) * c2 : cexp(-I * phi) * s2;
+}
+
+static cplx tb_dipole(const orc_tb *m, int ic, int jc, int x) {
+    if (m->model == 5) {
+        if (x == 0) {
+            if ((ic == 0 && jc == 1) || (ic == 1 && jc == 0)) return m->rchv;
+            if ((ic == 0 && jc == 2) || (ic == 2 && jc == 0)) return m->rchc;
+        }
+        if (x == 1) {
+            if (ic == 2 && jc == 2) return -m->a / sqrt(3.);
+            if (ic == 0 && jc == 0 && fabs(m->rchc) > 1.e-6) return -m->a / sqrt(3.);
+        }
+        return 0;
+    }
+    if (m->model == 1 || m->model == 3) {
+        if (x == 0 && ((ic == 0 && jc == 1) || (ic == 1 && jc == 0))) return 0.0697595094;
+        if (x == 1 && ic == 2 && jc == 2) return m->a / sqrt(3.);
+        return 0;
+    }
+    if (m->model == 2 || m->model == 4) {
+        if (x == 0 && ((ic == 0 && jc == 2) || (ic == 2 && jc == 0))) return 0.0869151662;
+        if (x == 1 && ((ic == 2 && jc == 2) || (ic == 0 && jc == 0))) return m->a / sqrt(3.);
+        return 0;
+    }
+    if (x == 1 && ic == 1 && jc == 1) return m->a / sqrt(3);
+    return 0;
+}
+
+/* H, U [nk][nb][nb] complex; D [nk][nb][nb][3] complex (interleaved doubles). */
+void orc_tb_tables(const orc_tb *m, const double *kpt, long nk, int nb, double *H, double *U, double *D) {
+    cplx *h = (cplx *)H, *u = (cplx *)U, *d = (cplx *)D;
+#pragma omp parallel for schedule(static)
+    for (long ik = 0; ik < nk; ik++)
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) {
+                h[(ik * nb + i) * nb + j] = tb_energy(m, i, j, &kpt[ik * 3]);
+                u[(ik * nb + i) * nb + j] = tb_unitary(m, i, j, &kpt[ik * 3]);
+                for (int x = 0; x < 3; x++) d[((ik * nb + i) * nb + j) * 3 + x] = tb_dipole(m, i, j, x);
+            }
+}
+
+/* ------------------------------------------------------------------ Wannier (Wannier.h:278-471, save_HUD.cpp:47-51)
+ * Direct sums with exp(i 2 pi k.R) per term; H diagonal imaginary zeroed;
+ * D = (D + D^T)/2 with zero diagonal imaginary parts. */
+void orc_wannier_tables(const double *kpt, long nk, int nR, const int32_t *R, const double *HR, const double *rR, int nw, double *H,
+                        double *D) {
+    const cplx *hr = (const cplx *)HR, *xr = (const cplx *)rR;
+    cplx *h = (cplx *)H, *d = (cplx *)D;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long ik = 0; ik < nk; ik++) {
+        cplx *dd = malloc(sizeof(cplx) * nw * nw * 3);
+        for (int i = 0; i < nw; i++)
+            for (int j = 0; j < nw; j++) {
+                cplx v = 0, vx[3] = {0, 0, 0};
+                for (int r = 0; r < nR; r++) {
+                    double kR = 0.;
+                    for (int l = 0; l < 3; l++) kR += kpt[ik * 3 + l] * R[r * 3 + l];
+                    kR *= 2 * ORC_PI;
+                    cplx ph = cexp(I * kR);
+                    v += ph * hr[((long)r * nw + i) * nw + j];
+                    for (int x = 0; x < 3; x++) vx[x] += ph * xr[(((long)r * nw + i) * nw + j) * 3 + x];
+                }
+                if (i == j) {
+                    v = creal(v);
+                    for (int x = 0; x < 3; x++) vx[x] = creal(vx[x]);
+                }
+                h[(ik * nw + i) * nw + j] = v;
+                for (int x = 0; x < 3; x++) dd[(i * nw + j) * 3 + x] = vx[x];
+            }
+        for (int i = 0; i < nw; i++)
+            for (int j = 0; j < nw; j++)
+                for (int x = 0; x < 3; x++) d[((ik * nw + i) * nw + j) * 3 + x] = 0.5 * (dd[(i * nw + j) * 3 + x] + dd[(j * nw + i) * 3 + x]);
+        free(dd);
+    }
+}
+
+/* ------------------------------------------------------------------ initial population (InitialPopulation.h:377-414)
+ * P0[ia][ib] += U[ii][ia] Pb[ii][ij] conj(U[ij][ib]) with Pb = diag(1 for ii < nbands). */
+void orc_rho0(const double *U, long nk, int nb, int nbands, double *P0) {
+    const cplx *u = (const cplx *)U;
+    cplx *p = (cplx *)P0;
+    for (long ik = 0; ik < nk; ik++)
+        for (int ia = 0; ia < nb; ia++)
+            for (int ib = 0; ib < nb; ib++) {
+                cplx acc = 0;
+                for (int ii = 0; ii < nb; ii++)
+                    for (int ij = 0; ij < nb; ij++) {
+                        cplx pb = (ii == ij && ii < nbands) ? 1. : 0.;
+                        acc += u[(ik * nb + ii) * nb + ia] * pb * conj(u[(ik * nb + ij) * nb + ib]);
+                    }
+                p[(ik * nb + ia) * nb + ib] = acc;
+            }
+}
+
+/* ------------------------------------------------------------------ right-hand side (RungeKutta.h:7-297) */
+typedef struct orc_sys {
+    long nk;
+    int nb, nch, ncc, n_nbr;
+    const cplx *H, *D, *U;     /* [nk][nb][nb], [nk][nb][nb][3], [nk][nb][nb] */
+    const double *T;           /* [nb][nb] */
+    const int32_t *nbr;        /* [nk][n_nbr] */
+    const double *wshell;      /* [n_nbr]: weight of the neighbour's shell */
+    const double *bvec;        /* [n_nbr][3] */
+    double wx2;
+    int n_diss, diss0[ORC_MAXNB * ORC_MAXNB], diss1[ORC_MAXNB * ORC_MAXNB];
+} orc_sys;
+
+/* EF[0] = pump sum, EF[1] = half probe */
+static void rhs(const orc_sys *s, const cplx *P, cplx *Pv, const double EF[2][3], int E_non0) {
+    const int Ncv = s->nb, Nch = s->nch;
+#pragma omp parallel for schedule(static)
+    for (long ik = 0; ik < s->nk; ik++) {
+        const cplx *Pk = &P[ik * Ncv * Ncv];
+        cplx *Pvk = &Pv[ik * Ncv * Ncv];
+        const cplx *Hk0 = &s->H[ik * Ncv * Ncv], *Dk = &s->D[ik * Ncv * Ncv * 3], *Uk = &s->U[ik * Ncv * Ncv];
+        cplx Hk[ORC_MAXNB][ORC_MAXNB];
+        for (int ic = 0; ic < Ncv; ic++)
+            for (int jc = ic; jc < Ncv; jc++) {
+                Hk[ic][jc] = Hk0[ic * Ncv + jc];
+                if (ic != jc) Hk[jc][ic] = conj(Hk[ic][jc]);
+            }
+#define DK(x, i, j) Dk[((i) * Ncv + (j)) * 3 + (x)]
+#define PK(i, j) Pk[(i) * Ncv + (j)]
+        cplx alpha[ORC_MAXNB];
+        /* populations */
+        for (int ic = 0; ic < Ncv; ic++) {
+            for (int j = 0; j < Ncv; j++) alpha[j] = 0.;
+            for (int j = 0; j < Ncv; j++) {
+                int w = (ic >= Nch && j < Nch) || (ic < Nch && j >= Nch);
+                alpha[j] += (Hk[ic][j] + EF[w][0] * DK(0, ic, j) + EF[w][1] * DK(1, ic, j) + EF[w][2] * DK(2, ic, j)) * PK(ic, j);
+            }
+            cplx beta = 0.;
+            for (int j = 0; j < Ncv; j++) beta += alpha[j];
+            Pvk[ic * Ncv + ic] = 2. * cimag(beta);
+        }
+        /* coherences */
+        for (int ic = 0; ic < Ncv; ic++)
+            for (int jc = ic + 1; jc < Ncv; jc++) {
+                for (int j = 0; j < Ncv; j++) alpha[j] = 0.;
+                for (int j = 0; j < Ncv; j++) {
+                    int w = (jc >= Nch && j < Nch) || (jc < Nch && j >= Nch);
+                    alpha[j] += (Hk[jc][j] + EF[w][0] * DK(0, jc, j) + EF[w][1] * DK(1, jc, j) + EF[w][2] * DK(2, jc, j)) * PK(ic, j);
+                }
+                for (int j = 0; j < Ncv; j++) {
+                    int w = (ic >= Nch && j < Nch) || (ic < Nch && j >= Nch);
+                    alpha[j] -= conj((Hk[ic][j] + EF[w][0] * DK(0, ic, j) + EF[w][1] * DK(1, ic, j) + EF[w][2] * DK(2, ic, j)) * PK(jc, j));
+                }
+                cplx beta = 0.;
+                for (int j = 0; j < Ncv; j++) beta += alpha[j];
+                Pvk[ic * Ncv + jc] = -I * beta;
+                if (ic < Nch && jc >= Nch) Pvk[ic * Ncv + jc] += -I * (-s->wx2 * PK(ic, jc));
+            }
+        /* dissipation (O(Nb^4) as written) */
+        if (s->n_diss > 0) {
+            cplx Mk[ORC_MAXNB][ORC_MAXNB];
+            memset(Mk, 0, sizeof Mk);
+            for (int i0 = 0; i0 < s->n_diss; i0++) {
+                int ii = s->diss0[i0], jj = s->diss1[i0];
+                for (int i = 0; i < Ncv; i++) {
+                    for (int j = 0; j < Ncv; j++) alpha[j] = conj(Uk[ii * Ncv + i]) * Uk[jj * Ncv + j] * PK(i, j);
+                    for (int j = 0; j < Ncv; j++) Mk[ii][jj] += alpha[j];
+                }
+            }
+            for (int ic = 0; ic < Ncv; ic++)
+                for (int jc = ic; jc < Ncv; jc++) {
+                    for (int i0 = 0; i0 < s->n_diss; i0++) {
+                        int ii = s->diss0[i0], jj = s->diss1[i0];
+                        /* hU[jc][jj] = conj(U[jj][jc]) */
+                        Pvk[ic * Ncv + jc] -= conj(Uk[jj * Ncv + jc]) * conj(Uk[ic * Ncv + ii]) * s->T[ii * Ncv + jj] * Mk[ii][jj];
+                    }
+                    cplx Tii = 0.;
+                    for (int i = 0; i < s->ncc; i++) Tii += conj(Uk[i * Ncv + jc]) * conj(conj(Uk[i * Ncv + ic])) * s->T[i * Ncv + i];
+                    Pvk[ic * Ncv + jc] += Tii;
+                }
+        }
+        /* gradient (pump field only) */
+        if (E_non0) {
+            for (int ic = 0; ic < Ncv; ic++)
+                for (int jc = ic; jc < Ncv; jc++) {
+                    cplx g = 0.;
+                    for (int n = 0; n < s->n_nbr; n++)
+                        for (int i = 0; i < 3; i++)
+                            g += s->wshell[n] * EF[0][i] * s->bvec[n * 3 + i] * P[(long)s->nbr[ik * s->n_nbr + n] * Ncv * Ncv + ic * Ncv + jc];
+                    Pvk[ic * Ncv + jc] += g;
+                }
+        }
+        /* hermitise */
+        for (int ic = 0; ic < Ncv; ic++) {
+            Pvk[ic * Ncv + ic] = creal(Pvk[ic * Ncv + ic]);
+            for (int jc = ic + 1; jc < Ncv; jc++) Pvk[jc * Ncv + ic] = conj(Pvk[ic * Ncv + jc]);
+        }
+#undef DK
+#undef PK
+    }
+}
+
+/* Public single-RHS entry (tests): P, Pv [nk][nb][nb] complex. */
+typedef struct orc_tables {
+    long nk;
+    int nb, nch, ncc, n_nbr;
+    const double *H, *D, *U, *T;
+    const int32_t *nbr;
+    const double *wshell, *bvec;
+    double wx2;
+} orc_tables;
+
+static void make_sys(const orc_tables *t, orc_sys *s) {
+    s->nk = t->nk; s->nb = t->nb; s->nch = t->nch; s->ncc = t->ncc; s->n_nbr = t->n_nbr;
+    s->H = (const cplx *)t->H; s->D = (const cplx *)t->D; s->U = (const cplx *)t->U; s->T = t->T;
+    s->nbr = t->nbr; s->wshell = t->wshell; s->bvec = t->bvec; s->wx2 = t->wx2;
+    s->n_diss = 0;
+    for (int ic = 0; ic < t->nb; ic++)
+        for (int jc = 0; jc < t->nb; jc++)
+            if (fabs(t->T[ic * t->nb + jc]) > 1e-24) { s->diss0[s->n_diss] = ic; s->diss1[s->n_diss] = jc; s->n_diss++; }
+}
+
+void orc_rhs(const orc_tables *t, const double *P, double *Pv, const double *EF6, int E_non0) {
+    orc_sys s;
+    make_sys(t, &s);
+    double EF[2][3] = {{EF6[0], EF6[1], EF6[2]}, {EF6[3], EF6[4], EF6[5]}};
+    rhs(&s, (const cplx *)P, (cplx *)Pv, EF, E_non0);
+}
+
+/* ------------------------------------------------------------------ observables */
+typedef struct orc_obs_cfg {
+    const double *gradH;    /* [nk][nb][nb][3] complex or NULL */
+    double j_norm, jacobian, dk3, nktot;
+} orc_obs_cfg;
+
+/* One output row: pop[nb] (hole-transformed as printed), j1[6], j2[3] (-2 Im),
+ * tabs[6] (divided by nktot as printed). Chunked sums in fixed order. */
+static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, double *pop, double *j1, double *j2, double *tabs) {
+    const int Ncv = s->nb;
+    const long CH = 256, nch = (s->nk + CH - 1) / CH;
+    double *ppop = calloc((size_t)nch * Ncv, sizeof(double));
+    cplx *pj1 = calloc((size_t)nch * 3, sizeof(cplx)), *pj2 = calloc((size_t)nch * 3, sizeof(cplx)), *pta = calloc((size_t)nch * 3, sizeof(cplx));
+#pragma omp parallel for schedule(dynamic)
+    for (long c = 0; c < nch; c++) {
+        for (long ik = c * CH; ik < (c + 1) * CH && ik < s->nk; ik++) {
+            const cplx *Pk = &P[ik * Ncv * Ncv], *Uk = &s->U[ik * Ncv * Ncv], *Hk = &s->H[ik * Ncv * Ncv], *Dk = &s->D[ik * Ncv * Ncv * 3];
+            /* get_P_in_dia_vect (InitParameters.h:264-335): P_sym from the upper triangle */
+            cplx Ps[ORC_MAXNB][ORC_MAXNB], Mk[ORC_MAXNB][ORC_MAXNB];
+            for (int ic = 0; ic < Ncv; ic++)
+                for (int jc = ic; jc < Ncv; jc++) {
+                    Ps[ic][jc] = Pk[ic * Ncv + jc];
+                    if (ic != jc) Ps[jc][ic] = conj(Ps[ic][jc]);
+                }
+            for (int ii = 0; ii < Ncv; ii++)
+                for (int jc = 0; jc < Ncv; jc++) {
+                    cplx acc = 0;
+                    for (int jj = 0; jj < Ncv; jj++) acc += Ps[ii][jj] * Uk[jc * Ncv + jj];
+                    Mk[jc][ii] = acc;
+                }
+            for (int ic = 0; ic < Ncv; ic++) {
+                cplx acc = 0;
+                for (int ii = 0; ii < Ncv; ii++) acc += conj(Uk[ic * Ncv + ii]) * Mk[ic][ii];
+                ppop[c * Ncv + ic] += creal(acc);
+            }
+            /* Current1 / Current2 / transient absorption */
+            for (int ic = 0; ic < Ncv; ic++)
+                for (int jc = 0; jc < Ncv; jc++) {
+                    if (oc->gradH)
+                        for (int x = 0; x < 3; x++) pj1[c * 3 + x] += Pk[ic * Ncv + jc] * ((const cplx *)oc->gradH)[((ik * Ncv + ic) * Ncv + jc) * 3 + x];
+                    for (int kc = 0; kc < Ncv; kc++)
+                        for (int x = 0; x < 3; x++) pj2[c * 3 + x] += Pk[ic * Ncv + jc] * Hk[ic * Ncv + kc] * conj(Dk[(jc * Ncv + kc) * 3 + x]);
+                }
+            for (int ic = 0; ic < s->nch; ic++)
+                for (int jc = s->nch; jc < Ncv; jc++)
+                    for (int x = 0; x < 3; x++) pta[c * 3 + x] -= 2. * Dk[(ic * Ncv + jc) * 3 + x] * Pk[ic * Ncv + jc];
+        }
+    }
+    double nktot_inv = 1. / oc->nktot;
+    for (int ic = 0; ic < Ncv; ic++) {
+        double sacc = 0;
+        for (long c = 0; c < nch; c++) sacc += ppop[c * Ncv + ic];
+        double v = sacc * nktot_inv;
+        pop[ic] = (ic < s->ncc) ? 1.0 - v : v;
+    }
+    for (int x = 0; x < 3; x++) {
+        cplx a = 0, b = 0, t = 0;
+        for (long c = 0; c < nch; c++) { a += pj1[c * 3 + x]; b += pj2[c * 3 + x]; t += pta[c * 3 + x]; }
+        a *= oc->j_norm;
+        b *= oc->j_norm;
+        t *= fabs(oc->jacobian) * oc->dk3;
+        j1[2 * x] = creal(a);
+        j1[2 * x + 1] = cimag(a);
+        j2[x] = -2. * cimag(b);
+        tabs[2 * x] = creal(t) / oc->nktot;
+        tabs[2 * x + 1] = cimag(t) / oc->nktot;
+    }
+    free(ppop); free(pj1); free(pj2); free(pta);
+}
+
+/* ------------------------------------------------------------------ time loop
+ * const_dt_evolution.cpp:164-217 with RungeKutta.cpp (stepper 0),
+ * Linear_Multistep_ODE_solvers.cpp (1) or Taylor_DE_solver_deriv_Polynom.cpp (2).
+ * Output rows (one per observation, it % it_res == 0, printed before the step):
+ *   [t_fs, EF pump x y z, pop[nb], j1 re/im x3, j2 x3, EF probe x y z, tabs re/im x3]
+ * -> 1 + 3 + nb + 6 + 3 + 3 + 6 = nb + 22 doubles.  P0 is advanced in place. */
+typedef struct orc_run_cfg {
+    int stepper, order;
+    double dt, t_start;
+    long nsteps;         /* steps it = 0..nsteps-1 */
+    int it_res;
+    int n_pumps;
+    orc_pulse pumps[8];
+    orc_pulse probe;
+    double time_au_fs;
+} orc_run_cfg;
+
+static void fields(const orc_run_cfg *r, double t, double EF[2][3]) {
+    for (int i = 0; i < 3; i++) EF[0][i] = 0.;
+    for (int p = 0; p < r->n_pumps; p++) {
+        double e[3];
+        orc_field(&r->pumps[p], t, e);
+        for (int i = 0; i < 3; i++) EF[0][i] += e[i];
+    }
+    double e2[3];
+    orc_field(&r->probe, t, e2);
+    for (int i = 0; i < 3; i++) EF[1][i] = .5 * e2[i];
+}
+
+static int field_nonzero(double EF[2][3]) {
+    double e = 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 3; b++) e += EF[a][b] * EF[a][b];
+    return sqrt(e) > 1e-20;
+}
+
+/* Inverse by LU with partial pivoting (Armadillo inv in the reference). */
+static void lu_inverse(int n, const double *A, double *Ai) {
+    double LU[64], col[8];
+    int piv[8];
+    memcpy(LU, A, sizeof(double) * n * n);
+    for (int i = 0; i < n; i++) piv[i] = i;
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        for (int r = c + 1; r < n; r++)
+            if (fabs(LU[r * n + c]) > fabs(LU[p * n + c])) p = r;
+        if (p != c) {
+            for (int j = 0; j < n; j++) { double t = LU[c * n + j]; LU[c * n + j] = LU[p * n + j]; LU[p * n + j] = t; }
+            int t = piv[c]; piv[c] = piv[p]; piv[p] = t;
+        }
+        for (int r = c + 1; r < n; r++) {
+            LU[r * n + c] /= LU[c * n + c];
+            for (int j = c + 1; j < n; j++) LU[r * n + j] -= LU[r * n + c] * LU[c * n + j];
+        }
+    }
+    for (int j = 0; j < n; j++) {
+        for (int i = 0; i < n; i++) col[i] = (piv[i] == j) ? 1.0 : 0.0;
+        for (int i = 0; i < n; i++)
+            for (int q = 0; q < i; q++) col[i] -= LU[i * n + q] * col[q];
+        for (int i = n - 1; i >= 0; i--) {
+            for (int q = i + 1; q < n; q++) col[i] -= LU[i * n + q] * col[q];
+            col[i] /= LU[i * n + i];
+        }
+        for (int i = 0; i < n; i++) Ai[i * n + j] = col[i];
+    }
+}
+
+long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, double *P0io, double *rows, long max_rows) {
+    orc_sys s;
+    make_sys(t, &s);
+    const long nk = t->nk;
+    const int nb = t->nb;
+    const size_t nel = (size_t)nk * nb * nb;
+    cplx *P0 = (cplx *)P0io;
+    cplx *P1 = calloc(nel, sizeof(cplx)), *P2 = calloc(nel, sizeof(cplx)), *Pv = calloc(nel, sizeof(cplx));
+    const int o = r->order;
+    cplx *hist = (r->stepper != 0) ? calloc(nel * o, sizeof(cplx)) : NULL;
+    int idx[16];
+    for (int i = 0; i < 16; i++) idx[i] = i;
+    double coef[5] = {0};
+    if (r->stepper == 1) {
+        /* ODE_solver.h:152-179 */
+        if (o == 1) coef[0] = 1.;
+        if (o == 2) { coef[0] = 3.0 / 2.0; coef[1] = -1.0 / 2.0; }
+        if (o == 3) { coef[0] = 23.0 / 12.0; coef[1] = -16.0 / 12.0; coef[2] = 5.0 / 12.0; }
+        if (o == 4) { coef[0] = 55.0 / 24.0; coef[1] = -59.0 / 24.0; coef[2] = 37.0 / 24.0; coef[3] = -9.0 / 24.0; }
+        if (o == 5) { coef[0] = 1901.0 / 720.0; coef[1] = -2774.0 / 720.0; coef[2] = 2616.0 / 720.0; coef[3] = -1274.0 / 720.0; coef[4] = 251.0 / 720.0; }
+    }
+    double tinv[64];
+    if (r->stepper == 2) {
+        int n = o - 1;
+        double ti[8], tau[64], scale = pow(10, 7);
+        ti[0] = r->dt;
+        for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + r->dt;
+        for (int i = 0; i < n; i++)
+            for (int p = 0; p < n; p++) tau[i * n + p] = scale * pow(ti[i], (p + 1));
+        lu_inverse(n, tau, tinv);
+        for (int i = 0; i < n * n; i++) tinv[i] *= scale;
+    }
+    const double dt = r->dt, dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
+    double EF[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int E_non0 = 1;
+    double time_loop = r->t_start;
+    long nrow = 0;
+    const int rowlen = nb + 22;
+    for (long it = 0; it < r->nsteps; it++) {
+        if (it % r->it_res == 0 && nrow < max_rows) {
+            double *row = &rows[nrow * rowlen];
+            row[0] = time_loop * r->time_au_fs;
+            for (int i = 0; i < 3; i++) row[1 + i] = EF[0][i];
+            observables(&s, oc, P0, &row[4], &row[4 + nb], &row[10 + nb], &row[16 + nb]);
+            for (int i = 0; i < 3; i++) row[13 + nb + i] = EF[1][i];
+            nrow++;
+        }
+        if (r->stepper == 0) {
+            /* RungeKutta.cpp */
+            double tRK = time_loop;
+            fields(r, tRK, EF);
+            rhs(&s, P0, Pv, EF, E_non0);
+            for (size_t e = 0; e < nel; e++) { P1[e] = P0[e] + Pv[e] * dt6; P2[e] = P0[e] + Pv[e] * dt2; }
+            tRK = time_loop + dt2;
+            fields(r, tRK, EF);
+            rhs(&s, P2, Pv, EF, E_non0);
+            for (size_t e = 0; e < nel; e++) { P1[e] += Pv[e] * dt3; P2[e] = P0[e] + Pv[e] * dt2; }
+            rhs(&s, P2, Pv, EF, E_non0);
+            for (size_t e = 0; e < nel; e++) { P1[e] += Pv[e] * dt3; P2[e] = P0[e] + Pv[e] * dt; }
+            tRK = time_loop + dt;
+            fields(r, tRK, EF);
+            rhs(&s, P2, Pv, EF, E_non0);
+            for (size_t e = 0; e < nel; e++) P0[e] = P1[e] + Pv[e] * dt6;
+            E_non0 = field_nonzero(EF);
+        } else {
+            fields(r, time_loop, EF);
+            E_non0 = field_nonzero(EF);
+            /* Multistep_index_shift (ODE_solver.h:66-75) */
+            int fill = idx[o - 1];
+            for (int q = o - 1; q > 0; q--) idx[q] = idx[q - 1];
+            idx[0] = fill;
+            rhs(&s, P0, Pv, EF, E_non0);
+            memcpy(&hist[(size_t)idx[0] * nel], Pv, nel * sizeof(cplx));
+            if (r->stepper == 1) {
+#pragma omp parallel for schedule(static)
+                for (long ik = 0; ik < nk; ik++)
+                    for (int ic = 0; ic < nb; ic++)
+                        for (int jc = ic; jc < nb; jc++) {
+                            size_t e = ((size_t)ik * nb + ic) * nb + jc;
+                            cplx d = 0.0;
+                            for (int q = 0; q < o; q++) d += coef[q] * hist[(size_t)idx[q] * nel + e];
+                            P0[e] += dt * d;
+                            if (ic != jc) P0[((size_t)ik * nb + jc) * nb + ic] = conj(P0[e]);
+                        }
+            } else {
+                const int n = o - 1;
+#pragma omp parallel for schedule(static)
+                for (long e = 0; e < (long)nel; e++) {
+                    cplx dP[8], al[8];
+                    for (int i = 0; i < n; i++) dP[i] = hist[(size_t)idx[0] * nel + e] - hist[(size_t)idx[i + 1] * nel + e];
+                    for (int i = 0; i < n; i++) {
+                        al[i] = 0;
+                        for (int p = 0; p < n; p++) al[i] += tinv[i * n + p] * dP[p];
+                    }
+                    cplx d = hist[(size_t)idx[0] * nel + e] * dt;
+                    for (int i = 1; i < o; i++) d += pow(dt, i + 1) * al[i - 1] / (double)(i + 1);
+                    P0[e] += d;
+                }
+            }
+        }
+        time_loop += dt;
+    }
+    free(P1); free(P2); free(Pv); free(hist);
+    return nrow;
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

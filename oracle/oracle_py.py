@@ -1,0 +1,214 @@
+"""ctypes binding of the CPU oracle (oracle/lib/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker (or the timed CPU
+baseline), never as the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+TIME_AU_FS = 0.02418884326505
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class OPulse(C.Structure):
+    _fields_ = [("gaussian", C.c_int), ("sin2", C.c_int), ("pw", C.c_int), ("envelope", C.c_int),
+                ("wl", C.c_double), ("E0", C.c_double), ("t0", C.c_double), ("tf", C.c_double),
+                ("w1", C.c_double), ("w2", C.c_double), ("sigma", C.c_double), ("phase", C.c_double),
+                ("pol1", C.c_double * 3), ("pol2", C.c_double * 3)]
+
+
+class OTB(C.Structure):
+    _fields_ = [("model", C.c_int), ("a", C.c_double), ("g", C.c_double), ("b", C.c_double), ("t2", C.c_double),
+                ("phi0", C.c_double), ("core", C.c_double), ("rchv", C.c_double), ("rchc", C.c_double)]
+
+
+class OTables(C.Structure):
+    _fields_ = [("nk", C.c_long), ("nb", C.c_int), ("nch", C.c_int), ("ncc", C.c_int), ("n_nbr", C.c_int),
+                ("H", _dp), ("D", _dp), ("U", _dp), ("T", _dp), ("nbr", _ip), ("wshell", _dp), ("bvec", _dp),
+                ("wx2", C.c_double)]
+
+
+class OObs(C.Structure):
+    _fields_ = [("gradH", _dp), ("j_norm", C.c_double), ("jacobian", C.c_double), ("dk3", C.c_double),
+                ("nktot", C.c_double)]
+
+
+class ORun(C.Structure):
+    _fields_ = [("stepper", C.c_int), ("order", C.c_int), ("dt", C.c_double), ("t_start", C.c_double),
+                ("nsteps", C.c_long), ("it_res", C.c_int), ("n_pumps", C.c_int), ("pumps", OPulse * 8),
+                ("probe", OPulse), ("time_au_fs", C.c_double)]
+
+
+def _load():
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    lib.orc_field.argtypes = [C.POINTER(OPulse), C.c_double, _dp]
+    lib.orc_kgrid.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+    lib.orc_shells.argtypes = [_dp, _ip, _dp, _dp, _ip, _dp, _dp, C.c_int]
+    lib.orc_shells.restype = C.c_int
+    lib.orc_neighbours.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, _ip]
+    lib.orc_tb_tables.argtypes = [C.POINTER(OTB), _dp, C.c_long, C.c_int, _dp, _dp, _dp]
+    lib.orc_wannier_tables.argtypes = [_dp, C.c_long, C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp]
+    lib.orc_rho0.argtypes = [_dp, C.c_long, C.c_int, C.c_int, _dp]
+    lib.orc_rhs.argtypes = [C.POINTER(OTables), _dp, _dp, _dp, C.c_int]
+    lib.orc_run.argtypes = [C.POINTER(OTables), C.POINTER(OObs), C.POINTER(ORun), _dp, _dp, C.c_long]
+    lib.orc_run.restype = C.c_long
+    lib.orc_set_threads.argtypes = [C.c_int]
+    lib.orc_max_threads.restype = C.c_int
+    return lib
+
+
+lib = _load()
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128).view(np.float64)
+
+
+def pulse_struct(p):
+    """OPulse from a dict with the fields of edus_amd.native.Pulse."""
+    o = OPulse()
+    o.gaussian, o.sin2, o.pw, o.envelope = p["gaussian"], p["sin2"], p["pw"], p["envelope"]
+    o.wl, o.E0, o.t0, o.tf, o.w1, o.w2, o.sigma, o.phase = (p["wl"], p["E0"], p["t0"], p["tf"], p["w1"], p["w2"],
+                                                            p["sigma"], p["phase"])
+    for i in range(3):
+        o.pol1[i] = p["pol1"][i]
+        o.pol2[i] = p["pol2"][i]
+    return o
+
+
+def field(p, t):
+    out = np.zeros(3)
+    lib.orc_field(C.byref(pulse_struct(p)), t, _d(out))
+    return out
+
+
+def kgrid(n0, n1, n2):
+    k = np.zeros((n0 * n1 * n2, 3))
+    lib.orc_kgrid(n0, n1, n2, _d(k))
+    return k
+
+
+def shells(kpt, N, b):
+    """Returns (bvec [n_nbr][3], shell sizes, weights, residual)."""
+    bvec = np.zeros((64, 3))
+    size = np.zeros(40, dtype=np.int32)
+    w = np.zeros(40)
+    res = np.zeros(1)
+    N = np.ascontiguousarray(N, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    ns = lib.orc_shells(_d(np.ascontiguousarray(kpt)), N.ctypes.data_as(_ip), _d(b), _d(bvec), size.ctypes.data_as(_ip),
+                        _d(w), _d(res), 64)
+    if ns < 0:
+        raise RuntimeError("too many shell vectors")
+    nn = int(size[:ns].sum())
+    return bvec[:nn].copy(), size[:ns].copy(), w[:ns].copy(), float(res[0])
+
+
+def neighbours(kpt, b, bvec):
+    nk = len(kpt)
+    nbr = np.zeros((nk, len(bvec)), dtype=np.int32)
+    lib.orc_neighbours(_d(np.ascontiguousarray(kpt)), nk, _d(np.ascontiguousarray(b, dtype=np.float64)),
+                       _d(np.ascontiguousarray(bvec)), len(bvec), nbr.ctypes.data_as(_ip))
+    return nbr
+
+
+TB_MODELS = {"BoronNitride": 0, "CoreBN_Nedge": 1, "CoreBN_Bedge": 2, "Haldane_CoreBN_Nedge": 3,
+             "Haldane_CoreBN_Bedge": 4, "gen2d_hexagonal": 5}
+
+
+def tb_tables(model, params, kpt, nb):
+    m = OTB()
+    m.model = TB_MODELS[model]
+    for k, v in params.items():
+        setattr(m, k, v)
+    nk = len(kpt)
+    H = np.zeros((nk, nb, nb), dtype=np.complex128)
+    U = np.zeros_like(H)
+    D = np.zeros((nk, nb, nb, 3), dtype=np.complex128)
+    lib.orc_tb_tables(C.byref(m), _d(np.ascontiguousarray(kpt)), nk, nb, _d(H.view(np.float64)), _d(U.view(np.float64)),
+                      _d(D.view(np.float64)))
+    return H, U, D
+
+
+def wannier_tables(kpt, R, HR, rR, nw):
+    nk = len(kpt)
+    H = np.zeros((nk, nw, nw), dtype=np.complex128)
+    D = np.zeros((nk, nw, nw, 3), dtype=np.complex128)
+    R = np.ascontiguousarray(R, dtype=np.int32)
+    lib.orc_wannier_tables(_d(np.ascontiguousarray(kpt)), nk, len(R), R.ctypes.data_as(_ip), _d(_c(HR)), _d(_c(rR)), nw,
+                           _d(H.view(np.float64)), _d(D.view(np.float64)))
+    return H, D
+
+
+def rho0(U, nbands):
+    nk, nb, _ = U.shape
+    P = np.zeros_like(U)
+    lib.orc_rho0(_d(_c(U)), nk, nb, nbands, _d(P.view(np.float64)))
+    return P
+
+
+class System:
+    """Tables of one problem in reference layout (complex arrays)."""
+
+    def __init__(self, H, D, U, T, nbr, shell_w, bvec, nch, ncc, wx2):
+        self.H, self.D, self.U = (np.ascontiguousarray(x, dtype=np.complex128) for x in (H, D, U))
+        self.T = np.ascontiguousarray(T, dtype=np.float64)
+        self.nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+        self.shell_w = np.ascontiguousarray(shell_w, dtype=np.float64)
+        self.bvec = np.ascontiguousarray(bvec, dtype=np.float64)
+        self.nk, self.nb = self.H.shape[0], self.H.shape[1]
+        self.t = OTables(self.nk, self.nb, nch, ncc, self.nbr.shape[1], _d(self.H.view(np.float64)),
+                         _d(self.D.view(np.float64)), _d(self.U.view(np.float64)), _d(self.T),
+                         self.nbr.ctypes.data_as(_ip), _d(self.shell_w), _d(self.bvec), wx2)
+
+    def rhs(self, P, EF6, E_non0=True):
+        P = np.ascontiguousarray(P, dtype=np.complex128)
+        Pv = np.zeros_like(P)
+        EF6 = np.ascontiguousarray(EF6, dtype=np.float64)
+        lib.orc_rhs(C.byref(self.t), _d(P.view(np.float64)), _d(Pv.view(np.float64)), _d(EF6), 1 if E_non0 else 0)
+        return Pv
+
+    def run(self, P0, stepper, order, dt, t_start, nsteps, it_res, pumps, probe, gradH=None, j_norm=0.0, jacobian=0.0,
+            dk3=0.0, nktot=1.0):
+        """Advances P0 (copied) and returns (rows [nobs][nb+22], P_final)."""
+        P = np.ascontiguousarray(P0, dtype=np.complex128).copy()
+        r = ORun()
+        r.stepper, r.order, r.dt, r.t_start, r.nsteps, r.it_res = stepper, order, dt, t_start, nsteps, it_res
+        r.n_pumps = len(pumps)
+        for i, p in enumerate(pumps):
+            r.pumps[i] = pulse_struct(p)
+        r.probe = pulse_struct(probe)
+        r.time_au_fs = TIME_AU_FS
+        gh = None if gradH is None else np.ascontiguousarray(gradH, dtype=np.complex128)
+        o = OObs(_d(gh.view(np.float64)) if gh is not None else None, j_norm, jacobian, dk3, nktot)
+        nrows = (nsteps + it_res - 1) // it_res
+        rows = np.zeros((max(nrows, 1), self.nb + 22))
+        n = lib.orc_run(C.byref(self.t), C.byref(o), C.byref(r), _d(P.view(np.float64)), _d(rows), nrows)
+        return rows[:n], P
+
+
+def set_threads(n):
+    lib.orc_set_threads(int(n))
+
+
+def max_threads():
+    return lib.orc_max_threads()

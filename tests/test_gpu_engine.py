@@ -1,0 +1,224 @@
+"""GPU parity of the engine (libedus_hip.so through its C-ABI) against the CPU
+oracle and the reference's committed AB5 run.
+
+Bars: right-hand side and states 1e-12 relative (FP64, different summation order);
+observable time series 1e-10 relative to the series' max; J2 (a sum that cancels
+~1e7-fold in these models) within 2e-15 of the sum of |terms|.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import helpers as Hh
+from helpers import N, O
+
+pytestmark = pytest.mark.gpu
+
+
+def tb_input(d, nk=24, stepper="", t1=0.0, t2=0.0, tch=0.0, probe_int=1e9, pump_int=3e12, nsteps=200, dt=0.1, res=5,
+             model="gen2d_hexagonal a= 2.50 gap= 5.62 t1= 2.92 t2= 0.1 phi0= 0.3 core= -20 chv_dipole= 0.05 chc_dipole= 0.07"):
+    t0 = -0.21 / O.TIME_AU_FS
+    t_fin = t0 + (nsteps - 0.5) * dt
+    dec = ""
+    if t1 or t2 or tch:
+        dec = "decoherence{\n" + (f"diagonal {t1} au\n" if t1 else "") + (f"offdiagonal {t2} au\n" if t2 else "") + \
+              (f"corehole {tch} au\n" if tch else "") + "}\n"
+    text = (f"NEWINPUT\ntdse{{\ntightbinding {model}\ndt {dt} au\nt_fin {t_fin!r} au\n{stepper}num_pump_pulses 1\n}}\n"
+            f"nkPT{{\n1 {nk} {nk}\n}}\n{dec}"
+            "laserpump sin2 {\ncycles 3\nwavelength 400 nm\n"
+            f"intensity {pump_int} wcm2\npolarization circular 0 1 0 0 0 1\n}}\n"
+            "laserprobe gaussian {\nsigma 0.3 fs\ndelay 0 fs\n"
+            f"intensity {probe_int}\nfrequency 15.06\npolarization 1 0 0\n}}\n"
+            f"observables{{\nit_resolution {res}\nCurrent\nTAbs\n}}\n")
+    p = os.path.join(d, "input.txt")
+    with open(p, "w") as f:
+        f.write(text)
+    return p
+
+
+def random_hermitian_state(P0, scale, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=P0.shape) + 1j * rng.normal(size=P0.shape)
+    X = 0.5 * (X + np.conj(np.swapaxes(X, 1, 2)))
+    return P0 + scale * X
+
+
+def check_series(gpu, ora, nb, what=""):
+    g, o = Hh.split_rows(gpu, nb), Hh.split_rows(ora, nb)
+    np.testing.assert_allclose(g["t"], o["t"], rtol=0, atol=0)
+    for key in ("EF", "EFprobe"):
+        np.testing.assert_array_equal(g[key], o[key])
+    for c in range(6):
+        assert Hh.rel_err(g["J1"][:, c], o["J1"][:, c], 1e-300) <= 1e-10, (what, "J1", c)
+        assert Hh.rel_err(g["TAbs"][:, c], o["TAbs"][:, c], 1e-300) <= 1e-10, (what, "TAbs", c)
+    for c in range(nb):
+        assert np.max(np.abs(g["pop"][:, c] - o["pop"][:, c])) <= max(1e-10 * np.max(np.abs(o["pop"][:, c])), 1e-14), (what, "pop", c)
+    return g, o
+
+
+def run_engine_rows(setup, nsteps, res=None):
+    """The engine through the C-ABI with the host's field schedule, rows in the oracle's layout."""
+    info = setup.info
+    res = res or info["it_resolution"]
+    eng = N.Engine(setup.problem())
+    nb = info["nb"]
+    rows = []
+    ef_last = np.zeros(6)
+    it = 0
+    while it < nsteps:
+        if it % res == 0:
+            ob = eng.observables()
+            pop = ob["pop"].copy()
+            pop[:info["ncc"]] = 1.0 - pop[:info["ncc"]]
+            row = [setup.time(it) * O.TIME_AU_FS, *ef_last[:3], *pop, *ob["j1"], *(-2 * ob["j2"][1::2]), *ef_last[3:],
+                   *(ob["tabs"] / info["nktot"])]
+            rows.append(row)
+        n = min((it // res + 1) * res, nsteps) - it
+        f, g = setup.fields(it, n)
+        eng.step(f, g)
+        ef_last = f[-1, -1].copy()
+        it += n
+    return np.array(rows), eng
+
+
+@pytest.mark.parametrize("diss", [(0.0, 0.0, 0.0), (0.004, 0.01, 0.02)])
+def test_rhs_small_kernel_matches_oracle(tmp_path, diss):
+    s = N.Setup(tb_input(str(tmp_path), nk=16, t1=diss[0], t2=diss[1], tch=diss[2]))
+    osys = Hh.oracle_system(s)
+    eng = N.Engine(s.problem())
+    P = random_hermitian_state(s.array("rho0", True).reshape(-1, 3, 3), 0.05, 1)
+    eng.set_state(P)
+    np.testing.assert_array_equal(eng.get_state(), P)
+    for field6, on in (([1e-3, 2e-3, -3e-3, 4e-4, 0, 0], True), ([0, 0, 0, 0, 0, 0], False), ([5e-3, 0, 1e-3, 0, 2e-4, 1e-4], True)):
+        pv = eng.rhs(np.array(field6), on)
+        ref = osys["sys"].rhs(P, np.array(field6), on)
+        scale = np.max(np.abs(ref))
+        assert np.max(np.abs(pv - ref)) <= 1e-12 * scale
+        np.testing.assert_array_equal(pv, np.conj(np.swapaxes(pv, 1, 2)))
+
+
+def _wannier_setup(d, nw, nk=12, rmax=2, t2=0.0, nval=None):
+    from edus_amd import workloads
+    p = workloads.wannier_synthetic(d, nw=nw, rmax=rmax, nk=nk, nsteps=60, dt=0.05, nvalence=nval or max(1, nw // 4),
+                                    seed=7 + nw, t2_fs=t2, e_v_ang=0.5, it_resolution=5)
+    return N.Setup(p)
+
+
+def _read_tb(path, nw):
+    """wannier90 _tb.dat -> R, H(R) (a.u.), r(R) (a.u.) -- independent numpy reader."""
+    toks = open(path).read().split("\n")
+    nw_f = int(toks[4])
+    nR = int(toks[5])
+    vals = " ".join(toks[6:]).split()
+    deg = np.array(vals[:nR], dtype=float)
+    pos = nR
+    R = np.zeros((nR, 3), dtype=np.int32)
+    HR = np.zeros((nR, nw, nw), dtype=np.complex128)
+    rR = np.zeros((nR, nw, nw, 3), dtype=np.complex128)
+    ev, ang = 1 / 27.211396, 1 / 0.52917721092
+    for r in range(nR):
+        R[r] = [int(float(x)) for x in vals[pos:pos + 3]]
+        pos += 3
+        for n in range(nw):
+            for m in range(nw):
+                HR[r, m, n] = (float(vals[pos + 2]) + 1j * float(vals[pos + 3])) / deg[r] * ev
+                pos += 4
+    for r in range(nR):
+        pos += 3
+        for n in range(nw):
+            for m in range(nw):
+                v = [float(x) for x in vals[pos + 2:pos + 8]]
+                rR[r, m, n] = (np.array(v[0::2]) + 1j * np.array(v[1::2])) / deg[r] * ang
+                pos += 8
+    assert nw_f == nw
+    return R, HR, rR
+
+
+@pytest.mark.parametrize("nw", [5, 8, 12, 16])
+def test_wannier_contraction_matches_direct_sum(tmp_path, nw):
+    s = _wannier_setup(str(tmp_path), nw)
+    seed = [f for f in os.listdir(tmp_path) if f.endswith("_tb.dat")][0]
+    R, HR, rR = _read_tb(os.path.join(tmp_path, seed), nw)
+    kpt = O.kgrid(1, 12, 12)
+    oH, oD = O.wannier_tables(kpt, R, HR, rR, nw)
+    H = s.array("H", True).reshape(oH.shape)
+    D = s.array("D", True).reshape(oD.shape)
+    assert np.max(np.abs(H - oH)) <= 1e-13 * np.max(np.abs(oH))
+    assert np.max(np.abs(D - oD)) <= 1e-13 * np.max(np.abs(oD))
+    # U diagonalises H: U H U^+ = diag(ascending)
+    U = s.array("U", True).reshape(oH.shape)
+    E = np.einsum("kai,kij,kbj->kab", U, H, np.conj(U))
+    off = E - np.einsum("kaa->ka", E)[:, :, None] * np.eye(nw)
+    assert np.max(np.abs(off)) <= 1e-12 * np.max(np.abs(H))
+    ev = np.real(np.einsum("kaa->ka", E))
+    assert np.all(np.diff(ev, axis=1) >= -1e-12)
+
+
+@pytest.mark.parametrize("nw,t2", [(5, 0.0), (8, 10.0), (12, 0.0), (16, 0.0), (16, 10.0)])
+def test_rhs_wave_kernel_matches_oracle(tmp_path, nw, t2):
+    s = _wannier_setup(str(tmp_path), nw, t2=t2)
+    osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
+    eng = N.Engine(s.problem())
+    P = random_hermitian_state(s.array("rho0", True).reshape(-1, nw, nw), 0.05, nw)
+    eng.set_state(P)
+    for field6, on in (([1e-3, 2e-3, -3e-3, 0, 0, 0], True), ([0, 0, 0, 0, 0, 0], False)):
+        pv = eng.rhs(np.array(field6), on)
+        ref = osys["sys"].rhs(P, np.array(field6), on)
+        assert np.max(np.abs(pv - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("stepper,order", [("RK4", 4), ("AB", 3), ("AB", 5), ("Taylor", 4), ("Taylor", 6)])
+def test_trajectory_small_tb_matches_oracle(tmp_path, stepper, order):
+    st = {"RK4": "", "AB": f"Adams_Bashforth {order}\n", "Taylor": f"Taylor\nTaylorOrder {order}\n"}[stepper]
+    s = N.Setup(tb_input(str(tmp_path), nk=20, stepper=st, t1=0.002, t2=0.01, tch=0.004, nsteps=300, dt=0.1))
+    osys = Hh.oracle_system(s)
+    n = s.info["itfi"] + 1
+    ora, Pora = Hh.oracle_run(s, osys, n)
+    gpu, eng = run_engine_rows(s, n)
+    check_series(gpu, ora, 3, stepper)
+    Pg = eng.get_state()
+    assert np.max(np.abs(Pg - Pora)) <= 1e-11 * np.max(np.abs(Pora))
+
+
+@pytest.mark.parametrize("nw,stepper", [(8, "RK4"), (16, "RK4"), (16, "Taylor")])
+def test_trajectory_wannier_matches_oracle(tmp_path, nw, stepper):
+    from edus_amd import workloads
+    p = workloads.wannier_synthetic(str(tmp_path), nw=nw, rmax=2, nk=12, nsteps=80, dt=0.05, stepper=stepper,
+                                    order=4, nvalence=4, seed=11, e_v_ang=0.5, it_resolution=4)
+    s = N.Setup(p)
+    osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
+    n = s.info["itfi"] + 1
+    ora, Pora = Hh.oracle_run(s, osys, n)
+    gpu, eng = run_engine_rows(s, n)
+    check_series(gpu, ora, nw, f"wannier{nw}")
+    assert np.max(np.abs(eng.get_state() - Pora)) <= 1e-11 * np.max(np.abs(Pora))
+
+
+def test_ab5_run_reproduces_reference_outputs(tmp_path):
+    """The drop-in writes the reference's Output files for the reference's own input."""
+    s = N.Setup(os.path.join(Hh.AB5, "input.txt"))
+    n = s.run(str(tmp_path))
+    assert n == 16591
+    out = os.path.join(str(tmp_path), "Output")
+    fx = Hh.ab5_fixture()
+    mine = {k: np.loadtxt(os.path.join(out, k + ".txt")) for k in fx}
+    for k in fx:
+        assert mine[k].shape == fx[k].shape, k
+    np.testing.assert_allclose(mine["Losses"][:, 0], fx["Losses"][:, 0], rtol=0, atol=1e-13)
+    for c in range(1, 7):
+        assert Hh.rel_err(mine["J1"][:, c], fx["J1"][:, c]) <= 1e-10
+    assert np.max(np.abs(mine["J2"][:, 1:] - fx["J2"][:, 1:])) <= 5e-16
+    assert np.max(np.abs(mine["Losses"][:, 1] - fx["Losses"][:, 1])) <= 1e-13
+    for c in (2, 3):
+        assert Hh.rel_err(mine["Losses"][:, c], fx["Losses"][:, c]) <= 1e-10
+    assert Hh.rel_err(mine["EF"][:, 1:], fx["EF"][:, 1:]) <= 5e-8
+    for c in range(1, 10):
+        assert Hh.rel_err(mine["TransientAbs"][:, c], fx["TransientAbs"][:, c], 1e-300) <= 5e-8
+    # stream formats: fixed time on the first J1 row, scientific afterwards
+    j1 = open(os.path.join(out, "J1.txt")).read().split("\n")
+    ref = open(os.path.join(Hh.AB5, "J1.txt")).read().split("\n")
+    assert j1[0][:15] == ref[0][:15] and j1[1][:15] == ref[1][:15]
+    assert [len(x) for x in j1[:5]] == [len(x) for x in ref[:5]]
+    kg = open(os.path.join(out, "k_grid.txt")).read()
+    assert kg == open(os.path.join(Hh.AB5, "k_grid.txt")).read()
