@@ -1,0 +1,145 @@
+"""Benchmark of the k-space time-stepping path on one MI355X.
+
+Workload (default): the synthetic Wannier model -- 16 bands, 441 lattice vectors
+(|R1|,|R2| <= 10, hexagonal, a = 2.5 A, seed 1001), 200x200 k-grid, 4 filled bands,
+gaussian pump 0.6 eV / 0.5 V/A along z, RK4 with dt = 0.05 a.u. (4000 steps in the
+full run).  A step is one RK4 step (4 fused RHS+stage launches) over all k, with the
+state and every operator resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line (see README of the contract in DESIGN.md).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--nk", type=int, default=200)
+    ap.add_argument("--nw", type=int, default=16)
+    ap.add_argument("--rmax", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps of the CPU sample (0: auto)")
+    ap.add_argument("--workdir", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the k-grid of one problem lives in one HBM); "
+              "multi-GPU k-slab sharding is a design in DESIGN.md, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    from edus_amd import native as N
+    from edus_amd import workloads
+
+    wd = args.workdir or tempfile.mkdtemp(prefix="edus_bench_")
+    nsteps_total = args.steps + args.warmup
+    t_setup = time.perf_counter()
+    inp = workloads.wannier_synthetic(wd, nw=args.nw, rmax=args.rmax, nk=args.nk, nsteps=max(4000, nsteps_total),
+                                      dt=0.05, stepper="RK4", e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0)
+    s = N.Setup(inp)
+    info = s.info
+    eng = N.Engine(s.problem())
+    t_setup = time.perf_counter() - t_setup
+    nk, nb = info["nk"], info["nb"]
+
+    # Field schedule of the steps around the pulse peak, computed before timing.
+    it0 = max(0, (info["itfi"] + 1) // 2 - nsteps_total // 2)
+    f, g = s.fields(it0, nsteps_total)
+    eng.step(f[:args.warmup], g[:args.warmup])
+    t0 = time.perf_counter()
+    eng.step(f[args.warmup:], g[args.warmup:])
+    dt_wall = time.perf_counter() - t0
+    dev_ms, nlaunch = eng.last_step_time()
+    value = nk * args.steps / dt_wall
+
+    # Roofline of the dominant kernel (the fused RK4 RHS+stage kernel): algorithmic
+    # bytes per launch with the packed Hermitian layout (S = 8 nb^2 bytes per k):
+    # per step 16 state transfers + 4 launches x n_op operators (H, Dx, Dy, Dz; +U with
+    # dephasing), i.e. (4 + n_op) * S * nk per launch.
+    S = 8 * nb * nb
+    n_op = 4 + (2 if info["n_diss"] > 0 else 0)
+    bytes_launch = (4 + n_op) * S * nk
+    avg_launch_s = dev_ms * 1e-3 / nlaunch
+    achieved = bytes_launch / avg_launch_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_stage_wave_nb{nb}_nk{nk}.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"k_stage_wave<{16 if nb <= 16 else 32}> (RK4 stages)", "bytes_per_launch": bytes_launch,
+            "avg_launch_us": round(avg_launch_s * 1e6, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(s, args.cpu_steps)
+
+    out = {
+        "metric": "k-point*steps/s (FP64 RK4, synthetic Wannier 16 bands)",
+        "value": value,
+        "unit": "k-point*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"wannier_nw{nb}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{args.nk}x{args.nk}_rk4_dt0.05",
+                   "nk": nk, "bands": nb, "n_R": (2 * args.rmax + 1) ** 2, "stepper": "RK4", "precision": "fp64",
+                   "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
+                   "device_ms_per_step": dev_ms / args.steps},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(s, steps):
+    """The CPU oracle (FP64 restatement of the reference loops, OpenMP over k) on a
+    bounded sample of the same workload: the full k-grid for a few RK4 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle_py as O
+    import helpers as Hh
+
+    info = s.info
+    nb = info["nb"]
+    osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
+    cores = O.max_threads()
+    P0 = s.array("rho0", True).reshape(-1, nb, nb)
+    pumps = [s.pulse(i) for i in range(info["n_pumps"])]
+    probe = s.pulse(info["n_pumps"])
+    it0 = (info["itfi"] + 1) // 2
+    t_start = s.time(it0)
+    n = steps or 2
+    t0 = time.perf_counter()
+    osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
+    el = time.perf_counter() - t0
+    if not steps and el < 10.0:  # scale the sample to ~10-30 s of CPU work
+        n = max(2, int(n * 12.0 / max(el, 1e-3)))
+        t0 = time.perf_counter()
+        osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
+        el = time.perf_counter() - t0
+    return {"value": info["nk"] * n / el, "unit": "k-point*steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} RK4 steps of the full {info['nk1']}x{info['nk2']} grid ({nb} bands), oracle, {cores} threads",
+            "seconds": round(el, 2)}
+
+
+if __name__ == "__main__":
+    main()

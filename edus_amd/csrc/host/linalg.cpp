@@ -107,7 +107,7 @@ void eigh_jacobi(int n, const cplx *H0, double *w, cplx *V) {
         double off = 0;
         for (int p = 0; p < n; p++)
             for (int q = p + 1; q < n; q++) off += std::norm(A[p * n + q]);
-        if (off <= 1e-34 * fro || off == 0.0) break;
+        if (off <= 1e-28 * fro || off == 0.0) break;
         for (int p = 0; p < n; p++)
             for (int q = p + 1; q < n; q++) {
                 cplx apq = A[p * n + q];

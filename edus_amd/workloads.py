@@ -24,9 +24,14 @@ def sigma_fs_for_intensity_fwhm(fwhm_fs):
 
 
 def t_fin_for(nsteps, dt):
-    """t_fin (a.u.) whose loop runs exactly nsteps steps from t = -0.21 fs."""
+    """t_fin (a.u.) whose loop runs exactly nsteps steps from t = -0.21 fs.  The
+    reference ignores t_fin <= 0 (runge_kutta_init.cpp:366-370), so short runs
+    must be cut by the caller (max_steps)."""
     t0 = -0.21 / TIME_AU_FS
-    return t0 + (nsteps - 0.5) * dt
+    t = t0 + (nsteps - 0.5) * dt
+    if t <= 0:
+        raise ValueError("t_fin would be <= 0; use a longer window and cut the run")
+    return t
 
 
 def _pump(freq_au, e_v_ang, fwhm_fs, pol, t_peak_au, phase=0.0):

@@ -670,6 +670,76 @@ static void lu_inverse(int n, const double *A, double *Ai) {
     }
 }
 
+/* Adams-Bashforth coefficients of ODE_solver.h:152-179 (orders 1..5). */
+static void ab_coefficients(int o, double *coef) {
+    for (int i = 0; i < 5; i++) coef[i] = 0;
+    if (o == 1) coef[0] = 1.;
+    if (o == 2) { coef[0] = 3.0 / 2.0; coef[1] = -1.0 / 2.0; }
+    if (o == 3) { coef[0] = 23.0 / 12.0; coef[1] = -16.0 / 12.0; coef[2] = 5.0 / 12.0; }
+    if (o == 4) { coef[0] = 55.0 / 24.0; coef[1] = -59.0 / 24.0; coef[2] = 37.0 / 24.0; coef[3] = -9.0 / 24.0; }
+    if (o == 5) { coef[0] = 1901.0 / 720.0; coef[1] = -2774.0 / 720.0; coef[2] = 2616.0 / 720.0; coef[3] = -1274.0 / 720.0; coef[4] = 251.0 / 720.0; }
+}
+
+/* tau^{-1} of Taylor_DE_solver_deriv_Polynom.cpp:100-123 for constant dt. */
+static void taylor_tau_inverse(int o, double dt, double *tinv);
+
+/* Increment of one element: AB (Linear_Multistep_ODE_solvers.cpp:296-306) from f[s] = f_{n-s}. */
+static cplx ab_delta(int o, const double *coef, const cplx *f, double dt) {
+    cplx d = 0.0;
+    for (int q = 0; q < o; q++) d += coef[q] * f[q];
+    return dt * d;
+}
+
+/* Increment of one element: Taylor (Taylor_DE_solver_deriv_Polynom.cpp:141-174). */
+static cplx taylor_delta(int o, const double *tinv, const cplx *f, double dt) {
+    const int n = o - 1;
+    cplx dP[8], al[8];
+    for (int i = 0; i < n; i++) dP[i] = f[0] - f[i + 1];
+    for (int i = 0; i < n; i++) {
+        al[i] = 0;
+        for (int p = 0; p < n; p++) al[i] += tinv[i * n + p] * dP[p];
+    }
+    cplx d = f[0] * dt;
+    for (int i = 1; i < o; i++) d += pow(dt, i + 1) * al[i - 1] / (double)(i + 1);
+    return d;
+}
+
+/* Public single-update hooks (tests): f [order][n] complex with f[s] = f_{n-s}. */
+void orc_ab_update(int order, double dt, const double *f, long n, double *delta) {
+    double coef[5];
+    ab_coefficients(order, coef);
+    const cplx *fc = (const cplx *)f;
+    cplx *d = (cplx *)delta;
+    for (long e = 0; e < n; e++) {
+        cplx fe[8];
+        for (int q = 0; q < order; q++) fe[q] = fc[(long)q * n + e];
+        d[e] = ab_delta(order, coef, fe, dt);
+    }
+}
+
+void orc_taylor_update(int order, double dt, const double *f, long n, double *delta) {
+    double tinv[64];
+    taylor_tau_inverse(order, dt, tinv);
+    const cplx *fc = (const cplx *)f;
+    cplx *d = (cplx *)delta;
+    for (long e = 0; e < n; e++) {
+        cplx fe[8];
+        for (int q = 0; q < order; q++) fe[q] = fc[(long)q * n + e];
+        d[e] = taylor_delta(order, tinv, fe, dt);
+    }
+}
+
+static void taylor_tau_inverse(int o, double dt, double *tinv) {
+    int n = o - 1;
+    double ti[8], tau[64], scale = pow(10, 7);
+    ti[0] = dt;
+    for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + dt;
+    for (int i = 0; i < n; i++)
+        for (int p = 0; p < n; p++) tau[i * n + p] = scale * pow(ti[i], (p + 1));
+    lu_inverse(n, tau, tinv);
+    for (int i = 0; i < n * n; i++) tinv[i] *= scale;
+}
+
 long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, double *P0io, double *rows, long max_rows) {
     orc_sys s;
     make_sys(t, &s);
@@ -683,25 +753,9 @@ long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, d
     int idx[16];
     for (int i = 0; i < 16; i++) idx[i] = i;
     double coef[5] = {0};
-    if (r->stepper == 1) {
-        /* ODE_solver.h:152-179 */
-        if (o == 1) coef[0] = 1.;
-        if (o == 2) { coef[0] = 3.0 / 2.0; coef[1] = -1.0 / 2.0; }
-        if (o == 3) { coef[0] = 23.0 / 12.0; coef[1] = -16.0 / 12.0; coef[2] = 5.0 / 12.0; }
-        if (o == 4) { coef[0] = 55.0 / 24.0; coef[1] = -59.0 / 24.0; coef[2] = 37.0 / 24.0; coef[3] = -9.0 / 24.0; }
-        if (o == 5) { coef[0] = 1901.0 / 720.0; coef[1] = -2774.0 / 720.0; coef[2] = 2616.0 / 720.0; coef[3] = -1274.0 / 720.0; coef[4] = 251.0 / 720.0; }
-    }
+    if (r->stepper == 1) ab_coefficients(o, coef);
     double tinv[64];
-    if (r->stepper == 2) {
-        int n = o - 1;
-        double ti[8], tau[64], scale = pow(10, 7);
-        ti[0] = r->dt;
-        for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + r->dt;
-        for (int i = 0; i < n; i++)
-            for (int p = 0; p < n; p++) tau[i * n + p] = scale * pow(ti[i], (p + 1));
-        lu_inverse(n, tau, tinv);
-        for (int i = 0; i < n * n; i++) tinv[i] *= scale;
-    }
+    if (r->stepper == 2) taylor_tau_inverse(o, r->dt, tinv);
     const double dt = r->dt, dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
     double EF[2][3] = {{0, 0, 0}, {0, 0, 0}};
     int E_non0 = 1;
@@ -749,24 +803,17 @@ long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, d
                     for (int ic = 0; ic < nb; ic++)
                         for (int jc = ic; jc < nb; jc++) {
                             size_t e = ((size_t)ik * nb + ic) * nb + jc;
-                            cplx d = 0.0;
-                            for (int q = 0; q < o; q++) d += coef[q] * hist[(size_t)idx[q] * nel + e];
-                            P0[e] += dt * d;
+                            cplx fe[8];
+                            for (int q = 0; q < o; q++) fe[q] = hist[(size_t)idx[q] * nel + e];
+                            P0[e] += ab_delta(o, coef, fe, dt);
                             if (ic != jc) P0[((size_t)ik * nb + jc) * nb + ic] = conj(P0[e]);
                         }
             } else {
-                const int n = o - 1;
 #pragma omp parallel for schedule(static)
                 for (long e = 0; e < (long)nel; e++) {
-                    cplx dP[8], al[8];
-                    for (int i = 0; i < n; i++) dP[i] = hist[(size_t)idx[0] * nel + e] - hist[(size_t)idx[i + 1] * nel + e];
-                    for (int i = 0; i < n; i++) {
-                        al[i] = 0;
-                        for (int p = 0; p < n; p++) al[i] += tinv[i * n + p] * dP[p];
-                    }
-                    cplx d = hist[(size_t)idx[0] * nel + e] * dt;
-                    for (int i = 1; i < o; i++) d += pow(dt, i + 1) * al[i - 1] / (double)(i + 1);
-                    P0[e] += d;
+                    cplx fe[8];
+                    for (int q = 0; q < o; q++) fe[q] = hist[(size_t)idx[q] * nel + e];
+                    P0[e] += taylor_delta(o, tinv, fe, dt);
                 }
             }
         }

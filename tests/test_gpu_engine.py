@@ -49,9 +49,11 @@ def check_series(gpu, ora, nb, what=""):
     np.testing.assert_allclose(g["t"], o["t"], rtol=0, atol=0)
     for key in ("EF", "EFprobe"):
         np.testing.assert_array_equal(g[key], o[key])
+    # error relative to the largest component of the series (Im J1 is zero up to rounding)
+    j1s, tas = max(np.max(np.abs(o["J1"])), 1e-300), max(np.max(np.abs(o["TAbs"])), 1e-300)
     for c in range(6):
-        assert Hh.rel_err(g["J1"][:, c], o["J1"][:, c], 1e-300) <= 1e-10, (what, "J1", c)
-        assert Hh.rel_err(g["TAbs"][:, c], o["TAbs"][:, c], 1e-300) <= 1e-10, (what, "TAbs", c)
+        assert np.max(np.abs(g["J1"][:, c] - o["J1"][:, c])) <= 1e-10 * j1s, (what, "J1", c)
+        assert np.max(np.abs(g["TAbs"][:, c] - o["TAbs"][:, c])) <= 1e-10 * tas, (what, "TAbs", c)
     for c in range(nb):
         assert np.max(np.abs(g["pop"][:, c] - o["pop"][:, c])) <= max(1e-10 * np.max(np.abs(o["pop"][:, c])), 1e-14), (what, "pop", c)
     return g, o
@@ -100,7 +102,7 @@ def test_rhs_small_kernel_matches_oracle(tmp_path, diss):
 
 def _wannier_setup(d, nw, nk=12, rmax=2, t2=0.0, nval=None):
     from edus_amd import workloads
-    p = workloads.wannier_synthetic(d, nw=nw, rmax=rmax, nk=nk, nsteps=60, dt=0.05, nvalence=nval or max(1, nw // 4),
+    p = workloads.wannier_synthetic(d, nw=nw, rmax=rmax, nk=nk, nsteps=400, dt=0.05, nvalence=nval or max(1, nw // 4),
                                     seed=7 + nw, t2_fs=t2, e_v_ang=0.5, it_resolution=5)
     return N.Setup(p)
 
@@ -184,11 +186,11 @@ def test_trajectory_small_tb_matches_oracle(tmp_path, stepper, order):
 @pytest.mark.parametrize("nw,stepper", [(8, "RK4"), (16, "RK4"), (16, "Taylor")])
 def test_trajectory_wannier_matches_oracle(tmp_path, nw, stepper):
     from edus_amd import workloads
-    p = workloads.wannier_synthetic(str(tmp_path), nw=nw, rmax=2, nk=12, nsteps=80, dt=0.05, stepper=stepper,
+    p = workloads.wannier_synthetic(str(tmp_path), nw=nw, rmax=2, nk=12, nsteps=400, dt=0.05, stepper=stepper,
                                     order=4, nvalence=4, seed=11, e_v_ang=0.5, it_resolution=4)
     s = N.Setup(p)
     osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
-    n = s.info["itfi"] + 1
+    n = min(80, s.info["itfi"] + 1)
     ora, Pora = Hh.oracle_run(s, osys, n)
     gpu, eng = run_engine_rows(s, n)
     check_series(gpu, ora, nw, f"wannier{nw}")
