@@ -59,6 +59,16 @@ struct ConstArgs {
     const int *nbr;      // [n_nbr][nkp]
     int nk, nkp, nb, nch, ncc, n_nbr, diss, pad;
     double wx2;
+    // Coulomb mean field (small-nb kernel only): per-k reconstruction from the
+    // per-column partial sums Tt, peak correction against PW0, renormalised H out.
+    int coul, N2, ncut, nu;
+    double n_cond;
+    const double *cxm, *sxm;   // [ncut][N1] cos/sin(2 pi m k1)
+    const double *Tt;          // [2][N2][ncut][nu] complex
+    const double *PW0;         // packed state at t = 0
+    const double *EH;          // [nb] Hartree shift (minus equilibrium)
+    const double *screen;      // [nb*nb] complex
+    double *Hr;                // packed renormalised H (written)
 };
 
 // Per-launch stage data.  `tab` points at the stage's row of the field table:
@@ -162,6 +172,50 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
             P[j][i] = cconj(P[i][j]);
         }
     }
+    // Renormalised H with the Hartree-Fock mean field (RungeKutta.h:69-138):
+    // X_W[i][j] = -sum_mn (X_mn)_{ji} trig(k), minus Screen_const * (P - P0)_{ji}
+    // when n_cond > 1e-20, real diagonal, plus the Hartree shift.
+    cd Hren[NB][NB];
+    if (c.coul) {
+        const int i1 = k / c.N2, i2 = k % c.N2, ncut = c.ncut, N1 = c.nk / c.N2;
+        int u = 0;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = i; j < NB; j++, u++) {
+                cd xk = mk(0.0, 0.0);
+                for (int m = 0; m < ncut; m++) {
+                    const double cm = c.cxm[m * N1 + i1], sm = c.sxm[m * N1 + i1];
+                    const double *tc = c.Tt + (((size_t)0 * c.N2 + i2) * ncut + m) * c.nu * 2 + 2 * u;
+                    const double *ts = c.Tt + (((size_t)1 * c.N2 + i2) * ncut + m) * c.nu * 2 + 2 * u;
+                    xk.x -= cm * tc[0] + sm * ts[0];
+                    xk.y -= cm * tc[1] + sm * ts[1];
+                }
+                if (c.n_cond > 1e-20) {
+                    const int idx = (i != j) ? i * NB + j : 0;
+                    cd scr = mk(c.screen[2 * idx], -c.screen[2 * idx + 1]);  // conj(Screen_const)
+                    cd dp;
+                    if (i == j) dp = mk(y[i] - c.PW0[i * nkp + k], 0.0);
+                    else {
+                        const int p = pair_index(i, j, NB);
+                        // (P - P0)_{ji} = conj((P - P0)_{ij})
+                        dp = mk(y[NB + 2 * p] - c.PW0[(NB + 2 * p) * nkp + k], -(y[NB + 2 * p + 1] - c.PW0[(NB + 2 * p + 1) * nkp + k]));
+                    }
+                    xk = csub(xk, cmul(scr, dp));
+                }
+                cd h;
+                if (i == j) {
+                    h = mk(c.H[i * nkp + k] + xk.x + c.EH[i], 0.0);
+                    c.Hr[i * nkp + k] = h.x;
+                } else {
+                    const int p = pair_index(i, j, NB);
+                    h = mk(c.H[(NB + 2 * p) * nkp + k] + xk.x, c.H[(NB + 2 * p + 1) * nkp + k] + xk.y);
+                    c.Hr[(NB + 2 * p) * nkp + k] = h.x;
+                    c.Hr[(NB + 2 * p + 1) * nkp + k] = h.y;
+                }
+                Hren[i][j] = h;
+            }
+    }
     // M = H + E.D with the probe on core<->non-core elements (RungeKutta.h:165-168).
 #pragma unroll
     for (int i = 0; i < NB; i++) {
@@ -180,6 +234,7 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
 #pragma unroll
                 for (int x = 0; x < 3; x++) dx[x] = mk(c.D[(x * NP + NB + 2 * p) * nkp + k], c.D[(x * NP + NB + 2 * p + 1) * nkp + k]);
             }
+            if (c.coul) h = Hren[i][j];
             cd mij = h, mji = cconj(h);
 #pragma unroll
             for (int x = 0; x < 3; x++) {
@@ -276,6 +331,250 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
     }
 #pragma unroll
     for (int e = 0; e < NP; e++) stage_update<MODE>(a, e * nkp + k, (size_t)NP * nkp, pv[e], y[e]);
+}
+
+// ================================================================ Coulomb mean field
+// Separable partial 2D Fourier sums of the packed state over the grid
+// k = (i1, i2) (Coulomb.h:273-350):
+//   S_y[i1][n][e] = sum_i2 y_n(i2) P_e(i1, i2)                     (y = cos, sin)
+//   F_xy[mm][e]   = w (1 + corr_mn) sum_i1 x_m(i1) S_y[i1][n][e]    (4 combinations)
+// then X_mn from A..D minus the equilibrium (Coulomb.h:413-470) and the column
+// sums of the reconstruction (Coulomb.h:510-541), the row sum being done per k
+// inside the RHS kernel.
+struct CoulArgs {
+    const double *Y;
+    int nk, nkp, nb, NP, N1, N2, ncut, nu;
+    double w;
+    const double *cxm, *sxm, *cyn, *syn;
+    double *S, *F;
+    double *X;                 // [4][ncut^2][nb][nb] complex
+    const double *X0;
+    const double *A, *B, *C, *D, *vh;  // complex
+    double *EH;
+    const double *EH0;
+    double *Tt;                // [2][N2][ncut][nu] complex
+};
+
+__global__ void k_coul_s1(CoulArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = a.N1 * a.ncut * a.NP;
+    if (t >= total) return;
+    const int e = t % a.NP, n = (t / a.NP) % a.ncut, i1 = t / (a.NP * a.ncut);
+    const double *row = a.Y + (size_t)e * a.nkp + (size_t)i1 * a.N2;
+    double sc = 0.0, ss = 0.0;
+    for (int i2 = 0; i2 < a.N2; i2++) {
+        const double p = row[i2];
+        sc += a.cyn[n * a.N2 + i2] * p;
+        ss += a.syn[n * a.N2 + i2] * p;
+    }
+    a.S[(((size_t)0 * a.N1 + i1) * a.ncut + n) * a.NP + e] = sc;
+    a.S[(((size_t)1 * a.N1 + i1) * a.ncut + n) * a.NP + e] = ss;
+}
+
+__global__ void k_coul_s2(CoulArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nmm = a.ncut * a.ncut;
+    if (t >= nmm * a.NP) return;
+    const int e = t % a.NP, mm = t / a.NP, m = mm / a.ncut, n = mm % a.ncut;
+    double cc = 0, cs = 0, sc = 0, ss = 0;
+    for (int i1 = 0; i1 < a.N1; i1++) {
+        const double xc = a.cxm[m * a.N1 + i1], xs = a.sxm[m * a.N1 + i1];
+        const double yc = a.S[(((size_t)0 * a.N1 + i1) * a.ncut + n) * a.NP + e];
+        const double ys = a.S[(((size_t)1 * a.N1 + i1) * a.ncut + n) * a.NP + e];
+        cc += xc * yc;
+        cs += xc * ys;
+        sc += xs * yc;
+        ss += xs * ys;
+    }
+    const double corr = -4 * M_PI * M_PI * (m * m + n * n) / ((double)a.nk * 27);
+    const double f = a.w * (1. + corr);
+    a.F[((size_t)0 * nmm + mm) * a.NP + e] = cc * f;
+    a.F[((size_t)1 * nmm + mm) * a.NP + e] = cs * f;
+    a.F[((size_t)2 * nmm + mm) * a.NP + e] = sc * f;
+    a.F[((size_t)3 * nmm + mm) * a.NP + e] = ss * f;
+}
+
+__global__ void k_coul_x(CoulArgs a) {
+    const int nb = a.nb, nmm = a.ncut * a.ncut, nb2 = nb * nb;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nb) {  // Hartree shift from mode 0 (Coulomb.h:385-410)
+        double e = 0.;
+        for (int jc = 0; jc < nb; jc++) e += a.F[jc] * a.vh[2 * (t * nb + jc)];
+        a.EH[t] = e - a.EH0[t];
+    }
+    if (t >= nmm * a.nu) return;
+    const int mm = t / a.nu, u = t % a.nu;
+    int i = 0, j = 0;
+    for (int q = u, r = 0; ; r++) {
+        if (q < nb - r) { i = r; j = r + q; break; }
+        q -= nb - r;
+    }
+    auto fget = [&](int qd) -> cd {
+        const double *F = a.F + ((size_t)qd * nmm + mm) * a.NP;
+        if (i == j) return mk(F[i], 0.0);
+        const int p = pair_index(i, j, nb);
+        return mk(F[nb + 2 * p], F[nb + 2 * p + 1]);
+    };
+    const cd fcc = fget(0), fcs = fget(1), fsc = fget(2), fss = fget(3);
+    const int d = i * nb + j, od = j * nb + i;
+    const size_t base = (size_t)mm * nb2;
+    const cd *A = reinterpret_cast<const cd *>(a.A), *B = reinterpret_cast<const cd *>(a.B);
+    const cd *C = reinterpret_cast<const cd *>(a.C), *D = reinterpret_cast<const cd *>(a.D);
+    const cd *X0 = reinterpret_cast<const cd *>(a.X0);
+    cd *X = reinterpret_cast<cd *>(a.X);
+    const cd Amn = A[base + (i == j ? 0 : d)], Dmn = D[base + (i == j ? 0 : d)];
+    cd xcc = csub(cadd(cmul(Amn, fcc), cmul(Dmn, fss)), X0[0 * (size_t)nmm * nb2 + base + d]);
+    cd xcs = csub(csub(cmul(Amn, fcs), cmul(Dmn, fsc)), X0[1 * (size_t)nmm * nb2 + base + d]);
+    cd xsc = csub(csub(cmul(Amn, fsc), cmul(Dmn, fcs)), X0[2 * (size_t)nmm * nb2 + base + d]);
+    cd xss = csub(cadd(cmul(Amn, fss), cmul(Dmn, fcc)), X0[3 * (size_t)nmm * nb2 + base + d]);
+    if (i != j) {
+        const cd Bmn = B[base + d], Cmn = C[base + d];
+        xcc = cadd(xcc, cadd(cmul(Bmn, fcs), cmul(Cmn, fsc)));
+        xcs = cadd(xcs, csub(cmul(Cmn, fss), cmul(Bmn, fcc)));
+        xsc = cadd(xsc, csub(cmul(Bmn, fss), cmul(Cmn, fcc)));
+        xss = csub(xss, cadd(cmul(Bmn, fsc), cmul(Cmn, fcs)));
+        X[0 * (size_t)nmm * nb2 + base + od] = cconj(xcc);
+        X[1 * (size_t)nmm * nb2 + base + od] = cconj(xcs);
+        X[2 * (size_t)nmm * nb2 + base + od] = cconj(xsc);
+        X[3 * (size_t)nmm * nb2 + base + od] = cconj(xss);
+    }
+    X[0 * (size_t)nmm * nb2 + base + d] = xcc;
+    X[1 * (size_t)nmm * nb2 + base + d] = xcs;
+    X[2 * (size_t)nmm * nb2 + base + d] = xsc;
+    X[3 * (size_t)nmm * nb2 + base + d] = xss;
+}
+
+// Column sums of the reconstruction for upper element u = (i <= j), which reads
+// (X_mn)_{ji} because the reference stores X_{jc,ic} (Coulomb.h:537).
+__global__ void k_coul_s3(CoulArgs a) {
+    const int nb = a.nb, nmm = a.ncut * a.ncut, nb2 = nb * nb;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.N2 * a.ncut * a.nu) return;
+    const int u = t % a.nu, m = (t / a.nu) % a.ncut, i2 = t / (a.nu * a.ncut);
+    int i = 0, j = 0;
+    for (int q = u, r = 0; ; r++) {
+        if (q < nb - r) { i = r; j = r + q; break; }
+        q -= nb - r;
+    }
+    const cd *X = reinterpret_cast<const cd *>(a.X);
+    cd tc = mk(0, 0), ts = mk(0, 0);
+    for (int n = 0; n < a.ncut; n++) {
+        const size_t d = (size_t)(m * a.ncut + n) * nb2 + j * nb + i;
+        const double yc = a.cyn[n * a.N2 + i2], ys = a.syn[n * a.N2 + i2];
+        tc = cadd(tc, cadd(cscale(X[0 * (size_t)nmm * nb2 + d], yc), cscale(X[1 * (size_t)nmm * nb2 + d], ys)));
+        ts = cadd(ts, cadd(cscale(X[2 * (size_t)nmm * nb2 + d], yc), cscale(X[3 * (size_t)nmm * nb2 + d], ys)));
+    }
+    cd *T = reinterpret_cast<cd *>(a.Tt);
+    T[((size_t)0 * a.N2 + i2) * a.ncut * a.nu + m * a.nu + u] = tc;
+    T[((size_t)1 * a.N2 + i2) * a.ncut * a.nu + m * a.nu + u] = ts;
+}
+
+// Observables with the mean field (Observables_MPI.h:286-304, if_resolution.cpp:125-192):
+// eigenvectors of the renormalised H (cyclic Jacobi), its shell gradient and
+// H_ren D^+ for Current2.  One thread per k, nb <= 4.
+template <int NB>
+__global__ __launch_bounds__(256) void k_coul_obs(const double *Hr, const double *Dp, const int *nbr, const double *wb, int n_nbr,
+                                                   int nk, int nkp, double *Uexc, double *GE, double *Q) {
+    constexpr int NP = NB * NB;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    cd A[NB][NB], V[NB][NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        A[i][i] = mk(Hr[i * (size_t)nkp + k], 0.0);
+        V[i][i] = mk(1.0, 0.0);
+#pragma unroll
+        for (int j = i + 1; j < NB; j++) {
+            const int p = pair_index(i, j, NB);
+            A[i][j] = mk(Hr[(NB + 2 * p) * (size_t)nkp + k], Hr[(NB + 2 * p + 1) * (size_t)nkp + k]);
+            A[j][i] = cconj(A[i][j]);
+            V[i][j] = V[j][i] = mk(0.0, 0.0);
+        }
+    }
+    for (int sweep = 0; sweep < 20; sweep++) {
+        double off = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                const double v = A[i][j].x * A[i][j].x + A[i][j].y * A[i][j].y;
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (off <= 1e-30 * tot) break;
+#pragma unroll
+        for (int p = 0; p < NB; p++)
+#pragma unroll
+            for (int q = p + 1; q < NB; q++) {
+                const double mag = sqrt(A[p][q].x * A[p][q].x + A[p][q].y * A[p][q].y);
+                if (mag == 0.0) continue;
+                const cd e = mk(A[p][q].x / mag, A[p][q].y / mag);
+                const double th = 0.5 * (A[q][q].x - A[p][p].x) / mag;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+#pragma unroll
+                for (int r = 0; r < NB; r++) {
+                    cd x = A[r][p], y = A[r][q];
+                    A[r][p] = csub(cscale(x, cs), cscale(cmul(cconj(e), y), sn));
+                    A[r][q] = cadd(cscale(cmul(e, x), sn), cscale(y, cs));
+                    x = V[r][p]; y = V[r][q];
+                    V[r][p] = csub(cscale(x, cs), cscale(cmul(cconj(e), y), sn));
+                    V[r][q] = cadd(cscale(cmul(e, x), sn), cscale(y, cs));
+                }
+#pragma unroll
+                for (int r = 0; r < NB; r++) {
+                    cd x = A[p][r], y = A[q][r];
+                    A[p][r] = csub(cscale(x, cs), cscale(cmul(e, y), sn));
+                    A[q][r] = cadd(cscale(cmul(cconj(e), x), sn), cscale(y, cs));
+                }
+            }
+    }
+    // ascending eigenvalues; Uexc rows are conj(v_a)
+    int ord[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) ord[i] = i;
+    for (int i = 0; i < NB; i++)
+        for (int j = i + 1; j < NB; j++)
+            if (A[ord[j]][ord[j]].x < A[ord[i]][ord[i]].x) { int tt = ord[i]; ord[i] = ord[j]; ord[j] = tt; }
+    double *u = Uexc + (size_t)k * NB * NB * 2;
+    for (int a2 = 0; a2 < NB; a2++)
+        for (int b = 0; b < NB; b++) {
+            const cd v = V[b][ord[a2]];
+            u[(a2 * NB + b) * 2] = v.x;
+            u[(a2 * NB + b) * 2 + 1] = -v.y;
+        }
+    // gradient of the renormalised H on the shell stencil (packed Hermitian)
+    for (int x = 0; x < 3; x++)
+        for (int e = 0; e < NP; e++) {
+            const double h0 = Hr[e * (size_t)nkp + k];
+            double g = 0.0;
+            for (int n = 0; n < n_nbr; n++) g += wb[n * 3 + x] * (Hr[e * (size_t)nkp + nbr[(size_t)n * nkp + k]] - h0);
+            GE[((size_t)k * 3 + x) * NP + e] = g;
+        }
+    // Q_x = H_ren D_x^+ (D symmetric, packed)
+    for (int x = 0; x < 3; x++)
+        for (int i = 0; i < NB; i++)
+            for (int j = 0; j < NB; j++) {
+                cd acc = mk(0, 0);
+                for (int kc = 0; kc < NB; kc++) {
+                    cd h, d;
+                    if (i == kc) h = mk(Hr[i * (size_t)nkp + k], 0.0);
+                    else {
+                        const int p = pair_index(min(i, kc), max(i, kc), NB);
+                        h = mk(Hr[(NB + 2 * p) * (size_t)nkp + k], Hr[(NB + 2 * p + 1) * (size_t)nkp + k]);
+                        if (i > kc) h = cconj(h);
+                    }
+                    if (j == kc) d = mk(Dp[(x * NP + j) * (size_t)nkp + k], 0.0);
+                    else {
+                        const int p = pair_index(min(j, kc), max(j, kc), NB);
+                        d = mk(Dp[(x * NP + NB + 2 * p) * (size_t)nkp + k], Dp[(x * NP + NB + 2 * p + 1) * (size_t)nkp + k]);
+                    }
+                    acc = cadd(acc, cmul(h, cconj(d)));
+                }
+                double *q = Q + ((((size_t)k * 3 + x) * NB + i) * NB + j) * 2;
+                q[0] = acc.x;
+                q[1] = acc.y;
+            }
 }
 
 // ================================================================ nb in 5..32: wavefront per k
@@ -751,6 +1050,13 @@ struct edus_ctx {
     float last_ms = 0;
     int last_launches = 0;
     std::vector<void *> allocs;
+    // Coulomb mean field
+    int coul = 0, N1 = 0, N2 = 0, ncut = 0, nu = 0;
+    double n_cond = 0.0;
+    double *cxm = nullptr, *sxm = nullptr, *cyn = nullptr, *syn = nullptr, *cS = nullptr, *cF = nullptr;
+    double *cX = nullptr, *cX0 = nullptr, *cA = nullptr, *cB = nullptr, *cC = nullptr, *cD = nullptr, *cvh = nullptr;
+    double *cEH = nullptr, *cEH0 = nullptr, *cT = nullptr, *cPW0 = nullptr, *cHr = nullptr, *cscreen = nullptr;
+    double *cUexc = nullptr, *cGE = nullptr, *cQ = nullptr, *dwb = nullptr;
 };
 
 namespace {
@@ -811,12 +1117,40 @@ int upload(edus_ctx *c, double *dst, const std::vector<double> &v) {
 int state_elems(const edus_ctx *c) { return c->soa ? c->np * c->nkp : c->np * c->nk; }
 
 // ---------------------------------------------------------------- launches
+CoulArgs coul_args(edus_ctx *c, const double *Y) {
+    CoulArgs a;
+    a.Y = Y; a.nk = c->nk; a.nkp = c->nkp; a.nb = c->nb; a.NP = c->np; a.N1 = c->N1; a.N2 = c->N2; a.ncut = c->ncut; a.nu = c->nu;
+    a.w = 1.0 / (double)c->nk;
+    a.cxm = c->cxm; a.sxm = c->sxm; a.cyn = c->cyn; a.syn = c->syn; a.S = c->cS; a.F = c->cF; a.X = c->cX; a.X0 = c->cX0;
+    a.A = c->cA; a.B = c->cB; a.C = c->cC; a.D = c->cD; a.vh = c->cvh; a.EH = c->cEH; a.EH0 = c->cEH0; a.Tt = c->cT;
+    return a;
+}
+
+// Mean-field coefficients of state Y: reduction, X_mn, column sums (4 launches).
+int launch_coulomb(edus_ctx *c, const double *Y) {
+    CoulArgs a = coul_args(c, Y);
+    const int t1 = c->N1 * c->ncut * c->np, t2 = c->ncut * c->ncut * c->np;
+    const int tx = std::max(c->ncut * c->ncut * c->nu, c->nb), t3 = c->N2 * c->ncut * c->nu;
+    hipLaunchKernelGGL(k_coul_s1, dim3((t1 + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_s2, dim3((t2 + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_x, dim3((tx + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_s3, dim3((t3 + 255) / 256), dim3(256), 0, c->stream, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 template <int MODE>
 int launch_stage(edus_ctx *c, const StageArgs &a) {
     ConstArgs ca;
     ca.H = c->H; ca.D = c->D; ca.U = c->U; ca.T = c->T; ca.nbr = c->nbr;
     ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
     ca.diss = c->diss; ca.pad = 0; ca.wx2 = c->wx2;
+    ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
+    ca.cxm = c->cxm; ca.sxm = c->sxm; ca.Tt = c->cT; ca.PW0 = c->cPW0; ca.EH = c->cEH; ca.screen = c->cscreen; ca.Hr = c->cHr;
+    if (c->coul) {
+        int rc = launch_coulomb(c, a.Y);
+        if (rc) return rc;
+    }
     if (c->soa) {
         const int blocks = (c->nk + 255) / 256;
         switch (c->nb) {
@@ -1034,6 +1368,61 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
                 return fail(set_err(EDUS_ERR_HIP, "memset state"));
         }
     }
+    if (p->coulomb) {
+        const edus_coulomb *q = p->coulomb;
+        if (!c->soa) return fail(set_err(EDUS_ERR_UNSUPPORTED, "the Coulomb term is implemented for nb <= 4"));
+        if (p->nk_dims[0] != 1 || p->nk_dims[1] * p->nk_dims[2] != p->nk)
+            return fail(set_err(EDUS_ERR_UNSUPPORTED, "the Coulomb term needs a 1 x n1 x n2 grid (sample in yz)"));
+        if (q->ncut < 1 || !q->A || !q->B || !q->C || !q->D || !q->screen || !q->v_hartree)
+            return fail(set_err(EDUS_ERR_INVALID, "incomplete Coulomb tables"));
+        c->coul = 1; c->N1 = p->nk_dims[1]; c->N2 = p->nk_dims[2]; c->ncut = q->ncut; c->nu = nb * (nb + 1) / 2;
+        const int nmm = c->ncut * c->ncut, nb2 = nb * nb;
+        // trig tables cos/sin(2 pi m k) with k = -0.5 + i/N (integrationWeight.h:139-175)
+        std::vector<double> cx((size_t)c->ncut * c->N1), sx(cx.size()), cy((size_t)c->ncut * c->N2), sy(cy.size());
+        for (int m = 0; m < c->ncut; m++) {
+            for (int i = 0; i < c->N1; i++) {
+                const double kk = (1. / c->N1 != 1.) ? -0.5 + (1. / c->N1) * i : 0.;
+                const double arg = 2 * M_PI * m * (kk);
+                cx[(size_t)m * c->N1 + i] = cos(arg);
+                sx[(size_t)m * c->N1 + i] = sin(arg);
+            }
+            for (int i = 0; i < c->N2; i++) {
+                const double kk = (1. / c->N2 != 1.) ? -0.5 + (1. / c->N2) * i : 0.;
+                const double arg = 2 * M_PI * m * (kk);
+                cy[(size_t)m * c->N2 + i] = cos(arg);
+                sy[(size_t)m * c->N2 + i] = sin(arg);
+            }
+        }
+        auto up = [&](double **dst, const std::vector<double> &v) -> int {
+            int r = dalloc(c, dst, v.size());
+            return r ? r : upload(c, *dst, v);
+        };
+        auto upc = [&](double **dst, const double *src, size_t n) -> int { return up(dst, std::vector<double>(src, src + n)); };
+        if ((rc = up(&c->cxm, cx)) || (rc = up(&c->sxm, sx)) || (rc = up(&c->cyn, cy)) || (rc = up(&c->syn, sy))) return fail(rc);
+        if ((rc = upc(&c->cA, q->A, (size_t)nmm * nb2 * 2)) || (rc = upc(&c->cB, q->B, (size_t)nmm * nb2 * 2)) ||
+            (rc = upc(&c->cC, q->C, (size_t)nmm * nb2 * 2)) || (rc = upc(&c->cD, q->D, (size_t)nmm * nb2 * 2)) ||
+            (rc = upc(&c->cscreen, q->screen, (size_t)nb2 * 2)) || (rc = upc(&c->cvh, q->v_hartree, (size_t)nb2 * 2)))
+            return fail(rc);
+        if ((rc = dalloc(c, &c->cS, (size_t)2 * c->N1 * c->ncut * c->np)) || (rc = dalloc(c, &c->cF, (size_t)4 * nmm * c->np)) ||
+            (rc = dalloc(c, &c->cX, (size_t)4 * nmm * nb2 * 2)) || (rc = dalloc(c, &c->cT, (size_t)2 * c->N2 * c->ncut * c->nu * 2)) ||
+            (rc = dalloc(c, &c->cEH, nb)) || (rc = dalloc(c, &c->cPW0, se)) || (rc = dalloc(c, &c->cHr, se)) ||
+            (rc = dalloc(c, &c->cUexc, (size_t)p->nk * nb2 * 2)) || (rc = dalloc(c, &c->cGE, (size_t)p->nk * 3 * c->np)) ||
+            (rc = dalloc(c, &c->cQ, (size_t)p->nk * 3 * nb2 * 2)))
+            return fail(rc);
+        if ((rc = up(&c->cX0, std::vector<double>((size_t)4 * nmm * nb2 * 2, 0.0))) || (rc = up(&c->cEH0, std::vector<double>(nb, 0.0))) ||
+            (rc = upc(&c->dwb, p->nbr_wb, (size_t)std::max(p->n_nbr, 1) * 3)))
+            return fail(rc);
+        // P_Wannier_0 = P0, renormalised H starts as H (main_MPI.cpp:214, 330)
+        if (hipMemcpyAsync(c->cPW0, c->P0, se * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->cHr, c->H, se * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return fail(set_err(EDUS_ERR_HIP, "copy P0/H"));
+        // equilibrium: X0 and the Hartree shift of P0 (main_MPI.cpp:276-321)
+        if ((rc = launch_coulomb(c, c->P0))) return fail(rc);
+        if (hipMemcpyAsync(c->cX0, c->cX, (size_t)4 * nmm * nb2 * 2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->cEH0, c->cEH, nb * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return fail(set_err(EDUS_ERR_HIP, "copy equilibrium"));
+        c->n_cond = 0.0;
+    }
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "sync after upload"));
     *out = c;
     g_err.clear();
@@ -1151,6 +1540,19 @@ int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *t
     if (rc) return rc;
     ObsArgs o;
     o.P = current_state(c); o.U = c->U; o.GE = c->GE; o.Q = c->Q; o.DT = c->DT; o.part = c->part;
+    if (c->coul) {
+        const int blocks = (c->nk + 255) / 256;
+        switch (c->nb) {
+            case 1: hipLaunchKernelGGL(k_coul_obs<1>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
+            case 2: hipLaunchKernelGGL(k_coul_obs<2>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
+            case 3: hipLaunchKernelGGL(k_coul_obs<3>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
+            default: hipLaunchKernelGGL(k_coul_obs<4>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
+        }
+        HIP_TRY(hipGetLastError());
+        o.U = c->cUexc;
+        o.Q = c->cQ;
+        if (c->GE) o.GE = c->cGE;
+    }
     o.nk = c->nk; o.nb = c->nb; o.nch = c->nch; o.ncc = c->ncc;
     o.sk = c->soa ? 1 : c->np;
     o.se = c->soa ? c->nkp : 1;
@@ -1173,6 +1575,14 @@ int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *t
     if (tabs)
         for (int x = 0; x < 3; x++) { tabs[2 * x] = h[nb + 9 + 2 * x] * c->tabs_norm; tabs[2 * x + 1] = h[nb + 10 + 2 * x] * c->tabs_norm; }
     if (pminmax) { pminmax[0] = h[nv]; pminmax[1] = h[nv + 1]; }
+    if (c->coul) {  // n_cond: largest printed population (Observables_MPI.h:413-426)
+        double nc = 0;
+        for (int i = 0; i < nb; i++) {
+            const double v = (i < c->ncc) ? 1.0 - h[i] * inv : h[i] * inv;
+            if (v > nc) nc = v;
+        }
+        c->n_cond = nc;
+    }
     return 0;
 }
 

@@ -9,6 +9,7 @@
 
 struct edus_setup {
     edus::Setup s;
+    edus_coulomb coul;
 };
 
 namespace edus {
@@ -38,6 +39,18 @@ void fill_problem(const Setup &s, edus_problem &p) {
     p.tabs_norm = std::fabs(s.L.jacobian) * dk[0] * dk[1] * dk[2];
     p.nktot = (double)s.nk;
     p.taylor_tau_inv = s.tau_inv.empty() ? nullptr : s.tau_inv.data();
+    for (int i = 0; i < 3; i++) p.nk_dims[i] = s.in.Nk[i];
+}
+
+void fill_coulomb(const Setup &s, edus_coulomb &c) {
+    std::memset(&c, 0, sizeof c);
+    c.ncut = s.ncut;
+    c.A = reinterpret_cast<const double *>(s.cA.data());
+    c.B = reinterpret_cast<const double *>(s.cB.data());
+    c.C = reinterpret_cast<const double *>(s.cC.data());
+    c.D = reinterpret_cast<const double *>(s.cD.data());
+    c.screen = reinterpret_cast<const double *>(s.cscreen.data());
+    c.v_hartree = reinterpret_cast<const double *>(s.cvh.data());
 }
 }  // namespace edus
 
@@ -61,6 +74,7 @@ int edus_setup_from_input(const char *input_path, int32_t flags, edus_setup **ou
     auto *h = new edus_setup();
     try {
         h->s.build(input_path, flags);
+        if (h->s.in.coulomb) edus::fill_coulomb(h->s, h->coul);
     } catch (...) {
         delete h;
         throw;
@@ -95,6 +109,7 @@ int edus_setup_get_info(const edus_setup *h, edus_setup_info *info) {
     info->j_norm = p.j_norm; info->tabs_norm = p.tabs_norm; info->nktot = p.nktot; info->wx2 = p.wx2;
     info->T1 = s.in.T1; info->T2 = s.in.T2; info->Tch = s.in.Tch;
     info->shell_residual = s.sh.residual;
+    info->ncut = s.in.coulomb ? s.ncut : 0;
     return 0;
 }
 
@@ -119,6 +134,12 @@ const double *edus_setup_array(const edus_setup *h, const char *name, int64_t *c
     if (n == "weights") return ret(s.sh.weights.data(), s.sh.weights.size());
     if (n == "taylor_tau_inv") return ret(s.tau_inv.data(), s.tau_inv.size());
     if (n == "times") return ret(s.times.data(), s.times.size());
+    if (n == "coul_A") return retc(s.cA);
+    if (n == "coul_B") return retc(s.cB);
+    if (n == "coul_C") return retc(s.cC);
+    if (n == "coul_D") return retc(s.cD);
+    if (n == "coul_screen") return retc(s.cscreen);
+    if (n == "coul_vh") return retc(s.cvh);
     if (count) *count = 0;
     return nullptr;
 }
@@ -190,6 +211,7 @@ int edus_setup_eval_tb(const edus_setup *h, const double *kcrys, int32_t n, doub
 int edus_setup_problem(const edus_setup *h, edus_problem *p) {
     if (!h || !p) return EDUS_ERR_INVALID;
     edus::fill_problem(h->s, *p);
+    if (h->s.in.coulomb) p->coulomb = &h->coul;
     return 0;
 }
 

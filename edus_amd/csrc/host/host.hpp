@@ -80,8 +80,9 @@ struct Input {
     int it_resolution = 10;
     bool iCurrent = false, iTAbs = true, iTAbsK = false, iWFDs = false;
     bool vectorization = true;
-    // Coulomb section (values only; the mean-field term is not in this build's
-    // time loop, see edus_setup_from_input).
+    // Coulomb section.  The mean-field term runs with coefficients read from
+    // files (coulomb_read) in the Wannier basis; the other options are parsed
+    // for their error messages only.
     bool coulomb = false, coulomb_diag_basis = false, rytova_keldysh = false, coulomb_read = false;
     double qTF = 0, epsilon_static = 1.0, r0 = 18.89726, G_distance = 50;
     int Ncut = 30;
@@ -166,6 +167,10 @@ struct Setup {
     int64_t itfi = 0;
     int stepper = 0, order = 1;
     std::vector<double> times;     // accumulated time_loop per step, [itfi+2]
+    // Coulomb mean field (read from the coefficient files)
+    int ncut = 0;
+    std::vector<cplx> cA, cB, cC, cD, cscreen, cvh;
+    void read_coulomb();
 
     void build(const std::string &input_path, int flags);
     void fields_at(double t, double ef[6]) const;   // pump sum + 0.5 probe

@@ -36,6 +36,7 @@ struct Engine {
 }  // namespace
 
 void fill_problem(const Setup &s, edus_problem &p);
+void fill_coulomb(const Setup &s, edus_coulomb &c);
 
 int64_t run_loop(const Setup &s, const std::string &outdir, int64_t max_steps, int verbose) {
     const double tfs = Units::time_au_fs;
@@ -48,6 +49,11 @@ int64_t run_loop(const Setup &s, const std::string &outdir, int64_t max_steps, i
 
     edus_problem p;
     fill_problem(s, p);
+    edus_coulomb coul;
+    if (s.in.coulomb) {
+        fill_coulomb(s, coul);
+        p.coulomb = &coul;
+    }
     Engine eng;
     if (edus_create(&p, &eng.ctx) != 0) throw Error(std::string("edus_create: ") + edus_last_error());
 

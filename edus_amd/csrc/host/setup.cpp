@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <omp.h>
 
 #include "../../../include/edus.h"
@@ -22,7 +23,11 @@ static inline size_t mi(int nb, int k, int i, int j) { return ((size_t)k * nb + 
 void Setup::build(const std::string &input_path, int flags) {
     in = parse_input(input_path);
     if (in.dyn_dt) throw Error("dynamical_dt_evolution is not supported by this build (constant dt only)");
-    if (in.coulomb) throw Error("the Coulomb (Hartree-Fock) term is not supported by this build");
+    if (in.coulomb) {
+        if (in.coulomb_diag_basis) throw Error("Coulomb_diag_basis is not supported by this build (Wannier basis only)");
+        if (!in.coulomb_read)
+            throw Error("the Coulomb coefficients must be read from files (Read_Coulomb_from_files); computing them is not in this build");
+    }
     if (in.iMode != "TB" && in.iMode != "W") throw Error("iMode = " + in.iMode + " is incorrect");
 
     L.set_from_b(in.b);
@@ -50,6 +55,11 @@ void Setup::build(const std::string &input_path, int flags) {
         det_a = std::fabs(in.a[1][1] * in.a[2][2] - in.a[1][2] * in.a[2][1]);
     }
 
+    if (in.coulomb) {
+        if (in.Nk[0] != 1 || in.Nk[2] < in.Nk[1]) throw Error("the Coulomb term needs a 1 x N1 x N2 grid in the yz plane");
+        if (nb > 4) throw Error("the Coulomb term is implemented for up to 4 bands");
+        read_coulomb();
+    }
     sh = find_shells(kpt, in.Nk, L);
     nbr = neighbours_closed_form(in.Nk, sh);
     const int nn = (int)sh.offsets.size();
@@ -206,6 +216,39 @@ void Setup::build(const std::string &input_path, int flags) {
     times.resize((size_t)itfi + 2);
     double t = t_start;
     for (int64_t it = 0; it <= itfi + 1; it++) { times[it] = t; t += in.dt; }
+}
+
+// Coefficient files of Fourier_Coulomb.cpp:505-622: V_Hartree (nb lines of nb
+// reals), A/B/C/D (ncut^2 lines of 2 nb^2 numbers, re im pairs), Screen_const
+// (nb^2 lines "re im"), all under Output/Coulomb/ next to the input file.
+void Setup::read_coulomb() {
+    ncut = in.Ncut;
+    const int nb2 = nb * nb, nmm = ncut * ncut;
+    const std::string dir = in.base_dir + "/Output/Coulomb/";
+    auto numbers = [&](const std::string &path, size_t expect) {
+        std::ifstream f(path);
+        if (!f) throw Error("cannot open Coulomb coefficient file " + path);
+        std::vector<double> v;
+        double x;
+        while (f >> x) v.push_back(x);
+        if (v.size() < expect) throw Error("Coulomb coefficient file " + path + " is too short");
+        return v;
+    };
+    std::vector<double> vh = numbers(dir + "V_Hartree_" + in.coulomb_label + ".txt", (size_t)nb2);
+    cvh.resize(nb2);
+    for (int i = 0; i < nb2; i++) cvh[i] = cplx(vh[i], 0.0);
+    auto table = [&](const char *name, std::vector<cplx> &out) {
+        std::vector<double> v = numbers(dir + name + in.coulomb_label + ".txt", (size_t)nmm * nb2 * 2);
+        out.resize((size_t)nmm * nb2);
+        for (size_t i = 0; i < out.size(); i++) out[i] = cplx(v[2 * i], v[2 * i + 1]);
+    };
+    table("A_coeff", cA);
+    table("B_coeff", cB);
+    table("C_coeff", cC);
+    table("D_coeff", cD);
+    std::vector<double> sc = numbers(dir + "Screen_const_" + in.coulomb_label + ".txt", (size_t)nb2 * 2);
+    cscreen.resize(nb2);
+    for (int i = 0; i < nb2; i++) cscreen[i] = cplx(sc[2 * i], sc[2 * i + 1]);
 }
 
 // Pump sum and half probe at time t (RungeKutta.cpp:24-30).

@@ -27,7 +27,8 @@ class Problem(C.Structure):
         ("nbr", _ip), ("nbr_wb", _dp), ("H", _dp), ("D", _dp), ("U", _dp), ("T", _dp),
         ("rho0", _dp), ("gradH", _dp),
         ("wx2", C.c_double), ("dt", C.c_double), ("j_norm", C.c_double), ("tabs_norm", C.c_double),
-        ("nktot", C.c_double), ("taylor_tau_inv", _dp),
+        ("nktot", C.c_double), ("taylor_tau_inv", _dp), ("coulomb", C.c_void_p), ("nk_dims", C.c_int32 * 3),
+        ("reserved1", C.c_int32),
     ]
 
 
@@ -42,6 +43,7 @@ class SetupInfo(C.Structure):
         ("det_a", C.c_double), ("det_b", C.c_double), ("jacobian", C.c_double),
         ("j_norm", C.c_double), ("tabs_norm", C.c_double), ("nktot", C.c_double), ("wx2", C.c_double),
         ("T1", C.c_double), ("T2", C.c_double), ("Tch", C.c_double), ("shell_residual", C.c_double),
+        ("ncut", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self):

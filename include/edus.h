@@ -43,6 +43,17 @@ extern "C" {
 
 typedef struct edus_ctx edus_ctx;
 
+/* Hartree-Fock mean field in the Wannier basis as a truncated separable Fourier
+ * series of V(k-k') (Coulomb.h:273-541, coefficients as read by
+ * Fourier_Coulomb.cpp:505-622).  Sample in the yz plane (grid 1 x n1 x n2). */
+typedef struct edus_coulomb {
+    int32_t ncut;            /* Fourier modes per direction (Coulomb_set.Ncut)       */
+    int32_t reserved;
+    const double *A, *B, *C, *D;  /* [ncut*ncut][nb*nb] complex                      */
+    const double *screen;    /* [nb*nb] complex Screen_const[0][*]                   */
+    const double *v_hartree; /* [nb][nb] complex V_Hartree                           */
+} edus_coulomb;
+
 /*
  * Everything the time-stepping path reads, in reference layout.  Built on the host
  * by the setup code (headers/InitParameters.h, CalculateWeigths.h, TightBinding.h,
@@ -72,6 +83,9 @@ typedef struct edus_problem {
     double tabs_norm;    /* |det M|*dk0*dk1*dk2       (Observables_MPI.h:568)             */
     double nktot;        /* Nk0*Nk1*Nk2               (Losses and TAbs normalisation)     */
     const double *taylor_tau_inv; /* [(order-1)^2] row-major tau^{-1} for EDUS_STEPPER_TAYLOR */
+    const edus_coulomb *coulomb;  /* NULL: no mean-field term (nb <= 4, grid 1 x n1 x n2)    */
+    int32_t nk_dims[3];           /* grid sizes (Nk0, Nk1, Nk2), needed with coulomb         */
+    int32_t reserved1;
 } edus_problem;
 
 /* Field schedule entry of one step: for every field evaluation of the step the

@@ -41,6 +41,8 @@ typedef struct edus_setup_info {
     double j_norm, tabs_norm, nktot, wx2;
     double T1, T2, Tch;
     double shell_residual;          /* norm(A w - q) of the accepted shells               */
+    int32_t ncut;                   /* Coulomb Fourier modes per direction (0: no Coulomb) */
+    int32_t reserved;
 } edus_setup_info;
 
 /* Parses `input_path` (reference grammar, headers/ReadInput.h:332-1234) and builds
@@ -61,6 +63,7 @@ int edus_setup_get_info(const edus_setup *s, edus_setup_info *info);
  *   "nbr_wb"[n_nbr][3]         weight * b-vector (cartesian)
  *   "bvec"  [n_nbr][3]         b-vectors (cartesian), "weights" [n_shells]
  *   "taylor_tau_inv" [(order-1)^2]
+ *   "coul_A" .. "coul_D" [ncut^2][nb^2] complex, "coul_screen" [nb^2], "coul_vh" [nb][nb]
  * `count` receives the number of doubles (or int32 for "nbr"). */
 const double *edus_setup_array(const edus_setup *s, const char *name, int64_t *count);
 const int32_t *edus_setup_nbr(const edus_setup *s, int64_t *count);

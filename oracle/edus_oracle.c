@@ -34,6 +34,9 @@
 typedef double complex cplx;
 
 #define ORC_PI (4.0 * atan(1.0))
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 #define ORC_MAXNB 32
 #define ORC_MAXNBR 64
 
@@ -405,7 +408,27 @@ void orc_rho0(const double *U, long nk, int nb, int nbands, double *P0) {
 }
 
 /* ------------------------------------------------------------------ right-hand side (RungeKutta.h:7-297) */
+/* Coulomb mean-field (Wannier basis): parameters and the mutable state of a run. */
+typedef struct orc_coul_params {
+    int ncut, dir1, dir2;          /* crystal directions of (x, y): (1, 2) for "011", (0, 1) for "110" */
+    const double *kpt;             /* [nk][3] crystal k */
+    const double *A, *B, *C, *D;   /* [ncut^2][nb^2] complex (Fourier_Coulomb.cpp:505-622) */
+    const double *screen;          /* [nb^2] complex Screen_const[0][*] */
+    const double *vh;              /* [nb][nb] complex V_Hartree */
+} orc_coul_params;
+
+typedef struct orc_coul {
+    const orc_coul_params *p;
+    double *cosx, *sinx, *cosy, *siny;  /* [ncut][nk] (integrationWeight.h:139-175) */
+    cplx *X[4], *X0[4];                 /* [ncut^2][nb^2]: cc, cs, sc, ss */
+    double EH[ORC_MAXNB], EH0[ORC_MAXNB];
+    cplx *PW0;                          /* P at t = 0 */
+    cplx *Hren;                         /* renormalised H of the last RHS */
+    double n_cond;
+} orc_coul;
+
 typedef struct orc_sys {
+    orc_coul *coul;
     long nk;
     int nb, nch, ncc, n_nbr;
     const cplx *H, *D, *U;     /* [nk][nb][nb], [nk][nb][nb][3], [nk][nb][nb] */
@@ -417,20 +440,175 @@ typedef struct orc_sys {
     int n_diss, diss0[ORC_MAXNB * ORC_MAXNB], diss1[ORC_MAXNB * ORC_MAXNB];
 } orc_sys;
 
+
+/* Calculate_X_coefficients_MPI (Coulomb.h:273-503): partial 2D Fourier sums of P
+ * over the grid with the corr_trig factor, Hartree shift from mode 0, then X_mn
+ * from A..D minus the equilibrium X0.  Sums run k-chunked in a fixed order. */
+static void coulomb_coefficients(const orc_sys *s, const cplx *P) {
+    orc_coul *c = s->coul;
+    const int Ncv = s->nb, Ncut = c->p->ncut, nb2 = Ncv * Ncv;
+    const long nk = s->nk;
+    const double N_BZ = (double)nk, w = 1.0 / N_BZ;
+    const cplx *A = (const cplx *)c->p->A, *B = (const cplx *)c->p->B, *Cc = (const cplx *)c->p->C, *D = (const cplx *)c->p->D;
+    const cplx *vh = (const cplx *)c->p->vh;
+    for (int mm = 0; mm < Ncut * Ncut; mm++) {
+        const int m = mm / Ncut, n = mm % Ncut;
+        const double corr = -4 * M_PI * M_PI * (m * m + n * n) / (N_BZ * 27);
+        cplx f[4][ORC_MAXNB * ORC_MAXNB];
+        memset(f, 0, sizeof f);
+        const long CH = 1024, nch = (nk + CH - 1) / CH;
+        cplx *part = calloc((size_t)nch * 4 * nb2, sizeof(cplx));
+#pragma omp parallel for schedule(static)
+        for (long ch = 0; ch < nch; ch++) {
+            cplx *pf = &part[(size_t)ch * 4 * nb2];
+            for (long ik = ch * CH; ik < (ch + 1) * CH && ik < nk; ik++) {
+                const double cc = c->cosx[m * nk + ik] * c->cosy[n * nk + ik], cs = c->cosx[m * nk + ik] * c->siny[n * nk + ik];
+                const double sc = c->sinx[m * nk + ik] * c->cosy[n * nk + ik], ss = c->sinx[m * nk + ik] * c->siny[n * nk + ik];
+                for (int ic = 0; ic < Ncv; ic++)
+                    for (int jc = ic; jc < Ncv; jc++) {
+                        const cplx Pc = w * P[(ik * Ncv + ic) * Ncv + jc];
+                        pf[0 * nb2 + ic * Ncv + jc] += Pc * cc * (1. + corr);
+                        pf[1 * nb2 + ic * Ncv + jc] += Pc * cs * (1. + corr);
+                        pf[2 * nb2 + ic * Ncv + jc] += Pc * sc * (1. + corr);
+                        pf[3 * nb2 + ic * Ncv + jc] += Pc * ss * (1. + corr);
+                    }
+            }
+        }
+        for (long ch = 0; ch < nch; ch++)
+            for (int q = 0; q < 4; q++)
+                for (int e = 0; e < nb2; e++) f[q][e] += part[(size_t)ch * 4 * nb2 + q * nb2 + e];
+        free(part);
+        if (mm == 0) {  /* Hartree shift of the diagonal (Coulomb.h:385-410) */
+            for (int ic = 0; ic < Ncv; ic++) {
+                double e = 0.;
+                for (int jc = 0; jc < Ncv; jc++) e += creal(f[0][jc * Ncv + jc]) * creal(vh[ic * Ncv + jc]);
+                c->EH[ic] = e - c->EH0[ic];
+            }
+        }
+        cplx Amn = 0, Bmn = 0, Cmn = 0, Dmn = 0;
+        cplx *Xcc = c->X[0] + (size_t)mm * nb2, *Xcs = c->X[1] + (size_t)mm * nb2, *Xsc = c->X[2] + (size_t)mm * nb2, *Xss = c->X[3] + (size_t)mm * nb2;
+        const cplx *X0cc = c->X0[0] + (size_t)mm * nb2, *X0cs = c->X0[1] + (size_t)mm * nb2, *X0sc = c->X0[2] + (size_t)mm * nb2, *X0ss = c->X0[3] + (size_t)mm * nb2;
+        for (int ic = 0; ic < Ncv; ic++)
+            for (int jc = ic; jc < Ncv; jc++) {
+                const int d = ic * Ncv + jc, od = jc * Ncv + ic;
+                if (ic == jc) { Amn = A[(size_t)mm * nb2 + 0]; Dmn = D[(size_t)mm * nb2 + 0]; }
+                else { Amn = A[(size_t)mm * nb2 + d]; Bmn = B[(size_t)mm * nb2 + d]; Cmn = Cc[(size_t)mm * nb2 + d]; Dmn = D[(size_t)mm * nb2 + d]; }
+                Xcc[d] = (Amn * f[0][d] + Dmn * f[3][d]) - X0cc[d];
+                if (jc != ic) { Xcc[d] += (Bmn * f[1][d] + Cmn * f[2][d]); Xcc[od] = conj(Xcc[d]); }
+                Xcs[d] = (Amn * f[1][d] - Dmn * f[2][d]) - X0cs[d];
+                if (jc != ic) { Xcs[d] += (-Bmn * f[0][d] + Cmn * f[3][d]); Xcs[od] = conj(Xcs[d]); }
+                Xsc[d] = (Amn * f[2][d] - Dmn * f[1][d]) - X0sc[d];
+                if (jc != ic) { Xsc[d] += (Bmn * f[3][d] - Cmn * f[0][d]); Xsc[od] = conj(Xsc[d]); }
+                Xss[d] = (Amn * f[3][d] + Dmn * f[0][d]) - X0ss[d];
+                if (jc != ic) { Xss[d] += (-Bmn * f[2][d] - Cmn * f[1][d]); Xss[od] = conj(Xss[d]); }
+            }
+    }
+}
+
+/* Calculate_X_MPI (Coulomb.h:510-541) + peak correction and renormalised H
+ * (RungeKutta.h:69-138) for one k; returns Hk (full). */
+static void coulomb_hk(const orc_sys *s, long ik, const cplx *P, cplx Hk[ORC_MAXNB][ORC_MAXNB]) {
+    orc_coul *c = s->coul;
+    const int Ncv = s->nb, Ncut = c->p->ncut, nb2 = Ncv * Ncv;
+    const long nk = s->nk;
+    cplx Xk[ORC_MAXNB][ORC_MAXNB];
+    memset(Xk, 0, sizeof Xk);
+    for (int mm = 0; mm < Ncut * Ncut; mm++) {
+        const int m = mm / Ncut, n = mm % Ncut;
+        const double cc = c->cosx[m * nk + ik] * c->cosy[n * nk + ik], cs = c->cosx[m * nk + ik] * c->siny[n * nk + ik];
+        const double sc = c->sinx[m * nk + ik] * c->cosy[n * nk + ik], ss = c->sinx[m * nk + ik] * c->siny[n * nk + ik];
+        for (int ic = 0; ic < Ncv; ic++)
+            for (int jc = 0; jc < Ncv; jc++) {
+                const size_t d = (size_t)mm * nb2 + ic * Ncv + jc;
+                cplx alpha = c->X[0][d] * cc + c->X[1][d] * cs + c->X[2][d] * sc + c->X[3][d] * ss;
+                Xk[jc][ic] -= alpha;
+            }
+    }
+    const cplx *Pk = &P[ik * nb2], *P0k = &c->PW0[ik * nb2];
+    const cplx *scr = (const cplx *)c->p->screen;
+    if (c->n_cond > pow(10, -20)) {
+        for (int ic = 0; ic < Ncv; ic++)
+            for (int jc = ic; jc < Ncv; jc++) {
+                const int idx = (ic != jc) ? ic * Ncv + jc : 0;
+                Xk[ic][jc] -= conj(scr[idx]) * (Pk[jc * Ncv + ic] - P0k[jc * Ncv + ic]);
+                if (ic != jc) Xk[jc][ic] = conj(Xk[ic][jc]);
+            }
+    }
+    const cplx *H0 = &s->H[ik * nb2];
+    cplx *Hr = &c->Hren[ik * nb2];
+    for (int ic = 0; ic < Ncv; ic++) {
+        Xk[ic][ic] = creal(Xk[ic][ic]);
+        for (int jc = ic; jc < Ncv; jc++) {
+            Hk[ic][jc] = H0[ic * Ncv + jc] + Xk[ic][jc];
+            if (ic == jc) Hk[ic][ic] += c->EH[ic];
+            if (ic != jc) Hk[jc][ic] = conj(Hk[ic][jc]);
+            Hr[ic * Ncv + jc] = Hk[ic][jc];
+            if (ic != jc) Hr[jc * Ncv + ic] = Hk[jc][ic];
+        }
+    }
+}
+
+/* Hermitian eigenvectors by cyclic Jacobi (eig_sym replacement for the
+ * renormalised-band populations); columns of V, ascending eigenvalues. */
+static void herm_eig(int n, const cplx *H, cplx *V) {
+    cplx A[ORC_MAXNB * ORC_MAXNB];
+    memcpy(A, H, sizeof(cplx) * n * n);
+    for (int i = 0; i < n * n; i++) V[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n * n; i++) { tot += creal(A[i] * conj(A[i])); if (i / n != i % n) off += creal(A[i] * conj(A[i])); }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double mag = cabs(A[p * n + q]);
+                if (mag == 0) continue;
+                const cplx e = A[p * n + q] / mag;
+                const double th = 0.5 * (creal(A[q * n + q]) - creal(A[p * n + p])) / mag;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                const double cs = 1 / sqrt(t * t + 1), sn = t * cs;
+                for (int k = 0; k < n; k++) {
+                    cplx a = A[k * n + p], b = A[k * n + q];
+                    A[k * n + p] = cs * a - sn * conj(e) * b;
+                    A[k * n + q] = sn * e * a + cs * b;
+                    a = V[k * n + p]; b = V[k * n + q];
+                    V[k * n + p] = cs * a - sn * conj(e) * b;
+                    V[k * n + q] = sn * e * a + cs * b;
+                }
+                for (int k = 0; k < n; k++) {
+                    cplx a = A[p * n + k], b = A[q * n + k];
+                    A[p * n + k] = cs * a - sn * e * b;
+                    A[q * n + k] = sn * conj(e) * a + cs * b;
+                }
+            }
+    }
+    /* sort ascending */
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++)
+            if (creal(A[j * n + j]) < creal(A[i * n + i])) {
+                cplx t = A[i * n + i]; A[i * n + i] = A[j * n + j]; A[j * n + j] = t;
+                for (int k = 0; k < n; k++) { t = V[k * n + i]; V[k * n + i] = V[k * n + j]; V[k * n + j] = t; }
+            }
+}
+
 /* EF[0] = pump sum, EF[1] = half probe */
 static void rhs(const orc_sys *s, const cplx *P, cplx *Pv, const double EF[2][3], int E_non0) {
     const int Ncv = s->nb, Nch = s->nch;
+    if (s->coul) coulomb_coefficients(s, P);
 #pragma omp parallel for schedule(static)
     for (long ik = 0; ik < s->nk; ik++) {
         const cplx *Pk = &P[ik * Ncv * Ncv];
         cplx *Pvk = &Pv[ik * Ncv * Ncv];
         const cplx *Hk0 = &s->H[ik * Ncv * Ncv], *Dk = &s->D[ik * Ncv * Ncv * 3], *Uk = &s->U[ik * Ncv * Ncv];
         cplx Hk[ORC_MAXNB][ORC_MAXNB];
-        for (int ic = 0; ic < Ncv; ic++)
-            for (int jc = ic; jc < Ncv; jc++) {
-                Hk[ic][jc] = Hk0[ic * Ncv + jc];
-                if (ic != jc) Hk[jc][ic] = conj(Hk[ic][jc]);
-            }
+        if (s->coul) {
+            coulomb_hk(s, ik, P, Hk);
+        } else {
+            for (int ic = 0; ic < Ncv; ic++)
+                for (int jc = ic; jc < Ncv; jc++) {
+                    Hk[ic][jc] = Hk0[ic * Ncv + jc];
+                    if (ic != jc) Hk[jc][ic] = conj(Hk[ic][jc]);
+                }
+        }
 #define DK(x, i, j) Dk[((i) * Ncv + (j)) * 3 + (x)]
 #define PK(i, j) Pk[(i) * Ncv + (j)]
         cplx alpha[ORC_MAXNB];
@@ -514,9 +692,11 @@ typedef struct orc_tables {
     const int32_t *nbr;
     const double *wshell, *bvec;
     double wx2;
+    const orc_coul_params *coul;   /* NULL: no Coulomb term */
 } orc_tables;
 
 static void make_sys(const orc_tables *t, orc_sys *s) {
+    s->coul = NULL;
     s->nk = t->nk; s->nb = t->nb; s->nch = t->nch; s->ncc = t->ncc; s->n_nbr = t->n_nbr;
     s->H = (const cplx *)t->H; s->D = (const cplx *)t->D; s->U = (const cplx *)t->U; s->T = t->T;
     s->nbr = t->nbr; s->wshell = t->wshell; s->bvec = t->bvec; s->wx2 = t->wx2;
@@ -526,11 +706,111 @@ static void make_sys(const orc_tables *t, orc_sys *s) {
             if (fabs(t->T[ic * t->nb + jc]) > 1e-24) { s->diss0[s->n_diss] = ic; s->diss1[s->n_diss] = jc; s->n_diss++; }
 }
 
+/* Coulomb state of a run: trig tables (integrationWeight.h:139-175), equilibrium
+ * X0 and Hartree shift from P0 (main_MPI.cpp:276-321), P_Wannier_0 = P0, the
+ * renormalised H starting as the bare H (main_MPI.cpp:214), n_cond = 0. */
+static orc_coul *coul_init(orc_sys *s, const orc_coul_params *p, const cplx *P0) {
+    orc_coul *c = calloc(1, sizeof(orc_coul));
+    const long nk = s->nk;
+    const int Ncut = p->ncut, nb2 = s->nb * s->nb;
+    c->p = p;
+    c->cosx = malloc(sizeof(double) * Ncut * nk); c->sinx = malloc(sizeof(double) * Ncut * nk);
+    c->cosy = malloc(sizeof(double) * Ncut * nk); c->siny = malloc(sizeof(double) * Ncut * nk);
+    for (int m = 0; m < Ncut; m++)
+        for (long ik = 0; ik < nk; ik++) {
+            double mkx = 2 * M_PI * m * (p->kpt[ik * 3 + p->dir1]);
+            double nky = 2 * M_PI * m * (p->kpt[ik * 3 + p->dir2]);
+            c->cosx[m * nk + ik] = cos(mkx); c->sinx[m * nk + ik] = sin(mkx);
+            c->cosy[m * nk + ik] = cos(nky); c->siny[m * nk + ik] = sin(nky);
+        }
+    for (int q = 0; q < 4; q++) {
+        c->X[q] = calloc((size_t)Ncut * Ncut * nb2, sizeof(cplx));
+        c->X0[q] = calloc((size_t)Ncut * Ncut * nb2, sizeof(cplx));
+    }
+    c->PW0 = malloc(sizeof(cplx) * nk * nb2);
+    memcpy(c->PW0, P0, sizeof(cplx) * nk * nb2);
+    c->Hren = malloc(sizeof(cplx) * nk * nb2);
+    memcpy(c->Hren, s->H, sizeof(cplx) * nk * nb2);
+    s->coul = c;
+    coulomb_coefficients(s, P0);
+    for (int q = 0; q < 4; q++) memcpy(c->X0[q], c->X[q], sizeof(cplx) * Ncut * Ncut * nb2);
+    for (int i = 0; i < s->nb; i++) c->EH0[i] = c->EH[i];
+    c->n_cond = 0;
+    return c;
+}
+
+static void coul_free(orc_coul *c) {
+    if (!c) return;
+    free(c->cosx); free(c->sinx); free(c->cosy); free(c->siny);
+    for (int q = 0; q < 4; q++) { free(c->X[q]); free(c->X0[q]); }
+    free(c->PW0); free(c->Hren); free(c);
+}
+
+/* Test hook: X(k) of a state P with X0 = 0 and no peak correction:
+ * Xk [nk][nb][nb] complex as Calculate_X_MPI returns it (X_{jc,ic} storage),
+ * EH [nb] the Hartree shift. */
+void orc_coulomb_x(const orc_tables *t, const double *P, double *Xk, double *EH) {
+    orc_sys s;
+    make_sys(t, &s);
+    const cplx *Pc = (const cplx *)P;
+    orc_coul *c = calloc(1, sizeof(orc_coul));
+    const long nk = s.nk;
+    const int Ncut = t->coul->ncut, nb = s.nb, nb2 = nb * nb;
+    orc_coul_params pp = *t->coul;
+    c->p = &pp;
+    c->cosx = malloc(sizeof(double) * Ncut * nk); c->sinx = malloc(sizeof(double) * Ncut * nk);
+    c->cosy = malloc(sizeof(double) * Ncut * nk); c->siny = malloc(sizeof(double) * Ncut * nk);
+    for (int m = 0; m < Ncut; m++)
+        for (long ik = 0; ik < nk; ik++) {
+            double mkx = 2 * M_PI * m * (pp.kpt[ik * 3 + pp.dir1]), nky = 2 * M_PI * m * (pp.kpt[ik * 3 + pp.dir2]);
+            c->cosx[m * nk + ik] = cos(mkx); c->sinx[m * nk + ik] = sin(mkx);
+            c->cosy[m * nk + ik] = cos(nky); c->siny[m * nk + ik] = sin(nky);
+        }
+    for (int q = 0; q < 4; q++) { c->X[q] = calloc((size_t)Ncut * Ncut * nb2, sizeof(cplx)); c->X0[q] = calloc((size_t)Ncut * Ncut * nb2, sizeof(cplx)); }
+    c->PW0 = calloc((size_t)nk * nb2, sizeof(cplx));
+    c->Hren = calloc((size_t)nk * nb2, sizeof(cplx));
+    s.coul = c;
+    coulomb_coefficients(&s, Pc);
+    cplx *out = (cplx *)Xk;
+    for (long ik = 0; ik < nk; ik++) {
+        cplx X[ORC_MAXNB][ORC_MAXNB];
+        memset(X, 0, sizeof X);
+        for (int mm = 0; mm < Ncut * Ncut; mm++) {
+            const int m = mm / Ncut, n = mm % Ncut;
+            const double cc = c->cosx[m * nk + ik] * c->cosy[n * nk + ik], cs = c->cosx[m * nk + ik] * c->siny[n * nk + ik];
+            const double sc = c->sinx[m * nk + ik] * c->cosy[n * nk + ik], ss = c->sinx[m * nk + ik] * c->siny[n * nk + ik];
+            for (int ic = 0; ic < nb; ic++)
+                for (int jc = 0; jc < nb; jc++) {
+                    const size_t d = (size_t)mm * nb2 + ic * nb + jc;
+                    X[jc][ic] -= c->X[0][d] * cc + c->X[1][d] * cs + c->X[2][d] * sc + c->X[3][d] * ss;
+                }
+        }
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) out[(ik * nb + i) * nb + j] = X[i][j];
+    }
+    for (int i = 0; i < nb; i++) EH[i] = c->EH[i];
+    coul_free(c);
+}
+
 void orc_rhs(const orc_tables *t, const double *P, double *Pv, const double *EF6, int E_non0) {
     orc_sys s;
     make_sys(t, &s);
     double EF[2][3] = {{EF6[0], EF6[1], EF6[2]}, {EF6[3], EF6[4], EF6[5]}};
     rhs(&s, (const cplx *)P, (cplx *)Pv, EF, E_non0);
+}
+
+/* Test hook: the right-hand side with the Coulomb term of a run started from P0
+ * (equilibrium X0, EH0 and P_Wannier_0 taken from P0) at state P, with the
+ * peak-correction gate n_cond given. */
+void orc_rhs_coulomb(const orc_tables *t, const double *P0, const double *P, double *Pv, const double *EF6, int E_non0,
+                     double n_cond) {
+    orc_sys s;
+    make_sys(t, &s);
+    orc_coul *c = coul_init(&s, t->coul, (const cplx *)P0);
+    c->n_cond = n_cond;
+    double EF[2][3] = {{EF6[0], EF6[1], EF6[2]}, {EF6[3], EF6[4], EF6[5]}};
+    rhs(&s, (const cplx *)P, (cplx *)Pv, EF, E_non0);
+    coul_free(c);
 }
 
 /* ------------------------------------------------------------------ observables */
@@ -550,6 +830,31 @@ static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, 
     for (long c = 0; c < nch; c++) {
         for (long ik = c * CH; ik < (c + 1) * CH && ik < s->nk; ik++) {
             const cplx *Pk = &P[ik * Ncv * Ncv], *Uk = &s->U[ik * Ncv * Ncv], *Hk = &s->H[ik * Ncv * Ncv], *Dk = &s->D[ik * Ncv * Ncv * 3];
+            cplx Uexc[ORC_MAXNB * ORC_MAXNB], GEk[ORC_MAXNB * ORC_MAXNB * 3];
+            const cplx *GEp = oc->gradH ? &((const cplx *)oc->gradH)[ik * Ncv * Ncv * 3] : NULL;
+            if (s->coul) {
+                /* populations in the eigenbasis of the renormalised H (Observables_MPI.h:286-304),
+                 * gradient and Current2 with it (if_resolution.cpp:125-192) */
+                const cplx *Hr = &s->coul->Hren[ik * Ncv * Ncv];
+                cplx V[ORC_MAXNB * ORC_MAXNB];
+                herm_eig(Ncv, Hr, V);
+                for (int a = 0; a < Ncv; a++)
+                    for (int b = 0; b < Ncv; b++) Uexc[a * Ncv + b] = conj(V[b * Ncv + a]);
+                Uk = Uexc;
+                Hk = Hr;
+                memset(GEk, 0, sizeof GEk);
+                for (int x = 0; x < 3; x++)
+                    for (int ic = 0; ic < Ncv; ic++)
+                        for (int jc = ic; jc < Ncv; jc++) {
+                            cplx g = 0;
+                            for (int n = 0; n < s->n_nbr; n++)
+                                g += s->wshell[n] * s->bvec[n * 3 + x] *
+                                     (s->coul->Hren[(long)s->nbr[ik * s->n_nbr + n] * Ncv * Ncv + ic * Ncv + jc] - Hr[ic * Ncv + jc]);
+                            GEk[(ic * Ncv + jc) * 3 + x] = g;
+                            GEk[(jc * Ncv + ic) * 3 + x] = conj(g);
+                        }
+                if (oc->gradH) GEp = GEk;
+            }
             /* get_P_in_dia_vect (InitParameters.h:264-335): P_sym from the upper triangle */
             cplx Ps[ORC_MAXNB][ORC_MAXNB], Mk[ORC_MAXNB][ORC_MAXNB];
             for (int ic = 0; ic < Ncv; ic++)
@@ -571,8 +876,8 @@ static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, 
             /* Current1 / Current2 / transient absorption */
             for (int ic = 0; ic < Ncv; ic++)
                 for (int jc = 0; jc < Ncv; jc++) {
-                    if (oc->gradH)
-                        for (int x = 0; x < 3; x++) pj1[c * 3 + x] += Pk[ic * Ncv + jc] * ((const cplx *)oc->gradH)[((ik * Ncv + ic) * Ncv + jc) * 3 + x];
+                    if (GEp)
+                        for (int x = 0; x < 3; x++) pj1[c * 3 + x] += Pk[ic * Ncv + jc] * GEp[(ic * Ncv + jc) * 3 + x];
                     for (int kc = 0; kc < Ncv; kc++)
                         for (int x = 0; x < 3; x++) pj2[c * 3 + x] += Pk[ic * Ncv + jc] * Hk[ic * Ncv + kc] * conj(Dk[(jc * Ncv + kc) * 3 + x]);
                 }
@@ -587,6 +892,12 @@ static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, 
         for (long c = 0; c < nch; c++) sacc += ppop[c * Ncv + ic];
         double v = sacc * nktot_inv;
         pop[ic] = (ic < s->ncc) ? 1.0 - v : v;
+    }
+    if (s->coul) {  /* n_cond: largest printed population (Observables_MPI.h:413-426) */
+        double nc = 0;
+        for (int ic = 0; ic < Ncv; ic++)
+            if (pop[ic] > nc) nc = pop[ic];
+        s->coul->n_cond = nc;
     }
     for (int x = 0; x < 3; x++) {
         cplx a = 0, b = 0, t = 0;
@@ -743,6 +1054,7 @@ static void taylor_tau_inverse(int o, double dt, double *tinv) {
 long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, double *P0io, double *rows, long max_rows) {
     orc_sys s;
     make_sys(t, &s);
+    if (t->coul) coul_init(&s, t->coul, (const cplx *)P0io);
     const long nk = t->nk;
     const int nb = t->nb;
     const size_t nel = (size_t)nk * nb * nb;
@@ -820,6 +1132,7 @@ long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, d
         time_loop += dt;
     }
     free(P1); free(P2); free(Pv); free(hist);
+    coul_free(s.coul);
     return nrow;
 }
 

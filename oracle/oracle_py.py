@@ -38,7 +38,12 @@ class OTB(C.Structure):
 class OTables(C.Structure):
     _fields_ = [("nk", C.c_long), ("nb", C.c_int), ("nch", C.c_int), ("ncc", C.c_int), ("n_nbr", C.c_int),
                 ("H", _dp), ("D", _dp), ("U", _dp), ("T", _dp), ("nbr", _ip), ("wshell", _dp), ("bvec", _dp),
-                ("wx2", C.c_double)]
+                ("wx2", C.c_double), ("coul", C.c_void_p)]
+
+
+class OCoul(C.Structure):
+    _fields_ = [("ncut", C.c_int), ("dir1", C.c_int), ("dir2", C.c_int), ("kpt", _dp), ("A", _dp), ("B", _dp),
+                ("C", _dp), ("D", _dp), ("screen", _dp), ("vh", _dp)]
 
 
 class OObs(C.Structure):
@@ -67,6 +72,8 @@ def _load():
     lib.orc_rhs.argtypes = [C.POINTER(OTables), _dp, _dp, _dp, C.c_int]
     lib.orc_run.argtypes = [C.POINTER(OTables), C.POINTER(OObs), C.POINTER(ORun), _dp, _dp, C.c_long]
     lib.orc_run.restype = C.c_long
+    lib.orc_coulomb_x.argtypes = [C.POINTER(OTables), _dp, _dp, _dp]
+    lib.orc_rhs_coulomb.argtypes = [C.POINTER(OTables), _dp, _dp, _dp, _dp, C.c_int, C.c_double]
     lib.orc_set_threads.argtypes = [C.c_int]
     lib.orc_max_threads.restype = C.c_int
     return lib
@@ -169,7 +176,7 @@ def rho0(U, nbands):
 class System:
     """Tables of one problem in reference layout (complex arrays)."""
 
-    def __init__(self, H, D, U, T, nbr, shell_w, bvec, nch, ncc, wx2):
+    def __init__(self, H, D, U, T, nbr, shell_w, bvec, nch, ncc, wx2, coulomb=None):
         self.H, self.D, self.U = (np.ascontiguousarray(x, dtype=np.complex128) for x in (H, D, U))
         self.T = np.ascontiguousarray(T, dtype=np.float64)
         self.nbr = np.ascontiguousarray(nbr, dtype=np.int32)
@@ -178,13 +185,44 @@ class System:
         self.nk, self.nb = self.H.shape[0], self.H.shape[1]
         self.t = OTables(self.nk, self.nb, nch, ncc, self.nbr.shape[1], _d(self.H.view(np.float64)),
                          _d(self.D.view(np.float64)), _d(self.U.view(np.float64)), _d(self.T),
-                         self.nbr.ctypes.data_as(_ip), _d(self.shell_w), _d(self.bvec), wx2)
+                         self.nbr.ctypes.data_as(_ip), _d(self.shell_w), _d(self.bvec), wx2, None)
+        self._coul = None
+        if coulomb is not None:
+            self.set_coulomb(**coulomb)
+
+    def set_coulomb(self, ncut, kpt, A, B, C_, D, screen, vh, dir1=1, dir2=2):
+        """Coulomb mean-field tables (Fourier_Coulomb.cpp:505-622 layout)."""
+        self._ckeep = [np.ascontiguousarray(kpt, dtype=np.float64)] + [
+            np.ascontiguousarray(x, dtype=np.complex128) for x in (A, B, C_, D, screen, vh)]
+        k, a, b, c, d, sc, v = self._ckeep
+        self._coul = OCoul(ncut, dir1, dir2, _d(k), _d(a.view(np.float64)), _d(b.view(np.float64)),
+                           _d(c.view(np.float64)), _d(d.view(np.float64)), _d(sc.view(np.float64)),
+                           _d(v.view(np.float64)))
+        self.t.coul = C.cast(C.pointer(self._coul), C.c_void_p)
+
+    def coulomb_x(self, P):
+        """X(k) [nk][nb][nb] and the Hartree shift of state P (equilibrium X0 = 0)."""
+        P = np.ascontiguousarray(P, dtype=np.complex128)
+        X = np.zeros_like(P)
+        EH = np.zeros(self.nb)
+        lib.orc_coulomb_x(C.byref(self.t), _d(P.view(np.float64)), _d(X.view(np.float64)), _d(EH))
+        return X, EH
 
     def rhs(self, P, EF6, E_non0=True):
         P = np.ascontiguousarray(P, dtype=np.complex128)
         Pv = np.zeros_like(P)
         EF6 = np.ascontiguousarray(EF6, dtype=np.float64)
         lib.orc_rhs(C.byref(self.t), _d(P.view(np.float64)), _d(Pv.view(np.float64)), _d(EF6), 1 if E_non0 else 0)
+        return Pv
+
+    def rhs_coulomb(self, P0, P, EF6, E_non0=True, n_cond=0.0):
+        """Right-hand side with the Coulomb term of a run started from P0, at state P."""
+        P0 = np.ascontiguousarray(P0, dtype=np.complex128)
+        P = np.ascontiguousarray(P, dtype=np.complex128)
+        Pv = np.zeros_like(P)
+        EF6 = np.ascontiguousarray(EF6, dtype=np.float64)
+        lib.orc_rhs_coulomb(C.byref(self.t), _d(P0.view(np.float64)), _d(P.view(np.float64)), _d(Pv.view(np.float64)),
+                            _d(EF6), 1 if E_non0 else 0, float(n_cond))
         return Pv
 
     def run(self, P0, stepper, order, dt, t_start, nsteps, it_res, pumps, probe, gradH=None, j_norm=0.0, jacobian=0.0,

@@ -9,6 +9,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 AB5 = os.path.join(GOLDEN, "ab5")
+COULOMB = os.path.join(GOLDEN, "coulomb")
+COULOMB_LABEL = "G_num502657_Ncut20_"
 
 from edus_amd import native as N  # noqa: E402
 from oracle import oracle_py as O  # noqa: E402
@@ -23,7 +25,44 @@ def ab5_fixture():
     return {name: load_cols(os.path.join(AB5, name + ".txt")) for name in ("J1", "J2", "Losses", "TransientAbs", "EF")}
 
 
-def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True):
+def read_coulomb_files(directory, label, nb, ncut):
+    """The coefficient files of Fourier_Coulomb.cpp:505-622, parsed with numpy
+    (independently of the host's reader)."""
+    def cx(name, rows):
+        a = np.loadtxt(os.path.join(directory, name + label + ".txt"), ndmin=2)[:rows]
+        return a[:, 0::2] + 1j * a[:, 1::2]
+    out = {q: cx(q + "_coeff", ncut * ncut) for q in "ABCD"}
+    out["screen"] = cx("Screen_const_", nb * nb)[:, 0]
+    out["vh"] = np.loadtxt(os.path.join(directory, "V_Hartree_" + label + ".txt"), ndmin=2)[:nb, :nb].astype(np.complex128)
+    return out
+
+
+def coulomb_input(d, nk=24, stepper="", dt=0.1, nsteps=400, res=2, pump_int=5e12, probe_int=1e9, cycles=1):
+    """Input of the reference's hBN Coulomb example (Haldane_CoreBN_Nedge, Ncut 20,
+    coefficient files read from Output/Coulomb) on a small grid with constant dt."""
+    t0 = -0.21 / O.TIME_AU_FS
+    t_fin = t0 + (nsteps - 0.5) * dt
+    text = ("NEWINPUT\n\ntdse{\ntightbinding Haldane_CoreBN_Nedge gap=  7.25 t1=  -2.3 t2= 0 phi0= 0.0\n"
+            f"dt {dt!r} au\nt_fin {t_fin!r} au\n{stepper}}}\n\n"
+            "Coulomb{\nqTF 0.01\nNcut 20\nepsilon_static 1.0\nRead_Coulomb_from_files\n"
+            f"labelInput {COULOMB_LABEL}\n\nRytova_Keldysh\n}}\n\n"
+            f"nkPT{{\n1 {nk} {nk}\n}}\n\ndecoherence{{\ncorehole 0.00396892 au\n}}\n\n"
+            f"laserpump sin2  {{\ncycles {cycles}\nwavelength 206.66666666666666 nm\n"
+            f"intensity {pump_int!r} wcm2\npolarization 0 1 1\n}}\n\n"
+            "laserprobe gaussian {\nsigma 0.08 fs\ndelay  0.0 fs\n"
+            f"intensity  {probe_int!r}\nfrequency 15.06\npolarization 0 1 0\n}}\n\n"
+            f"observables{{\nit_resolution {res}\nTAbs\nCurrent\n}}\n")
+    os.makedirs(os.path.join(d, "Output"), exist_ok=True)
+    link = os.path.join(d, "Output", "Coulomb")
+    if not os.path.exists(link):
+        os.symlink(COULOMB, link)
+    p = os.path.join(d, "input.txt")
+    with open(p, "w") as f:
+        f.write(text)
+    return p
+
+
+def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, coulomb=False):
     """Oracle tables for a setup: k-grid, shells, neighbours and model from the oracle's
     own restatement (tight binding), the setup's parameters and pulses."""
     info = setup.info
@@ -49,6 +88,10 @@ def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True):
         D = setup.array("D", True).reshape(-1, nb, nb, 3)
     T = setup.array("T").reshape(nb, nb)
     sys_ = O.System(H, D, U, T, nbr, wshell, bvec, info["nch"], info["ncc"], info["wx2"])
+    if coulomb:
+        ncut = info["ncut"]
+        cf = read_coulomb_files(COULOMB, COULOMB_LABEL, nb, ncut)
+        sys_.set_coulomb(ncut, kpt, cf["A"], cf["B"], cf["C"], cf["D"], cf["screen"], cf["vh"])
     return {"sys": sys_, "kpt": kpt, "bvec": bvec, "w": w, "sizes": sizes, "res": res, "nbr": nbr, "H": H, "U": U, "D": D,
             "T": T}
 

@@ -171,3 +171,48 @@ def test_neighbours_closed_form_equals_bruteforce(tmp_path, n1, n2):
     # every neighbour list is a permutation of the grid per direction
     for n in range(s.nbr().shape[1]):
         assert len(np.unique(s.nbr()[:, n])) == len(kpt)
+
+
+def test_coulomb_term_equals_bruteforce_convolution():
+    """The separable Fourier form of the exchange term (Coulomb.h:273-541) is the
+    convolution X_ij(k) = -(1/N) sum_k' sum_mn (1+corr_mn) V_mn(k-k') P_ij(k') with
+    V_mn(d) = A cos(m dx)cos(n dy) + D sin(m dx)sin(n dy) - B cos(m dx)sin(n dy)
+    - C sin(m dx)cos(n dy) (B, C on off-diagonal elements; the diagonal uses the
+    element-0 coefficients), stored as X[j][i]; the Hartree shift is
+    Re V_H . <P_jj>.  Checked by brute force on an 8x6 grid."""
+    rng = np.random.default_rng(21)
+    N1, N2, nb, ncut = 8, 6, 2, 3
+    kpt = O.kgrid(1, N1, N2)
+    nk = len(kpt)
+    X = rng.normal(size=(nk, nb, nb)) + 1j * rng.normal(size=(nk, nb, nb))
+    Pm = X + np.conj(np.swapaxes(X, 1, 2))
+    A, B, Cc, D = (rng.normal(size=(ncut * ncut, nb * nb)) + 1j * rng.normal(size=(ncut * ncut, nb * nb)) for _ in range(4))
+    screen = rng.normal(size=nb * nb) + 0j
+    vh = rng.normal(size=(nb, nb)) + 0j
+    H = np.zeros((nk, nb, nb), dtype=np.complex128)
+    sysd = O.System(H, np.zeros((nk, nb, nb, 3)), np.tile(np.eye(nb), (nk, 1, 1)), np.zeros((nb, nb)),
+                    np.zeros((nk, 1), dtype=np.int32), np.zeros(1), np.zeros((1, 3)), 0, 1, 0.0)
+    sysd.set_coulomb(ncut, kpt, A, B, Cc, D, screen, vh)
+    Xk, EH = sysd.coulomb_x(Pm)
+    ref = np.zeros((nk, nb, nb), dtype=np.complex128)
+    dx = 2 * np.pi * (kpt[:, None, 1] - kpt[None, :, 1])
+    dy = 2 * np.pi * (kpt[:, None, 2] - kpt[None, :, 2])
+    for i in range(nb):
+        for j in range(i, nb):
+            e = i * nb + j
+            acc = np.zeros(nk, dtype=np.complex128)
+            for m in range(ncut):
+                for n in range(ncut):
+                    mm = m * ncut + n
+                    corr = -4 * np.pi ** 2 * (m * m + n * n) / (nk * 27)
+                    if i == j:
+                        V = A[mm, 0] * np.cos(m * dx) * np.cos(n * dy) + D[mm, 0] * np.sin(m * dx) * np.sin(n * dy)
+                    else:
+                        V = (A[mm, e] * np.cos(m * dx) * np.cos(n * dy) + D[mm, e] * np.sin(m * dx) * np.sin(n * dy)
+                             - B[mm, e] * np.cos(m * dx) * np.sin(n * dy) - Cc[mm, e] * np.sin(m * dx) * np.cos(n * dy))
+                    acc += (1 + corr) * (V @ Pm[:, i, j]) / nk
+            ref[:, j, i] = -acc
+            if i != j:
+                ref[:, i, j] = np.conj(-acc)
+    np.testing.assert_allclose(Xk, ref, rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+    np.testing.assert_allclose(EH, np.real(vh) @ np.real(np.mean(np.einsum("kii->ki", Pm), axis=0)), rtol=1e-13)
