@@ -956,6 +956,33 @@ __global__ void k_obs_final(const double *part, int nparts, int nv, double *out)
     }
 }
 
+// ================================================================ adaptive dt
+// Diagonal of the state before a step (P_W_prev, RungeKutta.cpp:1-9,139-147 and
+// Taylor_DE_solver_deriv_Polynom.cpp:75-84).  Element e of k sits at k*sk + e*se.
+__global__ void k_diag_save(const double *Y, double *DG, int nk, int nb, size_t sk, size_t se) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    for (int i = 0; i < nb; i++) DG[(size_t)i * nk + k] = Y[k * sk + i * se];
+}
+
+// Block maxima of the step-size measure of dyn_dt_evolution.cpp:74-87:
+// 2 |P_ii(before) - P_ii(after)| / (cond + |P_ii(after)| + 1e-7), cond = P_cond_max + n_cond.
+__global__ __launch_bounds__(256) void k_diag_dp(const double *Y, const double *DG, int nk, int nb, size_t sk, size_t se,
+                                                 double cond, double *part) {
+    __shared__ double red[4];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (k < nk)
+        for (int i = 0; i < nb; i++) {
+            const double y = Y[k * sk + i * se];
+            m = fmax(m, 2. * fabs(DG[(size_t)i * nk + k] - y) / (cond + fabs(y) + 1.0e-7));
+        }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
 // ================================================================ Wannier contraction
 // Out[k][m] = sum_R phase(k,R) Val[R][m], Val = [H | r_x | r_y | r_z] per R.
 // phase(k,R) = prod_l (-1)^{R_l} w_l[(i_l R_l) mod N_l] for N_l > 1 (k_l = -1/2 + i_l/N_l).
@@ -1057,6 +1084,8 @@ struct edus_ctx {
     double *cX = nullptr, *cX0 = nullptr, *cA = nullptr, *cB = nullptr, *cC = nullptr, *cD = nullptr, *cvh = nullptr;
     double *cEH = nullptr, *cEH0 = nullptr, *cT = nullptr, *cPW0 = nullptr, *cHr = nullptr, *cscreen = nullptr;
     double *cUexc = nullptr, *cGE = nullptr, *cQ = nullptr, *dwb = nullptr;
+    // adaptive dt: diagonal before the step, block maxima
+    double *DG = nullptr, *dpart = nullptr;
 };
 
 namespace {
@@ -1452,36 +1481,78 @@ static int finish_step(edus_ctx *c, int launches) {
     return 0;
 }
 
+// One RK4 step of size dt from the four table rows at `tab` (RungeKutta.cpp:11-150).
+static int rk4_step(edus_ctx *c, const double *tab, int rs, double dt) {
+    const double dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
+    StageArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.dt = dt;
+    a.tab = tab;
+    a.Y = c->P0; a.P0 = c->P0; a.ACC = c->ACC; a.OUT = c->YA; a.ca = dt6; a.cb = dt2;
+    int rc;
+    if ((rc = launch_stage<RK1>(c, a))) return rc;
+    a.tab += rs; a.Y = c->YA; a.OUT = c->YB; a.ca = dt3; a.cb = dt2;
+    if ((rc = launch_stage<RK2>(c, a))) return rc;
+    a.tab += rs; a.Y = c->YB; a.OUT = c->YA; a.ca = dt3; a.cb = dt;
+    if ((rc = launch_stage<RK3>(c, a))) return rc;
+    a.tab += rs; a.Y = c->YA; a.OUT = c->P0; a.ca = dt6;
+    return launch_stage<RK4S>(c, a);
+}
+
+// Fields [nsteps][3][6] -> table rows [nsteps][4] (stage 3 reuses the midpoint field).
+static void rk4_rows(const edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on, std::vector<double> &rows) {
+    const int rs = 7 + c->n_nbr;
+    rows.assign((size_t)nsteps * 4 * rs, 0.0);
+    static const int evalof[4] = {0, 1, 1, 2};
+    for (int s = 0; s < nsteps; s++)
+        for (int st = 0; st < 4; st++)
+            table_row(c, fields + ((size_t)s * 3 + evalof[st]) * 6, grad_on[s], &rows[((size_t)s * 4 + st) * rs]);
+}
+
 int edus_step_rk4(edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on) {
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->stepper != EDUS_STEPPER_RK4) return set_err(EDUS_ERR_INVALID, "context was created for another stepper");
     if (nsteps < 0 || (nsteps > 0 && (!fields || !grad_on))) return set_err(EDUS_ERR_INVALID, "bad step arguments");
     const int rs = 7 + c->n_nbr;
-    std::vector<double> rows((size_t)nsteps * 4 * rs);
-    static const int evalof[4] = {0, 1, 1, 2};
-    for (int s = 0; s < nsteps; s++)
-        for (int st = 0; st < 4; st++)
-            table_row(c, fields + ((size_t)s * 3 + evalof[st]) * 6, grad_on[s], &rows[((size_t)s * 4 + st) * rs]);
+    std::vector<double> rows;
+    rk4_rows(c, nsteps, fields, grad_on, rows);
     if ((rc = upload_table(c, rows))) return rc;
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    const double dt = c->dt, dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
-    for (int s = 0; s < nsteps; s++) {
-        StageArgs a;
-        std::memset(&a, 0, sizeof a);
-        a.dt = dt;
-        a.tab = c->tab + (size_t)s * 4 * rs;
-        a.Y = c->P0; a.P0 = c->P0; a.ACC = c->ACC; a.OUT = c->YA; a.ca = dt6; a.cb = dt2;
-        if ((rc = launch_stage<RK1>(c, a))) return rc;
-        a.tab += rs; a.Y = c->YA; a.OUT = c->YB; a.ca = dt3; a.cb = dt2;
-        if ((rc = launch_stage<RK2>(c, a))) return rc;
-        a.tab += rs; a.Y = c->YB; a.OUT = c->YA; a.ca = dt3; a.cb = dt;
-        if ((rc = launch_stage<RK3>(c, a))) return rc;
-        a.tab += rs; a.Y = c->YA; a.OUT = c->P0; a.ca = dt6;
-        if ((rc = launch_stage<RK4S>(c, a))) return rc;
-    }
+    for (int s = 0; s < nsteps; s++)
+        if ((rc = rk4_step(c, c->tab + (size_t)s * 4 * rs, rs, c->dt))) return rc;
     c->nstep += nsteps;
     return finish_step(c, 4 * nsteps);
+}
+
+// One AB or Taylor step of size dt from the table row at `tab`
+// (Linear_Multistep_ODE_solvers.cpp:1-93, Taylor_DE_solver_deriv_Polynom.cpp:1-177).
+static int multistep_one(edus_ctx *c, const double *tab, bool taylor, double dt, const double *tinv) {
+    const int o = c->order;
+    StageArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.dt = dt;
+    a.order = o;
+    a.tab = tab;
+    a.HIST = c->HIST;
+    // The derivative of step n lives in slot (-n-1) mod order (ODE_solver.h:66-75
+    // shifts the ring before each step); slots[s] holds f_{n-s}.
+    const long long n = c->nstep;
+    for (int q = 0; q < o; q++) a.slots[q] = (int)((((q - n - 1) % o) + o) % o);
+    a.slot_out = a.slots[0];
+    if (taylor) {
+        std::copy(tinv, tinv + (size_t)(o - 1) * (o - 1), a.tinv);
+        for (int i = 0; i < o; i++) a.dtp[i] = std::pow(dt, i + 1);
+    } else {
+        std::copy(c->coef.begin(), c->coef.end(), a.coef);
+    }
+    double *in = c->cur ? c->YA : c->P0, *out = c->cur ? c->P0 : c->YA;
+    a.Y = in; a.P0 = in; a.OUT = out;
+    const int rc = taylor ? launch_stage<TAY>(c, a) : launch_stage<ABM>(c, a);
+    if (rc) return rc;
+    c->cur ^= 1;
+    c->nstep++;
+    return 0;
 }
 
 static int step_multistep(edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on, bool taylor) {
@@ -1489,36 +1560,13 @@ static int step_multistep(edus_ctx *c, int32_t nsteps, const double *fields, con
     if (rc) return rc;
     if (c->stepper != (taylor ? EDUS_STEPPER_TAYLOR : EDUS_STEPPER_AB)) return set_err(EDUS_ERR_INVALID, "context was created for another stepper");
     if (nsteps < 0 || (nsteps > 0 && (!fields || !grad_on))) return set_err(EDUS_ERR_INVALID, "bad step arguments");
-    const int rs = 7 + c->n_nbr, o = c->order;
+    const int rs = 7 + c->n_nbr;
     std::vector<double> rows((size_t)nsteps * rs);
     for (int s = 0; s < nsteps; s++) table_row(c, fields + (size_t)s * 6, grad_on[s], &rows[(size_t)s * rs]);
     if ((rc = upload_table(c, rows))) return rc;
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    for (int s = 0; s < nsteps; s++) {
-        StageArgs a;
-        std::memset(&a, 0, sizeof a);
-        a.dt = c->dt;
-        a.order = o;
-        a.tab = c->tab + (size_t)s * rs;
-        a.HIST = c->HIST;
-        // The derivative of step n lives in slot (-n-1) mod order (ODE_solver.h:66-75
-        // shifts the ring before each step); slots[s] holds f_{n-s}.
-        const long long n = c->nstep;
-        for (int q = 0; q < o; q++) a.slots[q] = (int)((((q - n - 1) % o) + o) % o);
-        a.slot_out = a.slots[0];
-        if (taylor) {
-            std::copy(c->tinv.begin(), c->tinv.end(), a.tinv);
-            for (int i = 0; i < o; i++) a.dtp[i] = std::pow(c->dt, i + 1);
-        } else {
-            std::copy(c->coef.begin(), c->coef.end(), a.coef);
-        }
-        double *in = c->cur ? c->YA : c->P0, *out = c->cur ? c->P0 : c->YA;
-        a.Y = in; a.P0 = in; a.OUT = out;
-        rc = taylor ? launch_stage<TAY>(c, a) : launch_stage<ABM>(c, a);
-        if (rc) return rc;
-        c->cur ^= 1;
-        c->nstep++;
-    }
+    for (int s = 0; s < nsteps; s++)
+        if ((rc = multistep_one(c, c->tab + (size_t)s * rs, taylor, c->dt, c->tinv.data()))) return rc;
     return finish_step(c, nsteps);
 }
 
@@ -1533,6 +1581,50 @@ int edus_step_taylor(edus_ctx *c, int32_t nsteps, const double *fields, const ui
 static const double *current_state(const edus_ctx *c) {
     if (c->stepper == EDUS_STEPPER_RK4) return c->P0;
     return c->cur ? c->YA : c->P0;
+}
+
+int edus_step_dyn(edus_ctx *c, double dt, const double *fields, uint8_t grad_on, const double *taylor_tau_inv, double cond,
+                  double *max_dp) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!(dt > 0) || !fields || !max_dp) return set_err(EDUS_ERR_INVALID, "bad adaptive step arguments");
+    const bool taylor = c->stepper == EDUS_STEPPER_TAYLOR;
+    if (!taylor && c->stepper != EDUS_STEPPER_RK4)
+        return set_err(EDUS_ERR_UNSUPPORTED, "adaptive steps run RK4 or Taylor contexts");
+    if (taylor && !taylor_tau_inv) return set_err(EDUS_ERR_INVALID, "Taylor adaptive step needs tau^-1");
+    const int nblk = (c->nk + 255) / 256;
+    if (!c->DG) {
+        if ((rc = dalloc(c, &c->DG, (size_t)c->nb * c->nk)) || (rc = dalloc(c, &c->dpart, (size_t)nblk))) return rc;
+    }
+    const int rs = 7 + c->n_nbr;
+    std::vector<double> rows;
+    if (taylor) {
+        rows.resize(rs);
+        table_row(c, fields, grad_on, rows.data());
+    } else {
+        rk4_rows(c, 1, fields, &grad_on, rows);
+    }
+    if ((rc = upload_table(c, rows))) return rc;
+    const size_t sk = c->soa ? 1 : (size_t)c->np, se = c->soa ? (size_t)c->nkp : 1;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_diag_save, dim3(nblk), dim3(256), 0, c->stream, current_state(c), c->DG, c->nk, c->nb, sk, se);
+    HIP_TRY(hipGetLastError());
+    if (taylor) rc = multistep_one(c, c->tab, true, dt, taylor_tau_inv);
+    else {
+        rc = rk4_step(c, c->tab, rs, dt);
+        c->nstep++;
+    }
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_diag_dp, dim3(nblk), dim3(256), 0, c->stream, current_state(c), (const double *)c->DG, c->nk, c->nb, sk, se,
+                       cond, c->dpart);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(nblk);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->dpart, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish_step(c, taylor ? 1 : 4))) return rc;
+    double m = 0.0;
+    for (double v : part) m = std::max(m, v);
+    *max_dp = m;
+    return 0;
 }
 
 int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *tabs, double *pminmax) {

@@ -110,6 +110,8 @@ int edus_setup_get_info(const edus_setup *h, edus_setup_info *info) {
     info->T1 = s.in.T1; info->T2 = s.in.T2; info->Tch = s.in.Tch;
     info->shell_residual = s.sh.residual;
     info->ncut = s.in.coulomb ? s.ncut : 0;
+    info->dyn_dt = s.in.dyn_dt ? 1 : 0;
+    info->eps_step = s.in.epsStepAbs;
     return 0;
 }
 

@@ -178,6 +178,11 @@ struct Setup {
     int fields_per_step() const { return stepper == 0 ? 3 : 1; }
 };
 
+// tau^{-1} of Taylor_DE_solver_deriv_Polynom.cpp:95-123 for the dt history
+// dt_prev [order] (most recent first): t_0 = dt_prev[0], t_i = t_{i-1} + dt_prev[i-1]
+// as written there, tau_ip = 1e7 t_i^(p+1), inverted, then scaled by 1e7.
+std::vector<double> taylor_tau_inverse(const std::vector<double> &dt_prev);
+
 void write_synthetic_tb(const std::string &path, int nw, int rmax, double a_ang, double sigma_h_ev,
                         double sigma_r_ang, double decay_ang, uint64_t seed);
 

@@ -38,15 +38,76 @@ struct Engine {
 void fill_problem(const Setup &s, edus_problem &p);
 void fill_coulomb(const Setup &s, edus_coulomb &c);
 
-int64_t run_loop(const Setup &s, const std::string &outdir, int64_t max_steps, int verbose) {
-    const double tfs = Units::time_au_fs;
-    std::string od = outdir + "/Output/";
-    mkdirs(od);
-    std::ofstream fE(od + "EF.txt"), fL(od + "Losses.txt"), fJ1, fJ2, fT;
-    if (s.in.iCurrent) { fJ1.open(od + "J1.txt"); fJ2.open(od + "J2.txt"); }
-    if (s.in.iTAbs) fT.open(od + "TransientAbs.txt");
-    if (!fE || !fL) throw Error("cannot open output files under " + od);
+// The observable files of one run, written in the reference's stream formats.
+struct Writers {
+    const Setup &s;
+    std::string od;
+    std::ofstream fE, fL, fJ1, fJ2, fT;
+    bool k_printed = false;
+    std::vector<double> pop;
+    double n_cond = 0.0;      // largest printed population (Observables_MPI.h:413-420)
+    double p_cond_max = 0.0;  // largest excited population over k (Observables_MPI.h:330-347)
 
+    Writers(const Setup &st, const std::string &outdir) : s(st), od(outdir + "/Output/"), pop(st.nb) {
+        mkdirs(od);
+        fE.open(od + "EF.txt");
+        fL.open(od + "Losses.txt");
+        if (s.in.iCurrent) { fJ1.open(od + "J1.txt"); fJ2.open(od + "J2.txt"); }
+        if (s.in.iTAbs) fT.open(od + "TransientAbs.txt");
+        if (!fE || !fL) throw Error("cannot open output files under " + od);
+    }
+
+    void observe(edus_ctx *ctx, double tl, const double ef_last[6], int verbose) {
+        const double tfs = Units::time_au_fs;
+        double j1[6], j2[6], ta[6], pm[2];
+        fE << std::setw(15) << std::setprecision(8) << tl * tfs;
+        for (int i = 0; i < 3; i++) fE << std::setw(15) << std::setprecision(8) << ef_last[i];
+        fE << std::endl;
+        int rc = edus_observables(ctx, pop.data(), j1, j2, ta, pm);
+        if (rc != 0) throw Error(std::string("edus_observables: ") + edus_last_error());
+        double trace = -s.ncc;
+        n_cond = 0.0;
+        fL << std::setprecision(16) << tl * tfs << "   ";
+        for (int ic = 0; ic < s.nb; ic++) {
+            double v = pop[ic];
+            trace += v;
+            if (ic < s.ncc) v = 1.0 - v;
+            if (v > n_cond) n_cond = v;
+            fL << std::setprecision(16) << v << "   ";
+        }
+        fL << std::endl;
+        p_cond_max = pm[1];
+        if (verbose && std::fabs(trace) > 1e-13) std::cout << "Preservation of particle num fail P_trace = " << trace << std::endl;
+        if (!k_printed) {
+            std::ofstream fk(od + "k_grid.txt");
+            for (int ik = 0; ik < s.nk; ik++)
+                fk << s.kpt[(size_t)ik * 3] << " " << s.kpt[(size_t)ik * 3 + 1] << " " << s.kpt[(size_t)ik * 3 + 2] << std::endl;
+            k_printed = true;
+        }
+        if (s.in.iCurrent) {
+            fJ1 << std::setw(15) << std::setprecision(8) << tl * tfs;
+            for (int x = 0; x < 3; x++) {
+                fJ1 << std::setw(25) << std::setprecision(15) << std::scientific << j1[2 * x];
+                fJ1 << std::setw(25) << std::setprecision(15) << std::scientific << j1[2 * x + 1];
+            }
+            fJ1 << std::endl;
+            fJ2 << std::setw(15) << std::setprecision(8) << tl * tfs;
+            for (int x = 0; x < 3; x++) fJ2 << std::setw(25) << std::setprecision(15) << std::scientific << -2. * j2[2 * x + 1];
+            fJ2 << std::endl;
+        }
+        if (s.in.iTAbs) {
+            fT << std::setw(15) << std::setprecision(8) << tl * tfs;
+            for (int i = 0; i < 3; i++) fT << std::setw(15) << std::setprecision(8) << ef_last[3 + i];
+            for (int x = 0; x < 3; x++) {
+                fT << std::setw(15) << std::setprecision(8) << ta[2 * x] / (double)s.nk;
+                fT << std::setw(15) << std::setprecision(8) << ta[2 * x + 1] / (double)s.nk;
+            }
+            fT << std::endl;
+        }
+    }
+};
+
+static void make_engine(const Setup &s, Engine &eng) {
     edus_problem p;
     fill_problem(s, p);
     edus_coulomb coul;
@@ -54,67 +115,28 @@ int64_t run_loop(const Setup &s, const std::string &outdir, int64_t max_steps, i
         fill_coulomb(s, coul);
         p.coulomb = &coul;
     }
-    Engine eng;
     if (edus_create(&p, &eng.ctx) != 0) throw Error(std::string("edus_create: ") + edus_last_error());
+}
 
+// Constant dt (Source_Main/const_dt_evolution.cpp).
+static int64_t run_const(const Setup &s, const std::string &outdir, int64_t max_steps, int verbose) {
+    const double tfs = Units::time_au_fs;
+    Writers w(s, outdir);
+    Engine eng;
+    make_engine(s, eng);
     const int nfe = s.fields_per_step();
     const int64_t last = (max_steps >= 0) ? std::min<int64_t>(s.itfi, max_steps - 1) : s.itfi;
     const int res = std::max(1, s.in.it_resolution);
     double ef_last[6] = {0, 0, 0, 0, 0, 0};
-    std::vector<double> pop(s.nb);
-    double j1[6], j2[6], ta[6], pm[2];
     std::vector<double> fields;
     std::vector<uint8_t> flags;
-    bool k_printed = false;
-    const double nktot_d = (double)s.nk;
-
     int64_t it = 0;
     while (it <= last) {
         const double tl = s.times[it];
         if (it % res == 0) {
             if (verbose && it % (10 * res) == 0)
                 std::cout << " it: " << it << " time: " << tl * tfs << " fs, time fin: " << s.in.t_fin * tfs << " fs" << std::endl;
-            fE << std::setw(15) << std::setprecision(8) << tl * tfs;
-            for (int i = 0; i < 3; i++) fE << std::setw(15) << std::setprecision(8) << ef_last[i];
-            fE << std::endl;
-            int rc = edus_observables(eng.ctx, pop.data(), j1, j2, ta, pm);
-            if (rc != 0) throw Error(std::string("edus_observables: ") + edus_last_error());
-            double trace = -s.ncc;
-            fL << std::setprecision(16) << tl * tfs << "   ";
-            for (int ic = 0; ic < s.nb; ic++) {
-                double v = pop[ic];
-                trace += v;
-                if (ic < s.ncc) v = 1.0 - v;
-                fL << std::setprecision(16) << v << "   ";
-            }
-            fL << std::endl;
-            if (verbose && std::fabs(trace) > 1e-13) std::cout << "Preservation of particle num fail P_trace = " << trace << std::endl;
-            if (!k_printed) {
-                std::ofstream fk(od + "k_grid.txt");
-                for (int ik = 0; ik < s.nk; ik++)
-                    fk << s.kpt[(size_t)ik * 3] << " " << s.kpt[(size_t)ik * 3 + 1] << " " << s.kpt[(size_t)ik * 3 + 2] << std::endl;
-                k_printed = true;
-            }
-            if (s.in.iCurrent) {
-                fJ1 << std::setw(15) << std::setprecision(8) << tl * tfs;
-                for (int x = 0; x < 3; x++) {
-                    fJ1 << std::setw(25) << std::setprecision(15) << std::scientific << j1[2 * x];
-                    fJ1 << std::setw(25) << std::setprecision(15) << std::scientific << j1[2 * x + 1];
-                }
-                fJ1 << std::endl;
-                fJ2 << std::setw(15) << std::setprecision(8) << tl * tfs;
-                for (int x = 0; x < 3; x++) fJ2 << std::setw(25) << std::setprecision(15) << std::scientific << -2. * j2[2 * x + 1];
-                fJ2 << std::endl;
-            }
-            if (s.in.iTAbs) {
-                fT << std::setw(15) << std::setprecision(8) << tl * tfs;
-                for (int i = 0; i < 3; i++) fT << std::setw(15) << std::setprecision(8) << ef_last[3 + i];
-                for (int x = 0; x < 3; x++) {
-                    fT << std::setw(15) << std::setprecision(8) << ta[2 * x] / nktot_d;
-                    fT << std::setw(15) << std::setprecision(8) << ta[2 * x + 1] / nktot_d;
-                }
-                fT << std::endl;
-            }
+            w.observe(eng.ctx, tl, ef_last, verbose);
         }
         // Steps up to the next observation point, as one engine call.
         int64_t next_obs = (it / res + 1) * res;
@@ -132,6 +154,68 @@ int64_t run_loop(const Setup &s, const std::string &outdir, int64_t max_steps, i
         it += n;
     }
     return last + 1;
+}
+
+// Adaptive dt (Source_Main/dyn_dt_evolution.cpp:1-183): while time_loop < t_fin,
+// observables every it_resolution steps, one step of the current dt, the
+// step-size measure, then dt *= eps/max_dP when max_dP > eps and dt > 0.01, else
+// dt *= 1.02 while dt < dt0; Taylor shifts its dt history; time_loop += the new dt.
+static int64_t run_dyn(const Setup &s, const std::string &outdir, int64_t max_steps, int verbose) {
+    const double tfs = Units::time_au_fs;
+    Writers w(s, outdir);
+    Engine eng;
+    make_engine(s, eng);
+    const bool taylor = s.stepper == EDUS_STEPPER_TAYLOR;
+    const int res = std::max(1, s.in.it_resolution);
+    const double eps = s.in.epsStepAbs, dt0 = s.in.dt;
+    double dt = dt0;
+    std::vector<double> dt_prev(taylor ? s.order : 0, dt), tinv;
+    double ef_last[6] = {0, 0, 0, 0, 0, 0};
+    double f[18];
+    bool E_non0 = true;  // main_MPI.cpp:274
+    double time_loop = s.t_start;
+    int64_t it = 0;
+    while (time_loop < s.in.t_fin && (max_steps < 0 || it < max_steps)) {
+        if (it % res == 0) w.observe(eng.ctx, time_loop, ef_last, verbose);
+        uint8_t on;
+        if (taylor) {
+            s.fields_at(time_loop, f);
+            double e2 = 0;
+            for (int i = 0; i < 6; i++) e2 += f[i] * f[i];
+            on = std::sqrt(e2) > 1e-20;
+            tinv = taylor_tau_inverse(dt_prev);
+            for (int i = 0; i < 6; i++) ef_last[i] = f[i];
+        } else {
+            s.fields_at(time_loop, f);
+            s.fields_at(time_loop + dt / 2.0, f + 6);
+            s.fields_at(time_loop + dt, f + 12);
+            on = E_non0;
+            double e2 = 0;
+            for (int i = 0; i < 6; i++) e2 += f[12 + i] * f[12 + i];
+            E_non0 = std::sqrt(e2) > 1e-20;
+            for (int i = 0; i < 6; i++) ef_last[i] = f[12 + i];
+        }
+        double max_dp = 0;
+        int rc = edus_step_dyn(eng.ctx, dt, f, on, taylor ? tinv.data() : nullptr, w.p_cond_max + w.n_cond, &max_dp);
+        if (rc != 0) throw Error(std::string("edus_step_dyn: ") + edus_last_error());
+        if (verbose && max_dp > 1.01 * eps)
+            std::cout << "!!!!!!!! dt too large  max_dP = " << max_dp << " dtDynamic: " << dt << " a.u." << std::endl;
+        if (max_dp > eps && dt > 0.01) dt *= eps / max_dp;
+        else if (dt < dt0) dt *= 1.02;
+        if (taylor) {
+            for (size_t i = dt_prev.size() - 1; i > 0; i--) dt_prev[i] = dt_prev[i - 1];
+            dt_prev[0] = dt;
+        }
+        if (verbose && it % (10 * res) == 0)
+            std::cout << " dtDynamic: " << dt << " a.u. | max_dP = " << max_dp << " time_loop fs: " << time_loop * tfs << std::endl;
+        time_loop += dt;
+        it++;
+    }
+    return it;
+}
+
+int64_t run_loop(const Setup &s, const std::string &outdir, int64_t max_steps, int verbose) {
+    return s.in.dyn_dt ? run_dyn(s, outdir, max_steps, verbose) : run_const(s, outdir, max_steps, verbose);
 }
 
 }  // namespace edus

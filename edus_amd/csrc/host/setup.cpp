@@ -22,7 +22,6 @@ static inline size_t mi(int nb, int k, int i, int j) { return ((size_t)k * nb + 
 
 void Setup::build(const std::string &input_path, int flags) {
     in = parse_input(input_path);
-    if (in.dyn_dt) throw Error("dynamical_dt_evolution is not supported by this build (constant dt only)");
     if (in.coulomb) {
         if (in.coulomb_diag_basis) throw Error("Coulomb_diag_basis is not supported by this build (Wannier basis only)");
         if (!in.coulomb_read)
@@ -183,24 +182,15 @@ void Setup::build(const std::string &input_path, int flags) {
             }
 
     // ---- stepper and time axis (const_dt_evolution.cpp:164-172, runge_kutta_init.cpp:356-370)
+    // The adaptive loop runs Taylor or else RK4 (dyn_dt_evolution.cpp:51-58).
     if (in.taylor) { stepper = EDUS_STEPPER_TAYLOR; order = in.solver_order; }
-    else if (in.adams_bashforth) { stepper = EDUS_STEPPER_AB; order = in.solver_order; }
+    else if (in.adams_bashforth && !in.dyn_dt) { stepper = EDUS_STEPPER_AB; order = in.solver_order; }
     else { stepper = EDUS_STEPPER_RK4; order = 4; }
     if (stepper != EDUS_STEPPER_RK4 && (order < 1 || order > EDUS_MAX_ORDER))
         throw Error("solver order out of range 1.." + std::to_string(EDUS_MAX_ORDER));
     if (stepper == EDUS_STEPPER_TAYLOR) {
         if (order < 2) throw Error("TaylorOrder must be >= 2");
-        // tau_ip = 1e7 * t_i^(p+1) with t_i = i*dt, inverted and rescaled by 1e7
-        // (Taylor_DE_solver_deriv_Polynom.cpp:100-123); constant for constant dt.
-        int n = order - 1;
-        std::vector<double> ti(n), tau((size_t)n * n);
-        ti[0] = in.dt;
-        for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + in.dt;
-        double scale = std::pow(10, 7);
-        for (int i = 0; i < n; i++)
-            for (int p = 0; p < n; p++) tau[(size_t)i * n + p] = scale * std::pow(ti[i], (p + 1));
-        tau_inv = inverse_gauss_jordan(tau, n);
-        for (double &v : tau_inv) v *= scale;
+        tau_inv = taylor_tau_inverse(std::vector<double>(order, in.dt));  // dt_prev.fill(dt), main_MPI.cpp:366-372
     }
     double t_fin = in.t_fin;
     if (!(t_fin > 0)) {
@@ -249,6 +239,19 @@ void Setup::read_coulomb() {
     std::vector<double> sc = numbers(dir + "Screen_const_" + in.coulomb_label + ".txt", (size_t)nb2 * 2);
     cscreen.resize(nb2);
     for (int i = 0; i < nb2; i++) cscreen[i] = cplx(sc[2 * i], sc[2 * i + 1]);
+}
+
+std::vector<double> taylor_tau_inverse(const std::vector<double> &dt_prev) {
+    const int n = (int)dt_prev.size() - 1;
+    std::vector<double> ti(n), tau((size_t)n * n);
+    ti[0] = dt_prev[0];
+    for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + dt_prev[i - 1];
+    const double scale = std::pow(10, 7);
+    for (int i = 0; i < n; i++)
+        for (int p = 0; p < n; p++) tau[(size_t)i * n + p] = scale * std::pow(ti[i], (p + 1));
+    std::vector<double> inv = inverse_gauss_jordan(tau, n);
+    for (double &v : inv) v *= scale;
+    return inv;
 }
 
 // Pump sum and half probe at time t (RungeKutta.cpp:24-30).

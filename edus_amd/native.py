@@ -43,7 +43,7 @@ class SetupInfo(C.Structure):
         ("det_a", C.c_double), ("det_b", C.c_double), ("jacobian", C.c_double),
         ("j_norm", C.c_double), ("tabs_norm", C.c_double), ("nktot", C.c_double), ("wx2", C.c_double),
         ("T1", C.c_double), ("T2", C.c_double), ("Tch", C.c_double), ("shell_residual", C.c_double),
-        ("ncut", C.c_int32), ("reserved", C.c_int32),
+        ("ncut", C.c_int32), ("dyn_dt", C.c_int32), ("eps_step", C.c_double),
     ]
 
     def as_dict(self):
@@ -79,6 +79,7 @@ hip.edus_create.argtypes = [C.POINTER(Problem), C.POINTER(C.c_void_p)]
 hip.edus_destroy.argtypes = [C.c_void_p]
 for _f in (hip.edus_step_rk4, hip.edus_step_ab, hip.edus_step_taylor):
     _f.argtypes = [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_uint8)]
+hip.edus_step_dyn.argtypes = [C.c_void_p, C.c_double, _dp, C.c_uint8, _dp, C.c_double, _dp]
 hip.edus_observables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
 hip.edus_rhs.argtypes = [C.c_void_p, _dp, C.c_uint8, _dp]
 hip.edus_get_state.argtypes = [C.c_void_p, _dp]
@@ -243,6 +244,15 @@ class Engine:
         n = len(grad_on)
         fn = {STEPPER_RK4: hip.edus_step_rk4, STEPPER_AB: hip.edus_step_ab, STEPPER_TAYLOR: hip.edus_step_taylor}[self.stepper]
         _check_hip(fn(self._h, n, _dptr(fields), grad_on.ctypes.data_as(C.POINTER(C.c_uint8))), "edus_step")
+
+    def step_dyn(self, dt, fields, grad_on, tau_inv=None, cond=0.0):
+        """One adaptive-dt step; returns the step-size measure max_dP."""
+        fields = np.ascontiguousarray(fields, dtype=np.float64)
+        ti = None if tau_inv is None else np.ascontiguousarray(tau_inv, dtype=np.float64)
+        out = np.zeros(1)
+        _check_hip(hip.edus_step_dyn(self._h, float(dt), _dptr(fields), 1 if grad_on else 0,
+                                     None if ti is None else _dptr(ti), float(cond), _dptr(out)), "edus_step_dyn")
+        return float(out[0])
 
     def observables(self):
         pop = np.zeros(self.nb)

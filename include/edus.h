@@ -119,10 +119,22 @@ int edus_step_ab(edus_ctx *ctx, int32_t nsteps, const double *fields, const uint
  * with ODE_solver.h:3-33,107-123. */
 int edus_step_taylor(edus_ctx *ctx, int32_t nsteps, const double *fields, const uint8_t *grad_on);
 
+/* One step of the adaptive-dt loop with step size `dt`: RK4 for an RK4 context
+ * (fields [3][6] at t, t+dt/2, t+dt), Taylor for a Taylor context (fields [6] at t,
+ * `taylor_tau_inv` [(order-1)^2] for the current dt history).  Then the step-size
+ * measure of Source_Main/dyn_dt_evolution.cpp:74-100,
+ *   max_dp = max_{k,i} 2 |P_ii(before) - P_ii(after)| / (cond + |P_ii(after)| + 1e-7),
+ * with cond = P_cond_max + n_cond from the last observables.  Replaces the step and
+ * the max-reduction of dyn_dt_evolution.cpp (with RungeKutta.cpp:1-9,139-147 and
+ * Taylor_DE_solver_deriv_Polynom.cpp:75-84); the dt update stays with the caller. */
+int edus_step_dyn(edus_ctx *ctx, double dt, const double *fields, uint8_t grad_on, const double *taylor_tau_inv,
+                  double cond, double *max_dp);
+
 /* Observable reductions of the current state, FP64 with Kahan-compensated
  * block sums.  Replaces PrintLossesMPI_exciton (Observables_MPI.h:217-484, without
  * Coulomb), Current1 (668-718), Current2 (726-775) and CalculateTransientAbs (559-569).
- *   pop    [nb]  sum_k Re diag(U P U^+) / nktot   (before the hole transform 1-x)
+ *   pop    [nb]  sum_k Re diag(U P U^+) / nktot   (before the hole transform 1-x;
+ *                with the Coulomb term, U diagonalises the renormalised H)
  *   j1     [6]   (re,im) x,y,z of sum_k sum_ij P_ij gradH_ij * j_norm
  *   j2     [6]   (re,im) x,y,z of sum_k P_ij H_ik conj(D_jk) * j_norm
  *   tabs   [6]   (re,im) x,y,z of -2 sum_k sum_{i<nch<=j} D_ij P_ij * tabs_norm

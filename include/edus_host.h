@@ -42,7 +42,8 @@ typedef struct edus_setup_info {
     double T1, T2, Tch;
     double shell_residual;          /* norm(A w - q) of the accepted shells               */
     int32_t ncut;                   /* Coulomb Fourier modes per direction (0: no Coulomb) */
-    int32_t reserved;
+    int32_t dyn_dt;                 /* 1: adaptive dt loop (dynamical_dt_evolution)       */
+    double eps_step;                /* epsStepAbs of the adaptive loop                    */
 } edus_setup_info;
 
 /* Parses `input_path` (reference grammar, headers/ReadInput.h:332-1234) and builds

@@ -821,10 +821,12 @@ typedef struct orc_obs_cfg {
 
 /* One output row: pop[nb] (hole-transformed as printed), j1[6], j2[3] (-2 Im),
  * tabs[6] (divided by nktot as printed). Chunked sums in fixed order. */
-static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, double *pop, double *j1, double *j2, double *tabs) {
+static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, double *pop, double *j1, double *j2, double *tabs,
+                        double *pcmax) {
     const int Ncv = s->nb;
     const long CH = 256, nch = (s->nk + CH - 1) / CH;
     double *ppop = calloc((size_t)nch * Ncv, sizeof(double));
+    double *pmx = calloc((size_t)nch, sizeof(double));
     cplx *pj1 = calloc((size_t)nch * 3, sizeof(cplx)), *pj2 = calloc((size_t)nch * 3, sizeof(cplx)), *pta = calloc((size_t)nch * 3, sizeof(cplx));
 #pragma omp parallel for schedule(dynamic)
     for (long c = 0; c < nch; c++) {
@@ -872,6 +874,9 @@ static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, 
                 cplx acc = 0;
                 for (int ii = 0; ii < Ncv; ii++) acc += conj(Uk[ic * Ncv + ii]) * Mk[ic][ii];
                 ppop[c * Ncv + ic] += creal(acc);
+                /* P_cond_max_loc (Observables_MPI.h:330-347) */
+                const double ex = (ic < s->ncc) ? fabs(1. - creal(acc)) : fabs(creal(acc));
+                if (ex > pmx[c]) pmx[c] = ex;
             }
             /* Current1 / Current2 / transient absorption */
             for (int ic = 0; ic < Ncv; ic++)
@@ -886,6 +891,12 @@ static void observables(const orc_sys *s, const orc_obs_cfg *oc, const cplx *P, 
                     for (int x = 0; x < 3; x++) pta[c * 3 + x] -= 2. * Dk[(ic * Ncv + jc) * 3 + x] * Pk[ic * Ncv + jc];
         }
     }
+    if (pcmax) {
+        double m = 0;
+        for (long c = 0; c < nch; c++) if (pmx[c] > m) m = pmx[c];
+        *pcmax = m;
+    }
+    free(pmx);
     double nktot_inv = 1. / oc->nktot;
     for (int ic = 0; ic < Ncv; ic++) {
         double sacc = 0;
@@ -991,8 +1002,14 @@ static void ab_coefficients(int o, double *coef) {
     if (o == 5) { coef[0] = 1901.0 / 720.0; coef[1] = -2774.0 / 720.0; coef[2] = 2616.0 / 720.0; coef[3] = -1274.0 / 720.0; coef[4] = 251.0 / 720.0; }
 }
 
-/* tau^{-1} of Taylor_DE_solver_deriv_Polynom.cpp:100-123 for constant dt. */
-static void taylor_tau_inverse(int o, double dt, double *tinv);
+/* tau^{-1} of Taylor_DE_solver_deriv_Polynom.cpp:95-123 for the dt history
+ * dt_prev[o] (most recent first). */
+static void taylor_tau_inverse_hist(int o, const double *dt_prev, double *tinv);
+static void taylor_tau_inverse(int o, double dt, double *tinv) {
+    double h[8];
+    for (int i = 0; i < 8; i++) h[i] = dt;
+    taylor_tau_inverse_hist(o, h, tinv);
+}
 
 /* Increment of one element: AB (Linear_Multistep_ODE_solvers.cpp:296-306) from f[s] = f_{n-s}. */
 static cplx ab_delta(int o, const double *coef, const cplx *f, double dt) {
@@ -1040,15 +1057,114 @@ void orc_taylor_update(int order, double dt, const double *f, long n, double *de
     }
 }
 
-static void taylor_tau_inverse(int o, double dt, double *tinv) {
+/* Test hook: Taylor update with a dt history dt_prev[order] (most recent first). */
+void orc_taylor_update_hist(int order, double dt, const double *dt_prev, const double *f, long n, double *delta) {
+    double tinv[64];
+    taylor_tau_inverse_hist(order, dt_prev, tinv);
+    const cplx *fc = (const cplx *)f;
+    cplx *d = (cplx *)delta;
+    for (long e = 0; e < n; e++) {
+        cplx fe[8];
+        for (int q = 0; q < order; q++) fe[q] = fc[(long)q * n + e];
+        d[e] = taylor_delta(order, tinv, fe, dt);
+    }
+}
+
+static void taylor_tau_inverse_hist(int o, const double *dt_prev, double *tinv) {
     int n = o - 1;
     double ti[8], tau[64], scale = pow(10, 7);
-    ti[0] = dt;
-    for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + dt;
+    /* t_i[0] = dt_prev[0]; t_i[i] = t_i[i-1] + dt_prev[i-1] (as written, :101-104) */
+    ti[0] = dt_prev[0];
+    for (int i = 1; i < n; i++) ti[i] = ti[i - 1] + dt_prev[i - 1];
     for (int i = 0; i < n; i++)
         for (int p = 0; p < n; p++) tau[i * n + p] = scale * pow(ti[i], (p + 1));
     lu_inverse(n, tau, tinv);
     for (int i = 0; i < n * n; i++) tinv[i] *= scale;
+}
+
+/* Work arrays of a run. */
+typedef struct orc_work {
+    cplx *P1, *P2, *Pv, *hist;
+    int idx[16];
+    double coef[5];
+} orc_work;
+
+static void work_init(orc_work *w, size_t nel, const orc_run_cfg *r) {
+    w->P1 = calloc(nel, sizeof(cplx)); w->P2 = calloc(nel, sizeof(cplx)); w->Pv = calloc(nel, sizeof(cplx));
+    w->hist = (r->stepper != 0) ? calloc(nel * r->order, sizeof(cplx)) : NULL;
+    for (int i = 0; i < 16; i++) w->idx[i] = i;
+    for (int i = 0; i < 5; i++) w->coef[i] = 0;
+    if (r->stepper == 1) ab_coefficients(r->order, w->coef);
+}
+
+static void work_free(orc_work *w) { free(w->P1); free(w->P2); free(w->Pv); free(w->hist); }
+
+/* One step of size dt at time_loop.  RK4 (RungeKutta.cpp) gates the gradient
+ * with *E_non0 from the previous step and updates it from its last field;
+ * AB (Linear_Multistep_ODE_solvers.cpp) and Taylor
+ * (Taylor_DE_solver_deriv_Polynom.cpp, with tinv for the current dt history)
+ * gate with the field at time_loop.  EF ends as the step's last field. */
+static void one_step(orc_sys *s, const orc_run_cfg *r, orc_work *w, cplx *P0, long nk, double time_loop, double dt,
+                     const double *tinv, int *E_non0, double EF[2][3]) {
+    const int nb = s->nb, o = r->order;
+    const size_t nel = (size_t)nk * nb * nb;
+    cplx *P1 = w->P1, *P2 = w->P2, *Pv = w->Pv, *hist = w->hist;
+    if (r->stepper == 0) {
+        const double dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
+        double tRK = time_loop;
+        fields(r, tRK, EF);
+        rhs(s, P0, Pv, EF, *E_non0);
+        for (size_t e = 0; e < nel; e++) { P1[e] = P0[e] + Pv[e] * dt6; P2[e] = P0[e] + Pv[e] * dt2; }
+        tRK = time_loop + dt2;
+        fields(r, tRK, EF);
+        rhs(s, P2, Pv, EF, *E_non0);
+        for (size_t e = 0; e < nel; e++) { P1[e] += Pv[e] * dt3; P2[e] = P0[e] + Pv[e] * dt2; }
+        rhs(s, P2, Pv, EF, *E_non0);
+        for (size_t e = 0; e < nel; e++) { P1[e] += Pv[e] * dt3; P2[e] = P0[e] + Pv[e] * dt; }
+        tRK = time_loop + dt;
+        fields(r, tRK, EF);
+        rhs(s, P2, Pv, EF, *E_non0);
+        for (size_t e = 0; e < nel; e++) P0[e] = P1[e] + Pv[e] * dt6;
+        *E_non0 = field_nonzero(EF);
+        return;
+    }
+    int *idx = w->idx;
+    fields(r, time_loop, EF);
+    *E_non0 = field_nonzero(EF);
+    /* Multistep_index_shift (ODE_solver.h:66-75) */
+    int fill = idx[o - 1];
+    for (int q = o - 1; q > 0; q--) idx[q] = idx[q - 1];
+    idx[0] = fill;
+    rhs(s, P0, Pv, EF, *E_non0);
+    memcpy(&hist[(size_t)idx[0] * nel], Pv, nel * sizeof(cplx));
+    if (r->stepper == 1) {
+#pragma omp parallel for schedule(static)
+        for (long ik = 0; ik < nk; ik++)
+            for (int ic = 0; ic < nb; ic++)
+                for (int jc = ic; jc < nb; jc++) {
+                    size_t e = ((size_t)ik * nb + ic) * nb + jc;
+                    cplx fe[8];
+                    for (int q = 0; q < o; q++) fe[q] = hist[(size_t)idx[q] * nel + e];
+                    P0[e] += ab_delta(o, w->coef, fe, dt);
+                    if (ic != jc) P0[((size_t)ik * nb + jc) * nb + ic] = conj(P0[e]);
+                }
+    } else {
+#pragma omp parallel for schedule(static)
+        for (long e = 0; e < (long)nel; e++) {
+            cplx fe[8];
+            for (int q = 0; q < o; q++) fe[q] = hist[(size_t)idx[q] * nel + e];
+            P0[e] += taylor_delta(o, tinv, fe, dt);
+        }
+    }
+}
+
+static void write_row(orc_sys *s, const orc_obs_cfg *oc, const orc_run_cfg *r, const cplx *P0, double time_loop,
+                      double EF[2][3], double *row, double *pcmax) {
+    const int nb = s->nb;
+    row[0] = time_loop * r->time_au_fs;
+    for (int i = 0; i < 3; i++) row[1 + i] = EF[0][i];
+    observables(s, oc, P0, &row[4], &row[4 + nb], &row[10 + nb], &row[16 + nb], pcmax);
+    for (int i = 0; i < 3; i++) row[13 + nb + i] = EF[1][i];
 }
 
 long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, double *P0io, double *rows, long max_rows) {
@@ -1057,18 +1173,11 @@ long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, d
     if (t->coul) coul_init(&s, t->coul, (const cplx *)P0io);
     const long nk = t->nk;
     const int nb = t->nb;
-    const size_t nel = (size_t)nk * nb * nb;
     cplx *P0 = (cplx *)P0io;
-    cplx *P1 = calloc(nel, sizeof(cplx)), *P2 = calloc(nel, sizeof(cplx)), *Pv = calloc(nel, sizeof(cplx));
-    const int o = r->order;
-    cplx *hist = (r->stepper != 0) ? calloc(nel * o, sizeof(cplx)) : NULL;
-    int idx[16];
-    for (int i = 0; i < 16; i++) idx[i] = i;
-    double coef[5] = {0};
-    if (r->stepper == 1) ab_coefficients(o, coef);
+    orc_work w;
+    work_init(&w, (size_t)nk * nb * nb, r);
     double tinv[64];
-    if (r->stepper == 2) taylor_tau_inverse(o, r->dt, tinv);
-    const double dt = r->dt, dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
+    if (r->stepper == 2) taylor_tau_inverse(r->order, r->dt, tinv);
     double EF[2][3] = {{0, 0, 0}, {0, 0, 0}};
     int E_non0 = 1;
     double time_loop = r->t_start;
@@ -1076,62 +1185,74 @@ long orc_run(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, d
     const int rowlen = nb + 22;
     for (long it = 0; it < r->nsteps; it++) {
         if (it % r->it_res == 0 && nrow < max_rows) {
-            double *row = &rows[nrow * rowlen];
-            row[0] = time_loop * r->time_au_fs;
-            for (int i = 0; i < 3; i++) row[1 + i] = EF[0][i];
-            observables(&s, oc, P0, &row[4], &row[4 + nb], &row[10 + nb], &row[16 + nb]);
-            for (int i = 0; i < 3; i++) row[13 + nb + i] = EF[1][i];
+            write_row(&s, oc, r, P0, time_loop, EF, &rows[nrow * rowlen], NULL);
             nrow++;
         }
-        if (r->stepper == 0) {
-            /* RungeKutta.cpp */
-            double tRK = time_loop;
-            fields(r, tRK, EF);
-            rhs(&s, P0, Pv, EF, E_non0);
-            for (size_t e = 0; e < nel; e++) { P1[e] = P0[e] + Pv[e] * dt6; P2[e] = P0[e] + Pv[e] * dt2; }
-            tRK = time_loop + dt2;
-            fields(r, tRK, EF);
-            rhs(&s, P2, Pv, EF, E_non0);
-            for (size_t e = 0; e < nel; e++) { P1[e] += Pv[e] * dt3; P2[e] = P0[e] + Pv[e] * dt2; }
-            rhs(&s, P2, Pv, EF, E_non0);
-            for (size_t e = 0; e < nel; e++) { P1[e] += Pv[e] * dt3; P2[e] = P0[e] + Pv[e] * dt; }
-            tRK = time_loop + dt;
-            fields(r, tRK, EF);
-            rhs(&s, P2, Pv, EF, E_non0);
-            for (size_t e = 0; e < nel; e++) P0[e] = P1[e] + Pv[e] * dt6;
-            E_non0 = field_nonzero(EF);
-        } else {
-            fields(r, time_loop, EF);
-            E_non0 = field_nonzero(EF);
-            /* Multistep_index_shift (ODE_solver.h:66-75) */
-            int fill = idx[o - 1];
-            for (int q = o - 1; q > 0; q--) idx[q] = idx[q - 1];
-            idx[0] = fill;
-            rhs(&s, P0, Pv, EF, E_non0);
-            memcpy(&hist[(size_t)idx[0] * nel], Pv, nel * sizeof(cplx));
-            if (r->stepper == 1) {
-#pragma omp parallel for schedule(static)
-                for (long ik = 0; ik < nk; ik++)
-                    for (int ic = 0; ic < nb; ic++)
-                        for (int jc = ic; jc < nb; jc++) {
-                            size_t e = ((size_t)ik * nb + ic) * nb + jc;
-                            cplx fe[8];
-                            for (int q = 0; q < o; q++) fe[q] = hist[(size_t)idx[q] * nel + e];
-                            P0[e] += ab_delta(o, coef, fe, dt);
-                            if (ic != jc) P0[((size_t)ik * nb + jc) * nb + ic] = conj(P0[e]);
-                        }
-            } else {
-#pragma omp parallel for schedule(static)
-                for (long e = 0; e < (long)nel; e++) {
-                    cplx fe[8];
-                    for (int q = 0; q < o; q++) fe[q] = hist[(size_t)idx[q] * nel + e];
-                    P0[e] += taylor_delta(o, tinv, fe, dt);
-                }
+        one_step(&s, r, &w, P0, nk, time_loop, r->dt, tinv, &E_non0, EF);
+        time_loop += r->dt;
+    }
+    work_free(&w);
+    coul_free(s.coul);
+    return nrow;
+}
+
+/* Adaptive dt (dyn_dt_evolution.cpp:1-183) with RK4 (stepper 0) or Taylor (2):
+ * while time_loop < t_fin (and it < r->nsteps), observables every it_res steps
+ * (n_cond = largest printed population, P_cond_max = largest excited population
+ * over k, Observables_MPI.h:330-347,413-420), one step of the current dt, then
+ * max_dP = max 2|P_ii(before) - P_ii(after)| / (P_cond_max + n_cond + |P_ii| + 1e-7),
+ * dt *= eps/max_dP if max_dP > eps and dt > 0.01, else dt *= 1.02 while dt < dt0;
+ * Taylor shifts dt_prev; time_loop += dt (the new dt).  dts[it] = dt used by step it. */
+long orc_run_dyn(const orc_tables *t, const orc_obs_cfg *oc, const orc_run_cfg *r, double t_fin, double eps, double *P0io,
+                 double *rows, long max_rows, double *dts, long *nsteps_out) {
+    orc_sys s;
+    make_sys(t, &s);
+    if (t->coul) coul_init(&s, t->coul, (const cplx *)P0io);
+    const long nk = t->nk;
+    const int nb = t->nb, o = r->order;
+    cplx *P0 = (cplx *)P0io;
+    orc_work w;
+    work_init(&w, (size_t)nk * nb * nb, r);
+    double *dg = malloc(sizeof(double) * nk * nb);
+    double dt = r->dt, dt0 = r->dt, dt_prev[8], tinv[64];
+    for (int i = 0; i < 8; i++) dt_prev[i] = dt;
+    double EF[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int E_non0 = 1;
+    double time_loop = r->t_start, n_cond = 1e-16, p_cond_max = 0.0;
+    long nrow = 0, it = 0;
+    const int rowlen = nb + 22;
+    while (time_loop < t_fin && it < r->nsteps) {
+        if (it % r->it_res == 0 && nrow < max_rows) {
+            double *row = &rows[nrow * rowlen];
+            write_row(&s, oc, r, P0, time_loop, EF, row, &p_cond_max);
+            n_cond = 0;
+            for (int ic = 0; ic < nb; ic++) if (row[4 + ic] > n_cond) n_cond = row[4 + ic];
+            nrow++;
+        }
+        for (long ik = 0; ik < nk; ik++)
+            for (int ic = 0; ic < nb; ic++) dg[ik * nb + ic] = creal(P0[((size_t)ik * nb + ic) * nb + ic]);
+        if (r->stepper == 2) taylor_tau_inverse_hist(o, dt_prev, tinv);
+        one_step(&s, r, &w, P0, nk, time_loop, dt, tinv, &E_non0, EF);
+        double max_dp = 0.0;
+        for (long ik = 0; ik < nk; ik++)
+            for (int ic = 0; ic < nb; ic++) {
+                const cplx pn = P0[((size_t)ik * nb + ic) * nb + ic];
+                const double v = 2 * cabs(dg[ik * nb + ic] - pn) / (p_cond_max + n_cond + cabs(pn) + 1.0e-7);
+                if (v > max_dp) max_dp = v;
             }
+        if (dts) dts[it] = dt;
+        if (max_dp > eps && dt > 0.01) dt *= eps / max_dp;
+        else if (dt < dt0) dt *= 1.02;
+        if (r->stepper == 2) {
+            for (int i = o - 1; i > 0; i--) dt_prev[i] = dt_prev[i - 1];
+            dt_prev[0] = dt;
         }
         time_loop += dt;
+        it++;
     }
-    free(P1); free(P2); free(Pv); free(hist);
+    *nsteps_out = it;
+    free(dg);
+    work_free(&w);
     coul_free(s.coul);
     return nrow;
 }

@@ -71,9 +71,13 @@ def _load():
     lib.orc_rho0.argtypes = [_dp, C.c_long, C.c_int, C.c_int, _dp]
     lib.orc_rhs.argtypes = [C.POINTER(OTables), _dp, _dp, _dp, C.c_int]
     lib.orc_run.argtypes = [C.POINTER(OTables), C.POINTER(OObs), C.POINTER(ORun), _dp, _dp, C.c_long]
+    lib.orc_run_dyn.restype = C.c_long
+    lib.orc_run_dyn.argtypes = [C.POINTER(OTables), C.POINTER(OObs), C.POINTER(ORun), C.c_double, C.c_double, _dp, _dp,
+                                C.c_long, _dp, C.POINTER(C.c_long)]
     lib.orc_run.restype = C.c_long
     lib.orc_coulomb_x.argtypes = [C.POINTER(OTables), _dp, _dp, _dp]
     lib.orc_rhs_coulomb.argtypes = [C.POINTER(OTables), _dp, _dp, _dp, _dp, C.c_int, C.c_double]
+    lib.orc_taylor_update_hist.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_long, _dp]
     lib.orc_set_threads.argtypes = [C.c_int]
     lib.orc_max_threads.restype = C.c_int
     return lib
@@ -242,6 +246,28 @@ class System:
         rows = np.zeros((max(nrows, 1), self.nb + 22))
         n = lib.orc_run(C.byref(self.t), C.byref(o), C.byref(r), _d(P.view(np.float64)), _d(rows), nrows)
         return rows[:n], P
+
+    def run_dyn(self, P0, stepper, order, dt, t_start, t_fin, eps, max_steps, it_res, pumps, probe, gradH=None,
+                j_norm=0.0, jacobian=0.0, dk3=0.0, nktot=1.0):
+        """Adaptive-dt run (stepper 0 RK4, 2 Taylor) of at most max_steps steps;
+        returns (rows, P_final, dts[nsteps])."""
+        P = np.ascontiguousarray(P0, dtype=np.complex128).copy()
+        r = ORun()
+        r.stepper, r.order, r.dt, r.t_start, r.nsteps, r.it_res = stepper, order, dt, t_start, max_steps, it_res
+        r.n_pumps = len(pumps)
+        for i, p in enumerate(pumps):
+            r.pumps[i] = pulse_struct(p)
+        r.probe = pulse_struct(probe)
+        r.time_au_fs = TIME_AU_FS
+        gh = None if gradH is None else np.ascontiguousarray(gradH, dtype=np.complex128)
+        o = OObs(_d(gh.view(np.float64)) if gh is not None else None, j_norm, jacobian, dk3, nktot)
+        nrows = (max_steps + it_res - 1) // it_res
+        rows = np.zeros((max(nrows, 1), self.nb + 22))
+        dts = np.zeros(max(max_steps, 1))
+        nst = C.c_long(0)
+        n = lib.orc_run_dyn(C.byref(self.t), C.byref(o), C.byref(r), float(t_fin), float(eps), _d(P.view(np.float64)),
+                            _d(rows), nrows, _d(dts), C.byref(nst))
+        return rows[:n], P, dts[:nst.value]
 
 
 def set_threads(n):

@@ -133,6 +133,19 @@ def oracle_run(setup, osys, nsteps, it_res=None, gradH=True):
     return rows, P
 
 
+def oracle_run_dyn(setup, osys, max_steps, it_res=None, gradH=True):
+    """The oracle's adaptive-dt loop with the setup's parameters."""
+    info = setup.info
+    pumps = [setup.pulse(i) for i in range(info["n_pumps"])]
+    probe = setup.pulse(info["n_pumps"])
+    G = gradH_from(osys["H"], osys["nbr"], np.repeat(osys["w"], osys["sizes"]), osys["bvec"]) if gradH else None
+    P0 = O.rho0(osys["U"], info["ncc"])
+    dk3 = 1.0 / (info["nk0"] * info["nk1"] * info["nk2"])
+    return osys["sys"].run_dyn(P0, info["stepper"], info["order"], info["dt"], info["t_start"], info["t_fin"],
+                               info["eps_step"], max_steps, it_res or info["it_resolution"], pumps, probe, gradH=G,
+                               j_norm=info["j_norm"], jacobian=info["jacobian"], dk3=dk3, nktot=info["nktot"])
+
+
 def split_rows(rows, nb):
     """Oracle rows -> dict of the printed columns."""
     return {"t": rows[:, 0], "EF": rows[:, 1:4], "pop": rows[:, 4:4 + nb], "J1": rows[:, 4 + nb:10 + nb],
