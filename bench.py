@@ -36,11 +36,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps of the CPU sample (0: auto)")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 20 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the k-grid of one problem lives in one HBM); "
               "multi-GPU k-slab sharding is a design in DESIGN.md, not built.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from edus_amd import native as N
     from edus_amd import workloads
@@ -91,7 +97,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(s, args.cpu_steps)
+        cpu = cpu_baseline(s, eng, args.cpu_steps)
 
     out = {
         "metric": "k-point\u00b7steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %",
@@ -116,9 +122,24 @@ def main():
     print(json.dumps(out))
 
 
-def cpu_baseline(s, steps):
+def copy_calibration(mib):
+    """A plain device-to-device copy kernel of known size: its FETCH_SIZE and
+    WRITE_SIZE per dispatch calibrate the HBM counters (expected mib MiB each)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(20):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_mib": mib, "dispatches": 20, "check": float(y[-1] - x[-1])}))
+
+
+def cpu_baseline(s, eng, steps):
     """The CPU oracle (FP64 restatement of the reference loops, OpenMP over k) on a
-    bounded sample of the same workload: the full k-grid for a few RK4 steps."""
+    bounded sample of the same workload: the full k-grid for a few RK4 steps from
+    the initial state at the pulse peak.  The GPU engine repeats the same steps, and
+    the J(t) error of the GPU path is stated from the two final states."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import oracle_py as O
     import helpers as Hh
@@ -134,16 +155,28 @@ def cpu_baseline(s, steps):
     t_start = s.time(it0)
     n = steps or 2
     t0 = time.perf_counter()
-    osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
+    _, Pc = osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
     el = time.perf_counter() - t0
     if not steps and el < 10.0:  # scale the sample to ~10-30 s of CPU work
         n = max(2, int(n * 12.0 / max(el, 1e-3)))
         t0 = time.perf_counter()
-        osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
+        _, Pc = osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
         el = time.perf_counter() - t0
-    return {"value": info["nk"] * n / el, "unit": "k-point*steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} RK4 steps of the full {info['nk1']}x{info['nk2']} grid ({nb} bands), oracle, {cores} threads",
-            "seconds": round(el, 2)}
+    # the same steps on the GPU, then J1 of both final states (GPU reductions)
+    f, g = s.fields(it0, n)
+    g[0] = 1  # the oracle's run starts with E_non0 = true
+    eng.set_state(P0)
+    eng.step(f, g)
+    Pg = eng.get_state()
+    jg = eng.observables()["j1"].copy()
+    eng.set_state(Pc)
+    jc = eng.observables()["j1"].copy()
+    j_err = float(np.max(np.abs(jg - jc)) / max(np.max(np.abs(jc)), 1e-300))
+    p_err = float(np.max(np.abs(Pg - Pc)) / np.max(np.abs(Pc)))
+    return {"value": info["nk"] * n / el, "unit": "k-point\u00b7steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} RK4 steps of the full {info['nk1']}x{info['nk2']} grid ({nb} bands) from rho0 at "
+                      f"t = {t_start:.3f} au, oracle, {cores} OpenMP threads",
+            "seconds": round(el, 2), "gpu_vs_oracle": {"J1_rel_err": j_err, "rho_rel_err": p_err, "steps": n}}
 
 
 if __name__ == "__main__":
