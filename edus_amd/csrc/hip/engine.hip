@@ -112,6 +112,12 @@ __device__ __forceinline__ int xcd_block() {
 // Pair index of (i<j) in row-major upper-triangle order.
 __host__ __device__ __forceinline__ int pair_index(int i, int j, int nb) { return i * nb - i * (i + 1) / 2 + (j - i - 1); }
 
+// Streaming accesses: operators, accumulators, histories and stage outputs are
+// touched once per launch, so they bypass the caches' normal retention and leave
+// L2 to the stage input, whose 12 stencil neighbours are re-read from it.
+__device__ __forceinline__ double ldnt(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stnt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+
 // ---------------------------------------------------------------- stage update
 // The elementwise tail shared by all kernels.  `kv` is one packed double of the
 // right-hand side, `y` the same double of the stage input, `o` its offset in the
@@ -119,32 +125,32 @@ __host__ __device__ __forceinline__ int pair_index(int i, int j, int nb) { retur
 template <int MODE>
 __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_t h, double kv, double y) {
     if constexpr (MODE == RK1) {
-        a.ACC[o] = y + kv * a.ca;
-        a.OUT[o] = y + kv * a.cb;
+        stnt(a.ACC + o, y + kv * a.ca);
+        stnt(a.OUT + o, y + kv * a.cb);
     } else if constexpr (MODE == RK2 || MODE == RK3) {
-        a.ACC[o] = a.ACC[o] + kv * a.ca;
-        a.OUT[o] = a.P0[o] + kv * a.cb;
+        stnt(a.ACC + o, ldnt(a.ACC + o) + kv * a.ca);
+        stnt(a.OUT + o, ldnt(a.P0 + o) + kv * a.cb);
     } else if constexpr (MODE == RK4S) {
-        a.OUT[o] = a.ACC[o] + kv * a.ca;
+        stnt(a.OUT + o, ldnt(a.ACC + o) + kv * a.ca);
     } else if constexpr (MODE == ABM) {
-        a.HIST[(size_t)a.slot_out * h + o] = kv;
+        stnt(a.HIST + (size_t)a.slot_out * h + o, kv);
         double delta = a.coef[0] * kv;
-        for (int s = 1; s < a.order; s++) delta += a.coef[s] * a.HIST[(size_t)a.slots[s] * h + o];
-        a.OUT[o] = y + a.dt * delta;
+        for (int s = 1; s < a.order; s++) delta += a.coef[s] * ldnt(a.HIST + (size_t)a.slots[s] * h + o);
+        stnt(a.OUT + o, y + a.dt * delta);
     } else if constexpr (MODE == RHSO) {
         a.OUT[o] = kv;
     } else {  // TAY
-        a.HIST[(size_t)a.slot_out * h + o] = kv;
+        stnt(a.HIST + (size_t)a.slot_out * h + o, kv);
         const int n = a.order - 1;
         double dp[EDUS_MAX_ORDER];
-        for (int i = 0; i < n; i++) dp[i] = kv - a.HIST[(size_t)a.slots[i + 1] * h + o];
+        for (int i = 0; i < n; i++) dp[i] = kv - ldnt(a.HIST + (size_t)a.slots[i + 1] * h + o);
         double delta = kv * a.dt;
         for (int i = 1; i <= n; i++) {
             double al = 0.0;
             for (int p = 0; p < n; p++) al += a.tinv[(i - 1) * n + p] * dp[p];
             delta += (a.dtp[i] * al) / (double)(i + 1);
         }
-        a.OUT[o] = y + delta;
+        stnt(a.OUT + o, y + delta);
     }
 }
 
@@ -225,14 +231,15 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
             const double *E = w ? Eq : Ep;
             cd h, dx[3];
             if (i == j) {
-                h = mk(c.H[i * nkp + k], 0.0);
+                h = mk(ldnt(c.H + i * nkp + k), 0.0);
 #pragma unroll
-                for (int x = 0; x < 3; x++) dx[x] = mk(c.D[(x * NP + i) * nkp + k], 0.0);
+                for (int x = 0; x < 3; x++) dx[x] = mk(ldnt(c.D + (x * NP + i) * nkp + k), 0.0);
             } else {
                 const int p = pair_index(i, j, NB);
-                h = mk(c.H[(NB + 2 * p) * nkp + k], c.H[(NB + 2 * p + 1) * nkp + k]);
+                h = mk(ldnt(c.H + (NB + 2 * p) * nkp + k), ldnt(c.H + (NB + 2 * p + 1) * nkp + k));
 #pragma unroll
-                for (int x = 0; x < 3; x++) dx[x] = mk(c.D[(x * NP + NB + 2 * p) * nkp + k], c.D[(x * NP + NB + 2 * p + 1) * nkp + k]);
+                for (int x = 0; x < 3; x++)
+                    dx[x] = mk(ldnt(c.D + (x * NP + NB + 2 * p) * nkp + k), ldnt(c.D + (x * NP + NB + 2 * p + 1) * nkp + k));
             }
             if (c.coul) h = Hren[i][j];
             cd mij = h, mji = cconj(h);
@@ -275,7 +282,7 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
 #pragma unroll
         for (int i = 0; i < NB; i++)
 #pragma unroll
-            for (int j = 0; j < NB; j++) U[i][j] = mk(u[(i * NB + j) * 2], u[(i * NB + j) * 2 + 1]);
+            for (int j = 0; j < NB; j++) U[i][j] = mk(ldnt(u + (i * NB + j) * 2), ldnt(u + (i * NB + j) * 2 + 1));
         cd G[NB][NB];
 #pragma unroll
         for (int ia = 0; ia < NB; ia++)
@@ -304,7 +311,9 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
                     w = cadd(w, cmul(cconj(U[i][ia]), t));
                 }
                 cd hole = mk(0.0, 0.0);
-                for (int v = 0; v < c.ncc; v++) hole = cadd(hole, cscale(cmul(cconj(U[v][j]), U[v][i]), c.T[v * NB + v]));
+#pragma unroll
+                for (int v = 0; v < NB; v++)  // static indices keep U in registers
+                    if (v < c.ncc) hole = cadd(hole, cscale(cmul(cconj(U[v][j]), U[v][i]), c.T[v * NB + v]));
                 if (i == j) {
                     pv[i] = pv[i] - w.x + hole.x;
                 } else {
@@ -674,10 +683,13 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         const int i = ei[m], j = ej[m], comp = ec[m];
         const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
         const double *E = w ? Eq : Ep;
-        const double hv = c.H[base + e];
+        const double hv = ldnt(c.H + base + e);
+        double dv[3];
+#pragma unroll
+        for (int x = 0; x < 3; x++) dv[x] = ldnt(c.D + (size_t)x * c.nk * NP + base + e);
         double dsum = hv;
 #pragma unroll
-        for (int x = 0; x < 3; x++) dsum += E[x] * c.D[(size_t)x * c.nk * NP + base + e];
+        for (int x = 0; x < 3; x++) dsum += E[x] * dv[x];
         double *Ad = reinterpret_cast<double *>(A), *Bd = reinterpret_cast<double *>(B);
         if (i == j) {
             A[i * LD + i] = mk(y[m], 0.0);
@@ -690,7 +702,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         } else {
             double dd = 0.0;
 #pragma unroll
-            for (int x = 0; x < 3; x++) dd += E[x] * c.D[(size_t)x * c.nk * NP + base + e];
+            for (int x = 0; x < 3; x++) dd += E[x] * dv[x];
             Ad[2 * (i * LD + j) + 1] = y[m];
             Ad[2 * (j * LD + i) + 1] = -y[m];
             Bd[2 * (i * LD + j) + 1] = dsum;
@@ -727,7 +739,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         // C <- U
         const double *u = c.U + (size_t)k * nb * nb * 2;
         for (int t = lane; t < nb * nb; t += 64)
-            if (valid) C[(t / nb) * LD + (t % nb)] = mk(u[2 * t], u[2 * t + 1]);
+            if (valid) C[(t / nb) * LD + (t % nb)] = mk(ldnt(u + 2 * t), ldnt(u + 2 * t + 1));
         __syncthreads();
         cd acc[RB][RB];
         gemm_lds<NBT, 0, 0>(A, C, nb, lane, acc);     // T1 = P U^T
