@@ -483,7 +483,7 @@ __global__ void k_coul_s3(CoulArgs a) {
 // H_ren D^+ for Current2.  One thread per k, nb <= 4.
 template <int NB>
 __global__ __launch_bounds__(256) void k_coul_obs(const double *Hr, const double *Dp, const int *nbr, const double *wb, int n_nbr,
-                                                   int nk, int nkp, double *Uexc, double *GE, double *Q) {
+                                                   int nk, int nkp, double *Uexc, double *GE, double *Q, double *Ek) {
     constexpr int NP = NB * NB;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
@@ -546,6 +546,7 @@ __global__ __launch_bounds__(256) void k_coul_obs(const double *Hr, const double
         for (int j = i + 1; j < NB; j++)
             if (A[ord[j]][ord[j]].x < A[ord[i]][ord[i]].x) { int tt = ord[i]; ord[i] = ord[j]; ord[j] = tt; }
     double *u = Uexc + (size_t)k * NB * NB * 2;
+    for (int a2 = 0; a2 < NB; a2++) Ek[(size_t)k * NB + a2] = A[ord[a2]][ord[a2]].x;
     for (int a2 = 0; a2 < NB; a2++)
         for (int b = 0; b < NB; b++) {
             const cd v = V[b][ord[a2]];
@@ -995,6 +996,30 @@ __global__ __launch_bounds__(256) void k_diag_dp(const double *Y, const double *
     if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// ================================================================ k-resolved snapshots
+// Element (ic, jc) of conj(U) P_sym U^T per k, P_sym from the stored upper
+// triangle, diagonal forced real (get_P_in_dia_vect, InitParameters.h:264-335).
+__global__ void k_eigen_element(const double *Y, const double *U, int nk, int nb, size_t sk, size_t se, int ic, int jc, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    const double *u = U + (size_t)k * nb * nb * 2;
+    auto pel = [&](int i, int j) -> cd {
+        if (i == j) return mk(Y[k * sk + i * se], 0.0);
+        const int a = min(i, j), b = max(i, j), p = pair_index(a, b, nb);
+        const cd v = mk(Y[k * sk + (nb + 2 * p) * se], Y[k * sk + (nb + 2 * p + 1) * se]);
+        return (i < j) ? v : cconj(v);
+    };
+    cd acc = mk(0.0, 0.0);
+    for (int ii = 0; ii < nb; ii++) {
+        cd m = mk(0.0, 0.0);
+        for (int jj = 0; jj < nb; jj++) m = cadd(m, cmul(pel(ii, jj), mk(u[(jc * nb + jj) * 2], u[(jc * nb + jj) * 2 + 1])));
+        acc = cadd(acc, cmul(mk(u[(ic * nb + ii) * 2], -u[(ic * nb + ii) * 2 + 1]), m));
+    }
+    if (ic == jc) acc.y = 0.0;
+    out[2 * (size_t)k] = acc.x;
+    out[2 * (size_t)k + 1] = acc.y;
+}
+
 // ================================================================ Wannier contraction
 // Out[k][m] = sum_R phase(k,R) Val[R][m], Val = [H | r_x | r_y | r_z] per R.
 // phase(k,R) = prod_l (-1)^{R_l} w_l[(i_l R_l) mod N_l] for N_l > 1 (k_l = -1/2 + i_l/N_l).
@@ -1095,7 +1120,9 @@ struct edus_ctx {
     double *cxm = nullptr, *sxm = nullptr, *cyn = nullptr, *syn = nullptr, *cS = nullptr, *cF = nullptr;
     double *cX = nullptr, *cX0 = nullptr, *cA = nullptr, *cB = nullptr, *cC = nullptr, *cD = nullptr, *cvh = nullptr;
     double *cEH = nullptr, *cEH0 = nullptr, *cT = nullptr, *cPW0 = nullptr, *cHr = nullptr, *cscreen = nullptr;
-    double *cUexc = nullptr, *cGE = nullptr, *cQ = nullptr, *dwb = nullptr;
+    double *cUexc = nullptr, *cGE = nullptr, *cQ = nullptr, *dwb = nullptr, *cEk = nullptr;
+    bool cEk_valid = false;
+    double *snap = nullptr;  // [nk] complex, k-resolved snapshot element
     // adaptive dt: diagonal before the step, block maxima
     double *DG = nullptr, *dpart = nullptr;
 };
@@ -1448,7 +1475,7 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
             (rc = dalloc(c, &c->cX, (size_t)4 * nmm * nb2 * 2)) || (rc = dalloc(c, &c->cT, (size_t)2 * c->N2 * c->ncut * c->nu * 2)) ||
             (rc = dalloc(c, &c->cEH, nb)) || (rc = dalloc(c, &c->cPW0, se)) || (rc = dalloc(c, &c->cHr, se)) ||
             (rc = dalloc(c, &c->cUexc, (size_t)p->nk * nb2 * 2)) || (rc = dalloc(c, &c->cGE, (size_t)p->nk * 3 * c->np)) ||
-            (rc = dalloc(c, &c->cQ, (size_t)p->nk * 3 * nb2 * 2)))
+            (rc = dalloc(c, &c->cQ, (size_t)p->nk * 3 * nb2 * 2)) || (rc = dalloc(c, &c->cEk, (size_t)p->nk * nb)))
             return fail(rc);
         if ((rc = up(&c->cX0, std::vector<double>((size_t)4 * nmm * nb2 * 2, 0.0))) || (rc = up(&c->cEH0, std::vector<double>(nb, 0.0))) ||
             (rc = upc(&c->dwb, p->nbr_wb, (size_t)std::max(p->n_nbr, 1) * 3)))
@@ -1639,6 +1666,31 @@ int edus_step_dyn(edus_ctx *c, double dt, const double *fields, uint8_t grad_on,
     return 0;
 }
 
+int edus_eigen_element(edus_ctx *c, int32_t ic, int32_t jc, double *out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!out || ic < 0 || jc < 0 || ic >= c->nb || jc >= c->nb) return set_err(EDUS_ERR_INVALID, "bad snapshot element");
+    if (!c->snap && (rc = dalloc(c, &c->snap, (size_t)2 * c->nk))) return rc;
+    const size_t sk = c->soa ? 1 : (size_t)c->np, se = c->soa ? (size_t)c->nkp : 1;
+    hipLaunchKernelGGL(k_eigen_element, dim3((c->nk + 255) / 256), dim3(256), 0, c->stream, current_state(c), (const double *)c->U, c->nk,
+                       c->nb, sk, se, ic, jc, c->snap);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, c->snap, (size_t)2 * c->nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int edus_renorm_energies(edus_ctx *c, double *out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!out) return set_err(EDUS_ERR_INVALID, "null output");
+    if (!c->coul) return set_err(EDUS_ERR_INVALID, "no Coulomb term in this context");
+    if (!c->cEk_valid) return set_err(EDUS_ERR_INVALID, "call edus_observables first");
+    HIP_TRY(hipMemcpyAsync(out, c->cEk, (size_t)c->nk * c->nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *tabs, double *pminmax) {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -1647,12 +1699,13 @@ int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *t
     if (c->coul) {
         const int blocks = (c->nk + 255) / 256;
         switch (c->nb) {
-            case 1: hipLaunchKernelGGL(k_coul_obs<1>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
-            case 2: hipLaunchKernelGGL(k_coul_obs<2>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
-            case 3: hipLaunchKernelGGL(k_coul_obs<3>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
-            default: hipLaunchKernelGGL(k_coul_obs<4>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ); break;
+            case 1: hipLaunchKernelGGL(k_coul_obs<1>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
+            case 2: hipLaunchKernelGGL(k_coul_obs<2>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
+            case 3: hipLaunchKernelGGL(k_coul_obs<3>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
+            default: hipLaunchKernelGGL(k_coul_obs<4>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
         }
         HIP_TRY(hipGetLastError());
+        c->cEk_valid = true;
         o.U = c->cUexc;
         o.Q = c->cQ;
         if (c->GE) o.GE = c->cGE;

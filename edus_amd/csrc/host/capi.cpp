@@ -112,6 +112,8 @@ int edus_setup_get_info(const edus_setup *h, edus_setup_info *info) {
     info->ncut = s.in.coulomb ? s.ncut : 0;
     info->dyn_dt = s.in.dyn_dt ? 1 : 0;
     info->eps_step = s.in.epsStepAbs;
+    info->print_flags = (s.in.print_population ? 1 : 0) | (s.in.print_band_dispersion ? 2 : 0);
+    info->print_start = s.in.print_start; info->print_end = s.in.print_end; info->print_step = s.in.print_step;
     return 0;
 }
 

@@ -76,7 +76,10 @@ struct Input {
     bool const_dt = true, dyn_dt = false, taylor = false, adams_bashforth = false;
     double epsStepAbs = 1e-2, gap_correction = 0;
     int solver_order = 5;
-    bool print_population = false;
+    // PrintPopulation [t0 t1 dt] (a.u.; defaults of variables.cpp:84-86) and the
+    // Coulomb section's Print_new_band_dispersion: k-resolved snapshots.
+    bool print_population = false, print_band_dispersion = false;
+    double print_start = 0.0, print_end = 100.0 * 0.02418884326505, print_step = 20.0 * 0.02418884326505;
     int it_resolution = 10;
     bool iCurrent = false, iTAbs = true, iTAbsK = false, iWFDs = false;
     bool vectorization = true;

@@ -456,7 +456,14 @@ Input parse_input(const std::string &path) {
                     if (ieq(k, "wfd")) in.iWFDs = true;
                     else if (ieq(k, "Current")) in.iCurrent = true;
                     else if (ieq(k, "it_resolution")) in.it_resolution = (int)tok_d(t, 1);
-                    else if (ieq(k, "PrintPopulation")) in.print_population = true;
+                    else if (ieq(k, "PrintPopulation")) {
+                        in.print_population = true;
+                        if (t.size() > 1) {
+                            in.print_start = tok_d(t, 1);
+                            in.print_end = tok_d(t, 2);
+                            in.print_step = tok_d(t, 3);
+                        }
+                    }
                     else if (ieq(k, "Coulomb_band_reconstruction")) in.coulomb = true;
                     else if (ieq(k, "TransientAbsorption") || ieq(k, "TAbs")) in.iTAbs = true;
                     else if (ieq(k, "kresolvedTAbs") || ieq(k, "kTAbs") || ieq(k, "kTransientAbsorption")) in.iTAbsK = true;
@@ -489,6 +496,7 @@ Input parse_input(const std::string &path) {
                     if (ieq(k, "Read_Coulomb_from_files")) in.coulomb_read = true;
                     if (ieq(k, "labelInput") && t.size() > 1) in.coulomb_label = t[1];
                     if (ieq(k, "Ncut")) in.Ncut = (int)tok_d(t, 1);
+                    if (ieq(k, "Print_new_band_dispersion")) in.print_band_dispersion = true;
                 }
                 if (!r.getline(s)) fail("Can't find the end of Coulomb section.\n");
             }

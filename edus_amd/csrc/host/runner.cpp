@@ -47,6 +47,10 @@ struct Writers {
     std::vector<double> pop;
     double n_cond = 0.0;      // largest printed population (Observables_MPI.h:413-420)
     double p_cond_max = 0.0;  // largest excited population over k (Observables_MPI.h:330-347)
+    // k-resolved snapshots (if_resolution.cpp:19-35, 232-273)
+    std::ofstream fTEk;
+    int icont = 0;
+    std::vector<int> print_order;  // Create_sorted_indices_k_MPI (Observables_MPI.h:617-657)
 
     Writers(const Setup &st, const std::string &outdir) : s(st), od(outdir + "/Output/"), pop(st.nb) {
         mkdirs(od);
@@ -55,6 +59,48 @@ struct Writers {
         if (s.in.iCurrent) { fJ1.open(od + "J1.txt"); fJ2.open(od + "J2.txt"); }
         if (s.in.iTAbs) fT.open(od + "TransientAbs.txt");
         if (!fE || !fL) throw Error("cannot open output files under " + od);
+        if (s.in.print_band_dispersion || s.in.print_population) {
+            fTEk.open(od + "t_Ek.txt");
+            if (s.in.print_population && s.ncc >= s.nb)
+                throw Error("PrintPopulation needs a conduction band (valence/conduction snapshot elements)");
+            sorted_order();
+        }
+    }
+
+    // Rows of the grid sorted by one crystal coordinate, as the reference orders
+    // the k-resolved files (Observables_MPI.h:617-657 with 38-60).
+    void sorted_order() {
+        const int *Nk = s.in.Nk;
+        int nk_x = Nk[1] * Nk[0], nk_y = s.nk / (Nk[1] * Nk[0]), pol = 1;
+        if (Nk[1] > Nk[2]) {
+            pol = 0;
+            nk_y = s.nk / (Nk[1] * Nk[2]);
+            nk_x = Nk[1] * Nk[2];
+        }
+        std::vector<double> kt(nk_y, 0.0);
+        for (int ik = 0; ik < s.nk; ik++)
+            if (ik % nk_x == 0 && ik / nk_x < nk_y) kt[ik / nk_x] = s.kpt[(size_t)ik * 3 + pol];
+        std::vector<int> kidx(nk_y);
+        for (int r = 0; r < nk_y; r++) {
+            int best = 0;
+            double mn = 1e15;
+            for (int i = 0; i < nk_y; i++)
+                if (kt[i] < mn) { mn = kt[i]; best = i; }
+            kt[best] = 1e15;
+            kidx[r] = best;
+        }
+        print_order.resize(s.nk);
+        for (int ik = 0; ik < s.nk; ik++) print_order[ik] = kidx[ik / nk_x] * nk_x + ik % nk_x;
+    }
+
+    void snapshot(edus_ctx *ctx, const char *label, int ic, int jc) {
+        std::vector<double> v((size_t)2 * s.nk);
+        if (edus_eigen_element(ctx, ic, jc, v.data()) != 0) throw Error(std::string("edus_eigen_element: ") + edus_last_error());
+        std::ofstream f(od + label + "_" + std::to_string(ic) + std::to_string(jc) + "_" + std::to_string(icont) + ".txt");
+        for (int ik = 0; ik < s.nk; ik++) {
+            const int q = print_order[ik];
+            f << v[2 * (size_t)q] << " " << v[2 * (size_t)q + 1] << std::endl;
+        }
     }
 
     void observe(edus_ctx *ctx, double tl, const double ef_last[6], int verbose) {
@@ -63,6 +109,13 @@ struct Writers {
         fE << std::setw(15) << std::setprecision(8) << tl * tfs;
         for (int i = 0; i < 3; i++) fE << std::setw(15) << std::setprecision(8) << ef_last[i];
         fE << std::endl;
+        bool print_k = false;
+        if ((s.in.print_band_dispersion || s.in.print_population) && tl > s.in.print_start + icont * s.in.print_step &&
+            tl < s.in.print_end) {
+            print_k = true;
+            icont++;
+            fTEk << tl * tfs << std::endl;
+        }
         int rc = edus_observables(ctx, pop.data(), j1, j2, ta, pm);
         if (rc != 0) throw Error(std::string("edus_observables: ") + edus_last_error());
         double trace = -s.ncc;
@@ -103,6 +156,20 @@ struct Writers {
                 fT << std::setw(15) << std::setprecision(8) << ta[2 * x + 1] / (double)s.nk;
             }
             fT << std::endl;
+        }
+        if (print_k && s.in.coulomb) {  // Edia_<icont>.txt (if_resolution.cpp:40-55, Observables_MPI.h:440-477)
+            std::vector<double> ek((size_t)s.nk * s.nb);
+            if (edus_renorm_energies(ctx, ek.data()) != 0) throw Error(std::string("edus_renorm_energies: ") + edus_last_error());
+            std::ofstream f(od + "Edia_" + std::to_string(icont) + ".txt");
+            for (int ik = 0; ik < s.nk; ik++) {
+                for (int ic = 0; ic < s.nb; ic++) f << std::setprecision(20) << ek[(size_t)ik * s.nb + ic] << " ";
+                f << std::endl;
+            }
+        }
+        if (print_k && s.in.print_population) {
+            snapshot(ctx, "P_val_cond_in_Eigenstate", s.ncc - 1, s.ncc);
+            snapshot(ctx, "P_val_in_Eigenstate", s.ncc - 1, s.ncc - 1);
+            snapshot(ctx, "P_cond_in_Eigenstate", s.ncc, s.ncc);
         }
     }
 };

@@ -44,6 +44,8 @@ class SetupInfo(C.Structure):
         ("j_norm", C.c_double), ("tabs_norm", C.c_double), ("nktot", C.c_double), ("wx2", C.c_double),
         ("T1", C.c_double), ("T2", C.c_double), ("Tch", C.c_double), ("shell_residual", C.c_double),
         ("ncut", C.c_int32), ("dyn_dt", C.c_int32), ("eps_step", C.c_double),
+        ("print_flags", C.c_int32), ("reserved2", C.c_int32), ("print_start", C.c_double), ("print_end", C.c_double),
+        ("print_step", C.c_double),
     ]
 
     def as_dict(self):
@@ -81,6 +83,8 @@ for _f in (hip.edus_step_rk4, hip.edus_step_ab, hip.edus_step_taylor):
     _f.argtypes = [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_uint8)]
 hip.edus_step_dyn.argtypes = [C.c_void_p, C.c_double, _dp, C.c_uint8, _dp, C.c_double, _dp]
 hip.edus_observables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+hip.edus_eigen_element.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp]
+hip.edus_renorm_energies.argtypes = [C.c_void_p, _dp]
 hip.edus_rhs.argtypes = [C.c_void_p, _dp, C.c_uint8, _dp]
 hip.edus_get_state.argtypes = [C.c_void_p, _dp]
 hip.edus_set_state.argtypes = [C.c_void_p, _dp]
@@ -253,6 +257,17 @@ class Engine:
         _check_hip(hip.edus_step_dyn(self._h, float(dt), _dptr(fields), 1 if grad_on else 0,
                                      None if ti is None else _dptr(ti), float(cond), _dptr(out)), "edus_step_dyn")
         return float(out[0])
+
+    def eigen_element(self, ic, jc):
+        """(conj(U) P U^T)[ic][jc] per k."""
+        out = np.zeros(self.nk, dtype=np.complex128)
+        _check_hip(hip.edus_eigen_element(self._h, ic, jc, _dptr(out.view(np.float64))), "edus_eigen_element")
+        return out
+
+    def renorm_energies(self):
+        out = np.zeros((self.nk, self.nb))
+        _check_hip(hip.edus_renorm_energies(self._h, _dptr(out)), "edus_renorm_energies")
+        return out
 
     def observables(self):
         pop = np.zeros(self.nb)

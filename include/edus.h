@@ -148,6 +148,17 @@ int edus_observables(edus_ctx *ctx, double *pop, double *j1, double *j2, double 
  * (headers/RungeKutta.h:7-297) as a standalone call. */
 int edus_rhs(edus_ctx *ctx, const double *field6, uint8_t grad_on, double *pv);
 
+/* Element (ic, jc) of the current state in the eigenbasis, conj(U) P U^T per k
+ * (get_P_in_dia_vect, InitParameters.h:264-335; the diagonal forced real), as the
+ * k-resolved snapshots of if_resolution.cpp:232-273 print it.  out [nk] complex.
+ * U is the setup's (bare) eigenbasis, also with the Coulomb term. */
+int edus_eigen_element(edus_ctx *ctx, int32_t ic, int32_t jc, double *out);
+
+/* Eigenvalues (ascending) of the renormalised H per k at the last
+ * edus_observables call, as PrintLossesMPI_exciton stores them for the Edia
+ * files (Observables_MPI.h:286-304, 440-477).  Coulomb contexts only; out [nk][nb]. */
+int edus_renorm_energies(edus_ctx *ctx, double *out);
+
 /* Copies the current density matrix to/from the host, reference layout [nk][nb][nb]. */
 int edus_get_state(edus_ctx *ctx, double *rho);
 int edus_set_state(edus_ctx *ctx, const double *rho);

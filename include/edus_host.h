@@ -44,6 +44,9 @@ typedef struct edus_setup_info {
     int32_t ncut;                   /* Coulomb Fourier modes per direction (0: no Coulomb) */
     int32_t dyn_dt;                 /* 1: adaptive dt loop (dynamical_dt_evolution)       */
     double eps_step;                /* epsStepAbs of the adaptive loop                    */
+    int32_t print_flags;            /* bit 0 PrintPopulation, bit 1 Print_new_band_dispersion */
+    int32_t reserved2;
+    double print_start, print_end, print_step;  /* snapshot window and spacing, a.u.   */
 } edus_setup_info;
 
 /* Parses `input_path` (reference grammar, headers/ReadInput.h:332-1234) and builds

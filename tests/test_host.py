@@ -68,3 +68,18 @@ def test_missing_wannier_file(tmp_path):
             "laserpump sin2 {\ncycles 3\nwavelength 400 nm\nintensity 1e11 wcm2\npolarization 0 0 1\n}\n")
     with pytest.raises(N.EdusError, match="missing_seed_tb.dat"):
         N.Setup(_write(tmp_path, text))
+
+
+def test_print_population_options(tmp_path):
+    """PrintPopulation t0 t1 dt (a.u., ReadInput.h:1022-1028) and its defaults
+    (variables.cpp:84-86: 0, 100 and 20 times the fs->a.u. factor as written)."""
+    line = [x for x in BASE.split("\n") if x.startswith("PrintPopulation")][0]
+    # the reference's own AB5 input: a window starting after the run (no snapshots)
+    i = N.Setup(_write(tmp_path, BASE)).info
+    assert (i["print_flags"], i["print_start"], i["print_end"], i["print_step"]) == (
+        1, 45888.924403583114, 82.68274667312272, 0.8268274667312273)
+    i = N.Setup(_write(tmp_path, BASE.replace(line, "PrintPopulation"))).info
+    assert (i["print_flags"], i["print_start"], i["print_end"], i["print_step"]) == (
+        1, 0.0, 100.0 * 0.02418884326505, 20.0 * 0.02418884326505)
+    i = N.Setup(_write(tmp_path, BASE.replace(line, ""))).info
+    assert i["print_flags"] == 0
