@@ -13,7 +13,14 @@ from test_adaptive_dt import dyn_input
 pytestmark = pytest.mark.gpu
 
 
-def _compare_files(out, rows, nb, what):
+def _compare_files(out, rows, nb, what, rel=1e-10):
+    """rel: bar relative to the series max.  The step-size measure divides the change
+    of P_ii over one step (~1e-8 of P_ii under a weak field) by P_ii-sized terms, so
+    its rounding is ~1e-8 relative and the dt sequences of two correct
+    implementations agree only to that level; runs whose dt control engages often
+    are compared at the resulting shift in time.  On the mean-field input below a
+    1e-12 relative change of epsStepAbs alone moves J1 by 1.8e-10 of its max in the
+    oracle itself."""
     o = Hh.split_rows(rows, nb)
     L = np.loadtxt(os.path.join(out, "Losses.txt"), ndmin=2)
     J1 = np.loadtxt(os.path.join(out, "J1.txt"), ndmin=2)
@@ -24,9 +31,9 @@ def _compare_files(out, rows, nb, what):
     np.testing.assert_allclose(L[:, 0], o["t"], rtol=1e-9, atol=1e-12)
     for c in range(nb):
         ref = o["pop"][:, c]
-        assert np.max(np.abs(L[:, 1 + c] - ref)) <= max(1e-10 * np.max(np.abs(ref)), 1e-14), (what, "pop", c)
+        assert np.max(np.abs(L[:, 1 + c] - ref)) <= max(rel * np.max(np.abs(ref)), 1e-14), (what, "pop", c)
     j1s = max(np.max(np.abs(o["J1"])), 1e-300)
-    assert np.max(np.abs(J1[:, 1:] - o["J1"])) <= 1e-10 * j1s, (what, "J1")
+    assert np.max(np.abs(J1[:, 1:] - o["J1"])) <= rel * j1s, (what, "J1")
     tas = max(np.max(np.abs(o["TAbs"])), 1e-300)
     assert np.max(np.abs(TA[:, 4:] - o["TAbs"])) <= 5e-8 * tas, (what, "TAbs")
     efs = max(np.max(np.abs(o["EF"])), 1e-300)
@@ -58,4 +65,4 @@ def test_adaptive_taylor_with_mean_field_matches_oracle(tmp_path):
     out = str(tmp_path / "run")
     steps = s.run(out, max_steps=n)
     assert steps == len(dts)
-    _compare_files(os.path.join(out, "Output"), rows, 3, "coulomb taylor")
+    _compare_files(os.path.join(out, "Output"), rows, 3, "coulomb taylor", rel=1e-8)
