@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1089,6 +1090,156 @@ __global__ __launch_bounds__(256) void k_wannier(const int *__restrict__ R, cons
         }
 }
 
+// ================================================================ Coulomb coefficient generator
+// W(q) = sum_G V(|q+G|) e^{i (q+G).(d_i - d_j)} on a grid of q (Create_Wq_with_G_sum,
+// Coulomb.h:87-175), one thread per q point, the G list streamed through LDS in
+// tiles (an all-pairs sum: every q meets every G).  The phase is split as
+// e^{i q.beta} e^{i G.beta} with beta_p = b.(d_i - d_j) per band pair p.
+constexpr int kCgTile = 256;
+constexpr int kCgMaxPair = 6;  // nb <= 4
+
+struct CGenArgs {
+    int npair, n_g, N, mode;    // mode 0: grid q = -0.5 + i h (N per direction); 1: list of (ix, iy) on that grid
+    double h, qtf2, eps, r0, rk;
+    double b1[3], b2[3];
+    double beta1[kCgMaxPair], beta2[kCgMaxPair];
+    const double *g;            // [n_g][2] crystal offsets
+    const double *gph;          // [n_g][npair][2] e^{i G.beta_p}
+    const int *qlist;           // mode 1: [nq][2]
+};
+
+__global__ __launch_bounds__(256) void k_cgen_w(CGenArgs a, long long iq0, int nq, double *W) {
+    __shared__ double sg[kCgTile * 2];
+    __shared__ double sph[kCgTile * kCgMaxPair * 2];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < nq;
+    int ix = 0, iy = 0;
+    if (valid) {
+        if (a.mode == 0) {
+            const long long iq = iq0 + t;
+            ix = (int)(iq / a.N);
+            iy = (int)(iq % a.N);
+        } else {
+            ix = a.qlist[2 * t];
+            iy = a.qlist[2 * t + 1];
+        }
+    }
+    // Grid_and_Simpson (integrationWeight.h:5-8): q_i = -0.5 + i h
+    const double qx = -0.5 + ix * a.h, qy = -0.5 + iy * a.h;
+    double pr[kCgMaxPair], pi[kCgMaxPair], wr[kCgMaxPair], wi[kCgMaxPair];
+    for (int p = 0; p < a.npair; p++) {
+        double s, c;
+        sincos(qx * a.beta1[p] + qy * a.beta2[p], &s, &c);
+        pr[p] = c;
+        pi[p] = s;
+        wr[p] = wi[p] = 0.0;
+    }
+    double w00 = 0.0;
+    const double twopi = 2 * M_PI;
+    for (int g0 = 0; g0 < a.n_g; g0 += kCgTile) {
+        const int nt = min(kCgTile, a.n_g - g0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nt * 2; i += blockDim.x) sg[i] = a.g[(size_t)g0 * 2 + i];
+        for (int i = threadIdx.x; i < nt * a.npair * 2; i += blockDim.x) sph[i] = a.gph[(size_t)g0 * a.npair * 2 + i];
+        __syncthreads();
+        for (int j = 0; j < nt; j++) {
+            const double c1 = qx + sg[2 * j], c2 = qy + sg[2 * j + 1];
+            const double x = a.b1[0] * c1 + a.b2[0] * c2, y = a.b1[1] * c1 + a.b2[1] * c2, z = a.b1[2] * c1 + a.b2[2] * c2;
+            const double q2 = x * x + y * y + z * z;
+            // Coloumb_THh_F (Coulomb.h:60-84)
+            const double den = (a.rk != 0.0) ? a.eps * sqrt(q2 + a.qtf2) * (sqrt(q2) * a.r0 + 1.0) : a.eps * sqrt(q2 + a.qtf2);
+            const double V = twopi / den;
+            w00 += V;
+            for (int p = 0; p < a.npair; p++) {
+                const double gr = sph[(j * a.npair + p) * 2], gi = sph[(j * a.npair + p) * 2 + 1];
+                const double er = pr[p] * gr - pi[p] * gi, ei = pr[p] * gi + pi[p] * gr;
+                wr[p] += V * er;
+                wi[p] += V * ei;
+            }
+        }
+    }
+    if (!valid) return;
+    const int nw = 1 + 2 * a.npair;
+    double *o = W + (size_t)t * nw;
+    o[0] = w00;
+    for (int p = 0; p < a.npair; p++) {
+        o[1 + 2 * p] = wr[p];
+        o[2 + 2 * p] = wi[p];
+    }
+}
+
+// T[t][iy][c] = sum_ix w[ix] trig_t(ix) W[ix][iy][c], trig_t = cos(2 pi m q), t < ncut,
+// sin(2 pi (t - ncut) q) otherwise (first pass of the separable Simpson integral,
+// Fourier_Coulomb.cpp:230-265 with Coulomb.h:182-232).
+__global__ void k_cgen_pass1(const double *W, const double *wtrig, int N, int ntrig, int nw, double *T) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)ntrig * nw * N) return;
+    const int iy = (int)(id % N), c = (int)((id / N) % nw), t = (int)(id / ((long long)N * nw));
+    double s = 0.0;
+    for (int ix = 0; ix < N; ix++) s += wtrig[(size_t)t * N + ix] * W[((size_t)ix * N + iy) * nw + c];
+    T[((size_t)t * N + iy) * nw + c] = s;
+}
+
+// S[t1][t2][c] = sum_iy w[iy] trig_t2(iy) T[t1][iy][c].
+__global__ void k_cgen_pass2(const double *T, const double *wtrig, int N, int ntrig, int nw, double *S) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ntrig * ntrig * nw) return;
+    const int c = id % nw, t2 = (id / nw) % ntrig, t1 = id / (nw * ntrig);
+    double s = 0.0;
+    for (int iy = 0; iy < N; iy++) s += wtrig[(size_t)t2 * N + iy] * T[((size_t)t1 * N + iy) * nw + c];
+    S[id] = s;
+}
+
+// Correction around q = 0 (Fourier_Coulomb.cpp:375-456): per listed point,
+// (W0 - Fourier series) * dk2 for the entries ic <= jc, block partial sums.
+__global__ __launch_bounds__(256) void k_cgen_peak(const double *W0, const int *qlist, int np, int nb, int ncut, double h, double dk2,
+                                                   const double *A, const double *B, const double *C, const double *D, double *part) {
+    __shared__ double red[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb2 = nb * nb, npair = nb * (nb - 1) / 2, nw = 1 + 2 * npair;
+    double qx = 0, qy = 0;
+    if (t < np) {
+        qx = -0.5 + qlist[2 * t] * h;
+        qy = -0.5 + qlist[2 * t + 1] * h;
+    }
+    int p = 0;
+    for (int ic = 0; ic < nb; ic++)
+        for (int jc = ic; jc < nb; jc++) {
+            const int idx = ic * nb + jc;
+            double vr = 0, vi = 0;
+            if (t < np) {
+                // Fourier_series_Coloumb (Coulomb.h:235-267)
+                for (int m = 0; m < ncut; m++) {
+                    double sm, cm;
+                    sincos(2 * M_PI * m * qx, &sm, &cm);
+                    for (int n = 0; n < ncut; n++) {
+                        double sn, cn;
+                        sincos(2 * M_PI * n * qy, &sn, &cn);
+                        const size_t r = ((size_t)(n + ncut * m) * nb2 + idx) * 2;
+                        vr += A[r] * cm * cn + B[r] * cm * sn + C[r] * sm * cn + D[r] * sm * sn;
+                        vi += A[r + 1] * cm * cn + B[r + 1] * cm * sn + C[r + 1] * sm * cn + D[r + 1] * sm * sn;
+                    }
+                }
+                const double *w = W0 + (size_t)t * nw;
+                double wr = w[0], wi = 0.0;
+                if (jc != ic) { wr = w[1 + 2 * p]; wi = w[2 + 2 * p]; }
+                vr = (wr - vr) * dk2;
+                vi = (wi - vi) * dk2;
+            }
+            if (jc != ic) p++;
+            for (int comp = 0; comp < 2; comp++) {
+                red[threadIdx.x] = comp ? vi : vr;
+                __syncthreads();
+                for (int s = 128; s > 0; s >>= 1) {
+                    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+                    __syncthreads();
+                }
+                if (threadIdx.x == 0) part[((size_t)blockIdx.x * nb2 + idx) * 2 + comp] = red[0];
+                __syncthreads();
+            }
+        }
+}
+
 }  // namespace
 
 // ==================================================================== context
@@ -1789,6 +1940,211 @@ int edus_last_step_time(edus_ctx *c, double *ms, int32_t *nlaunch) {
     if (!c) return set_err(EDUS_ERR_INVALID, "null context");
     if (ms) *ms = c->last_ms;
     if (nlaunch) *nlaunch = c->last_launches;
+    return 0;
+}
+
+int edus_coulomb_coefficients(const edus_coulomb_gen *g, double *Ao, double *Bo, double *Co, double *Do, double *screen, double *vh) {
+    if (!g || !Ao || !Bo || !Co || !Do || !screen || !vh || !g->g || !g->disp) return set_err(EDUS_ERR_INVALID, "null argument");
+    const int nb = g->nb, nc = g->ncut, N = g->n_sum, Np = g->n_peak;
+    if (nb < 1 || nb > 4) return set_err(EDUS_ERR_UNSUPPORTED, "the coefficient generator supports 1..4 bands");
+    if (nc < 1 || N < 3 || Np < 3 || g->n_g < 1) return set_err(EDUS_ERR_INVALID, "ncut, n_sum, n_peak, n_g out of range");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(EDUS_ERR_NODEVICE, "no HIP device visible");
+    const int nb2 = nb * nb, npair = nb * (nb - 1) / 2, nw = 1 + 2 * npair, nmm = nc * nc, ntrig = 2 * nc;
+    const double twopi = 2 * M_PI;
+    auto vq = [&](double q2, double qtf) {
+        return (g->rytova_keldysh ? g->eps_area * std::sqrt(q2 + qtf * qtf) * (std::sqrt(q2) * g->r0 + 1.0)
+                                  : g->eps_area * std::sqrt(q2 + qtf * qtf));
+    };
+    // band pairs ic < jc and their beta = b.(d_ic - d_jc)
+    std::vector<int> pic, pjc;
+    CGenArgs a;
+    std::memset(&a, 0, sizeof a);
+    for (int ic = 0; ic < nb; ic++)
+        for (int jc = ic + 1; jc < nb; jc++) {
+            const int p = (int)pic.size();
+            pic.push_back(ic);
+            pjc.push_back(jc);
+            double b1d = 0, b2d = 0;
+            for (int x = 0; x < 3; x++) {
+                const double dd = g->disp[ic * 3 + x] - g->disp[jc * 3 + x];
+                b1d += g->b1[x] * dd;
+                b2d += g->b2[x] * dd;
+            }
+            a.beta1[p] = b1d;
+            a.beta2[p] = b2d;
+        }
+    // Hartree term at q = 0 on the host (Fourier_Coulomb.cpp:173-193): V = 0 where
+    // the G offsets sum to <= 0, as written there.
+    {
+        double w00 = 0;
+        std::vector<std::complex<double>> wp(npair, 0.0);
+        for (int ig = 0; ig < g->n_g; ig++) {
+            const double gx = g->g[2 * ig], gy = g->g[2 * ig + 1];
+            double cart[3];
+            for (int x = 0; x < 3; x++) cart[x] = g->b1[x] * gx + g->b2[x] * gy;
+            const double q2 = cart[0] * cart[0] + cart[1] * cart[1] + cart[2] * cart[2];
+            const double V = ((gx + gy) < 1e-16) ? 0.0 : twopi / vq(q2, g->qtf);
+            w00 += V;
+            for (int p = 0; p < npair; p++) {
+                double phi = 0;
+                for (int x = 0; x < 3; x++) phi += cart[x] * (g->disp[pic[p] * 3 + x] - g->disp[pjc[p] * 3 + x]);
+                wp[p] += V * std::exp(std::complex<double>(0.0, phi));
+            }
+        }
+        for (int i = 0; i < nb2 * 2; i++) vh[i] = 0.0;
+        for (int ic = 0; ic < nb; ic++) vh[(ic * nb + ic) * 2] = w00;
+        for (int p = 0; p < npair; p++) {
+            const double r = wp[p].real();
+            vh[(pic[p] * nb + pjc[p]) * 2] = r;
+            vh[(pjc[p] * nb + pic[p]) * 2] = r;
+        }
+    }
+    // per-G data: offsets and e^{i G.beta_p}
+    std::vector<double> gd((size_t)g->n_g * 2), gph((size_t)g->n_g * std::max(npair, 1) * 2, 0.0);
+    for (int ig = 0; ig < g->n_g; ig++) {
+        const double gx = g->g[2 * ig], gy = g->g[2 * ig + 1];
+        gd[2 * ig] = gx;
+        gd[2 * ig + 1] = gy;
+        for (int p = 0; p < npair; p++) {
+            const double ph = gx * a.beta1[p] + gy * a.beta2[p];
+            gph[((size_t)ig * npair + p) * 2] = std::cos(ph);
+            gph[((size_t)ig * npair + p) * 2 + 1] = std::sin(ph);
+        }
+    }
+    // Simpson weights and trig tables (integrationWeight.h:5-18)
+    auto simpson = [](int n, std::vector<double> &q, std::vector<double> &w) {
+        q.resize(n);
+        w.resize(n);
+        const double h = 2 * 0.5 / (n - 1), hs = 3 * (n - 1);
+        for (int i = 0; i < n; i++) q[i] = -0.5 + i * h;
+        w[0] = 1 / hs;
+        for (int i = 1; i < n; i += 2) w[i] = 4.0 / hs;
+        for (int i = 2; i < n; i += 2) w[i] = 2.0 / hs;
+        w[n - 1] = 1 / hs;
+    };
+    std::vector<double> q, w;
+    simpson(N, q, w);
+    std::vector<double> wtrig((size_t)ntrig * N);
+    for (int t = 0; t < ntrig; t++)
+        for (int i = 0; i < N; i++) {
+            const int m = t < nc ? t : t - nc;
+            const double arg = 2 * M_PI * m * q[i];
+            wtrig[(size_t)t * N + i] = w[i] * (t < nc ? std::cos(arg) : std::sin(arg));
+        }
+    const size_t nq = (size_t)N * N;
+    double *dG = nullptr, *dPh = nullptr, *dW = nullptr, *dTr = nullptr, *dT = nullptr, *dS = nullptr, *dCoef = nullptr, *dW0 = nullptr,
+           *dPart = nullptr;
+    int *dQl = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {(void *)dG, (void *)dPh, (void *)dW, (void *)dTr, (void *)dT, (void *)dS, (void *)dCoef, (void *)dW0, (void *)dPart,
+                        (void *)dQl})
+            (void)hipFree(p);
+    };
+    auto fail = [&](int code, const std::string &m) {
+        cleanup();
+        return set_err(code, m);
+    };
+    if (hipMalloc(&dG, gd.size() * 8) != hipSuccess || hipMalloc(&dPh, gph.size() * 8) != hipSuccess ||
+        hipMalloc(&dW, nq * nw * 8) != hipSuccess || hipMalloc(&dTr, wtrig.size() * 8) != hipSuccess ||
+        hipMalloc(&dT, (size_t)ntrig * N * nw * 8) != hipSuccess || hipMalloc(&dS, (size_t)ntrig * ntrig * nw * 8) != hipSuccess ||
+        hipMalloc(&dCoef, (size_t)4 * nmm * nb2 * 2 * 8) != hipSuccess)
+        return fail(EDUS_ERR_NOMEM, "coefficient generator buffers");
+    if (hipMemcpy(dG, gd.data(), gd.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dPh, gph.data(), gph.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dTr, wtrig.data(), wtrig.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(EDUS_ERR_HIP, "coefficient generator upload");
+    a.npair = npair; a.n_g = g->n_g; a.N = N; a.mode = 0; a.h = 2 * 0.5 / (N - 1);
+    a.qtf2 = g->qtf * g->qtf; a.eps = g->eps_area; a.r0 = g->r0; a.rk = g->rytova_keldysh ? 1.0 : 0.0;
+    for (int x = 0; x < 3; x++) { a.b1[x] = g->b1[x]; a.b2[x] = g->b2[x]; }
+    a.g = dG; a.gph = dPh; a.qlist = nullptr;
+    // W on the Simpson grid, in chunks with a progress line each
+    const long long chunk = 1LL << 21;
+    for (long long q0 = 0; q0 < (long long)nq; q0 += chunk) {
+        const int n = (int)std::min<long long>(chunk, (long long)nq - q0);
+        hipLaunchKernelGGL(k_cgen_w, dim3((n + 255) / 256), dim3(256), 0, 0, a, q0, n, dW + (size_t)q0 * nw);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail(EDUS_ERR_HIP, "k_cgen_w");
+        if (g->verbose) std::fprintf(stderr, "Coulomb coefficients: %lld / %zu q points\n", q0 + n, nq);
+    }
+    {
+        const long long n1 = (long long)ntrig * nw * N;
+        hipLaunchKernelGGL(k_cgen_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, 0, (const double *)dW, (const double *)dTr, N, ntrig, nw, dT);
+        const int n2 = ntrig * ntrig * nw;
+        hipLaunchKernelGGL(k_cgen_pass2, dim3((n2 + 255) / 256), dim3(256), 0, 0, (const double *)dT, (const double *)dTr, N, ntrig, nw, dS);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail(EDUS_ERR_HIP, "k_cgen_pass");
+    }
+    std::vector<double> S((size_t)ntrig * ntrig * nw);
+    if (hipMemcpy(S.data(), dS, S.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return fail(EDUS_ERR_HIP, "download S");
+    auto sv = [&](int t1, int t2, int c) { return S[((size_t)t1 * ntrig + t2) * nw + c]; };
+    for (size_t i = 0; i < (size_t)nmm * nb2 * 2; i++) Ao[i] = Bo[i] = Co[i] = Do[i] = 0.0;
+    for (int mm = 0; mm < nmm; mm++) {
+        const int m = mm / nc, n = mm % nc;
+        const double sc = 4.0 * ((m == 0 && n == 0) ? 0.25 : ((m == 0 || n == 0) ? 0.5 : 1.0));
+        Ao[(size_t)mm * nb2 * 2] = sc * sv(m, n, 0);
+        Do[(size_t)mm * nb2 * 2] = sc * sv(nc + m, nc + n, 0);
+        for (int p = 0; p < npair; p++) {
+            const size_t o = ((size_t)mm * nb2 + pic[p] * nb + pjc[p]) * 2;
+            Ao[o] = sc * sv(m, n, 1 + 2 * p);
+            Bo[o + 1] = sc * sv(m, nc + n, 2 + 2 * p);
+            Co[o + 1] = sc * sv(nc + m, n, 2 + 2 * p);
+            Do[o] = sc * sv(nc + m, nc + n, 1 + 2 * p);
+        }
+    }
+    // correction around q = 0: points of the n_peak grid with 1e-17 < |q|^2 < q_peak^2
+    for (int i = 0; i < nb2 * 2; i++) screen[i] = 0.0;
+    const double hp = 2 * 0.5 / (Np - 1);
+    std::vector<int> ql;
+    {
+        const double g11 = g->b1[0] * g->b1[0] + g->b1[1] * g->b1[1] + g->b1[2] * g->b1[2];
+        const double g22 = g->b2[0] * g->b2[0] + g->b2[1] * g->b2[1] + g->b2[2] * g->b2[2];
+        const double g12 = g->b1[0] * g->b2[0] + g->b1[1] * g->b2[1] + g->b1[2] * g->b2[2];
+        const double lmin = 0.5 * (g11 + g22) - std::sqrt(0.25 * (g11 - g22) * (g11 - g22) + g12 * g12);
+        const double rc = (lmin > 0) ? g->q_peak / std::sqrt(lmin) : 1.0;
+        const int lo = std::max(0, (int)std::floor((0.5 - rc) / hp) - 2), hi = std::min(Np - 1, (int)std::ceil((0.5 + rc) / hp) + 2);
+        const double qp2 = g->q_peak * g->q_peak;
+        for (int ix = lo; ix <= hi; ix++)
+            for (int iy = lo; iy <= hi; iy++) {
+                const double qx = -0.5 + ix * hp, qy = -0.5 + iy * hp;
+                double cart[3];
+                for (int x = 0; x < 3; x++) cart[x] = g->b1[x] * qx + g->b2[x] * qy;
+                const double q2 = cart[0] * cart[0] + cart[1] * cart[1] + cart[2] * cart[2];
+                if (q2 < qp2 && q2 > std::pow(10, -17)) {
+                    ql.push_back(ix);
+                    ql.push_back(iy);
+                }
+            }
+    }
+    const int npk = (int)(ql.size() / 2);
+    if (npk > 0) {
+        const int nblk = (npk + 255) / 256;
+        std::vector<double> coef((size_t)4 * nmm * nb2 * 2);
+        std::copy(Ao, Ao + (size_t)nmm * nb2 * 2, coef.begin());
+        std::copy(Bo, Bo + (size_t)nmm * nb2 * 2, coef.begin() + (size_t)nmm * nb2 * 2);
+        std::copy(Co, Co + (size_t)nmm * nb2 * 2, coef.begin() + (size_t)2 * nmm * nb2 * 2);
+        std::copy(Do, Do + (size_t)nmm * nb2 * 2, coef.begin() + (size_t)3 * nmm * nb2 * 2);
+        if (hipMalloc(&dQl, ql.size() * sizeof(int)) != hipSuccess || hipMalloc(&dW0, (size_t)npk * nw * 8) != hipSuccess ||
+            hipMalloc(&dPart, (size_t)nblk * nb2 * 2 * 8) != hipSuccess)
+            return fail(EDUS_ERR_NOMEM, "peak buffers");
+        if (hipMemcpy(dQl, ql.data(), ql.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dCoef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(dPart, 0, (size_t)nblk * nb2 * 2 * 8) != hipSuccess)
+            return fail(EDUS_ERR_HIP, "peak upload");
+        CGenArgs b = a;
+        b.mode = 1; b.N = Np; b.h = hp; b.qtf2 = 0.0; b.qlist = dQl;
+        hipLaunchKernelGGL(k_cgen_w, dim3(nblk), dim3(256), 0, 0, b, 0LL, npk, dW0);
+        const size_t cs = (size_t)nmm * nb2 * 2;
+        hipLaunchKernelGGL(k_cgen_peak, dim3(nblk), dim3(256), 0, 0, (const double *)dW0, (const int *)dQl, npk, nb, nc, hp,
+                           1. / ((double)Np * Np), (const double *)dCoef, (const double *)(dCoef + cs), (const double *)(dCoef + 2 * cs),
+                           (const double *)(dCoef + 3 * cs), dPart);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail(EDUS_ERR_HIP, "peak kernels");
+        std::vector<double> part((size_t)nblk * nb2 * 2);
+        if (hipMemcpy(part.data(), dPart, part.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return fail(EDUS_ERR_HIP, "download peak");
+        for (int b2i = 0; b2i < nblk; b2i++)
+            for (int i = 0; i < nb2 * 2; i++) screen[i] += part[(size_t)b2i * nb2 * 2 + i];
+        if (g->verbose) std::fprintf(stderr, "Coulomb coefficients: %d points in the q ~ 0 correction\n", npk);
+    }
+    cleanup();
+    g_err.clear();
     return 0;
 }
 

@@ -173,7 +173,10 @@ struct Setup {
     // Coulomb mean field (read from the coefficient files)
     int ncut = 0;
     std::vector<cplx> cA, cB, cC, cD, cscreen, cvh;
+    std::string coulomb_label_written;  // label of generated coefficient files
     void read_coulomb();
+    std::vector<double> disp_at_reference_point() const;
+    void generate_coulomb(const std::vector<double> &disp);
 
     void build(const std::string &input_path, int flags);
     void fields_at(double t, double ef[6]) const;   // pump sum + 0.5 probe

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iomanip>
+#include <sys/stat.h>
 #include <fstream>
 #include <omp.h>
 
@@ -22,11 +24,8 @@ static inline size_t mi(int nb, int k, int i, int j) { return ((size_t)k * nb + 
 
 void Setup::build(const std::string &input_path, int flags) {
     in = parse_input(input_path);
-    if (in.coulomb) {
-        if (in.coulomb_diag_basis) throw Error("Coulomb_diag_basis is not supported by this build (Wannier basis only)");
-        if (!in.coulomb_read)
-            throw Error("the Coulomb coefficients must be read from files (Read_Coulomb_from_files); computing them is not in this build");
-    }
+    if (in.coulomb && in.coulomb_diag_basis)
+        throw Error("Coulomb_diag_basis is not supported by this build (Wannier basis only)");
     if (in.iMode != "TB" && in.iMode != "W") throw Error("iMode = " + in.iMode + " is incorrect");
 
     L.set_from_b(in.b);
@@ -57,7 +56,7 @@ void Setup::build(const std::string &input_path, int flags) {
     if (in.coulomb) {
         if (in.Nk[0] != 1 || in.Nk[2] < in.Nk[1]) throw Error("the Coulomb term needs a 1 x N1 x N2 grid in the yz plane");
         if (nb > 4) throw Error("the Coulomb term is implemented for up to 4 bands");
-        read_coulomb();
+        if (in.coulomb_read) read_coulomb();
     }
     sh = find_shells(kpt, in.Nk, L);
     nbr = neighbours_closed_form(in.Nk, sh);
@@ -136,6 +135,7 @@ void Setup::build(const std::string &input_path, int flags) {
             for (int i = 0; i < nb * nb; i++) H[(size_t)ik * nb * nb + i] += add[i];
         }
     }
+    if (in.coulomb && !in.coulomb_read) generate_coulomb(disp_at_reference_point());
     // GradientEnergy for Current1 (save_HUD.cpp:236-254): sum_shell,b w*b_x*(H(k+b)-H(k)).
     if (in.iCurrent) {
         gradH.assign((size_t)nk * nb * nb * 3, 0.);
@@ -206,6 +206,109 @@ void Setup::build(const std::string &input_path, int flags) {
     times.resize((size_t)itfi + 2);
     double t = t_start;
     for (int64_t it = 0; it <= itfi + 1; it++) { times[it] = t; t += in.dt; }
+}
+
+// Orbital positions for the Coulomb phases: the real diagonal of the dipole at
+// crystal k = (0, 0.3, 0.1) (Fourier_Coulomb.cpp:43-65, "011" orientation).
+std::vector<double> Setup::disp_at_reference_point() const {
+    const double k[3] = {0.0, 0.3, 0.1};
+    std::vector<double> d((size_t)nb * 3, 0.0);
+    if (in.iMode == "TB") {
+        TBModel tb;
+        tb.init(in.TBtype, nch, nb, in.hopping);
+        for (int i = 0; i < nb; i++)
+            for (int x = 0; x < 3; x++) d[(size_t)i * 3 + x] = tb.dipole(i, i, x, k).real();
+    } else {
+        WannierModel w;
+        std::string seed = in.ref_coord;
+        w.read_tb((seed.size() && seed[0] == '/') ? seed + "_tb.dat" : in.base_dir + "/" + seed + "_tb.dat");
+        const double twopi = 2 * Units::pi();
+        for (int r = 0; r < w.nR; r++) {
+            const double ph = twopi * (k[0] * w.R[3 * r] + k[1] * w.R[3 * r + 1] + k[2] * w.R[3 * r + 2]);
+            const cplx e(std::cos(ph), std::sin(ph));
+            for (int i = 0; i < nb; i++)
+                for (int x = 0; x < 3; x++) d[(size_t)i * 3 + x] += (e * w.rR[(((size_t)r * nb + i) * nb + i) * 3 + x]).real();
+        }
+    }
+    return d;
+}
+
+// Fourier_Coulomb.cpp:95-500: the G shell |G| < G_distance sqrt(det_b) scanned as
+// there (offsets -4999..4999, first offset outer), the reference's fixed grids
+// (4999 Simpson points per direction, 39999 for the q ~ 0 correction), the
+// coefficients on the GPU, then the coefficient files under Output/Coulomb/ with
+// the label G_num<n>_Ncut<ncut><labelInput>.
+void Setup::generate_coulomb(const std::vector<double> &disp) {
+    ncut = in.Ncut;
+    const double gdist = in.G_distance * std::sqrt(det_b);
+    std::vector<int32_t> g;
+    const int bord = 4999;
+    for (int ix = -bord; ix <= bord; ix++)
+        for (int iy = -bord; iy <= bord; iy++) {
+            const double crys[3] = {0.0, (double)ix, (double)iy};
+            double cart[3];
+            L.to_cart(crys, cart);
+            if (std::sqrt(cart[0] * cart[0] + cart[1] * cart[1] + cart[2] * cart[2]) < gdist) {
+                g.push_back(ix);
+                g.push_back(iy);
+            }
+        }
+    edus_coulomb_gen p;
+    std::memset(&p, 0, sizeof p);
+    p.nb = nb;
+    p.ncut = ncut;
+    p.n_sum = 4999;
+    p.n_peak = 39999;
+    p.rytova_keldysh = in.rytova_keldysh ? 1 : 0;
+    p.n_g = (int32_t)(g.size() / 2);
+    p.g = g.data();
+    for (int x = 0; x < 3; x++) { p.b1[x] = in.b[1][x]; p.b2[x] = in.b[2][x]; }
+    p.eps_area = in.epsilon_static * det_a;
+    p.qtf = in.qTF;
+    p.r0 = in.r0;
+    p.q_peak = (in.qTF < 0.02) ? 0.02 : in.qTF;
+    p.disp = disp.data();
+    p.verbose = 1;
+    const int nb2 = nb * nb, nmm = ncut * ncut;
+    cA.assign((size_t)nmm * nb2, 0.);
+    cB = cA; cC = cA; cD = cA;
+    cscreen.assign(nb2, 0.);
+    cvh.assign(nb2, 0.);
+    int rc = edus_coulomb_coefficients(&p, reinterpret_cast<double *>(cA.data()), reinterpret_cast<double *>(cB.data()),
+                                       reinterpret_cast<double *>(cC.data()), reinterpret_cast<double *>(cD.data()),
+                                       reinterpret_cast<double *>(cscreen.data()), reinterpret_cast<double *>(cvh.data()));
+    if (rc != 0) throw Error(std::string("Coulomb coefficients on the GPU failed: ") + edus_last_error());
+    const std::string dir = in.base_dir + "/Output/Coulomb/";
+    ::mkdir((in.base_dir + "/Output").c_str(), 0755);
+    ::mkdir(dir.c_str(), 0755);
+    const std::string label = "G_num" + std::to_string(p.n_g) + "_Ncut" + std::to_string(ncut) + in.coulomb_label;
+    coulomb_label_written = label;
+    {   // V_Hartree (Fourier_Coulomb.cpp:197-208)
+        std::ofstream f(dir + "V_Hartree_" + label + ".txt");
+        if (!f) throw Error("cannot write " + dir);
+        for (int ic = 0; ic < nb; ic++) {
+            for (int jc = 0; jc < nb; jc++) f << std::setprecision(20) << cvh[(size_t)ic * nb + jc].real() << "    ";
+            f << std::endl;
+        }
+    }
+    {   // A..D (Fourier_Coulomb.cpp:341-367)
+        std::ofstream fa(dir + "A_coeff" + label + ".txt"), fb(dir + "B_coeff" + label + ".txt");
+        std::ofstream fc(dir + "C_coeff" + label + ".txt"), fd(dir + "D_coeff" + label + ".txt");
+        std::ofstream *fs[4] = {&fa, &fb, &fc, &fd};
+        const std::vector<cplx> *vs[4] = {&cA, &cB, &cC, &cD};
+        for (int mm = 0; mm < nmm; mm++) {
+            for (int i = 0; i < nb2; i++)
+                for (int q = 0; q < 4; q++) {
+                    const cplx v = (*vs[q])[(size_t)mm * nb2 + i];
+                    *fs[q] << std::setprecision(20) << std::scientific << v.real() << "    " << v.imag() << "    ";
+                }
+            for (int q = 0; q < 4; q++) *fs[q] << std::endl;
+        }
+    }
+    {   // Screen_const (Fourier_Coulomb.cpp:469-478)
+        std::ofstream f(dir + "Screen_const_" + label + ".txt");
+        for (int i = 0; i < nb2; i++) f << std::setprecision(20) << cscreen[i].real() << " " << cscreen[i].imag() << std::endl;
+    }
 }
 
 // Coefficient files of Fourier_Coulomb.cpp:505-622: V_Hartree (nb lines of nb

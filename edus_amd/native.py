@@ -56,6 +56,15 @@ class SetupInfo(C.Structure):
         return d
 
 
+class CoulombGen(C.Structure):
+    """edus_coulomb_gen of include/edus.h."""
+    _fields_ = [("nb", C.c_int32), ("ncut", C.c_int32), ("n_sum", C.c_int32), ("n_peak", C.c_int32),
+                ("rytova_keldysh", C.c_int32), ("n_g", C.c_int32), ("g", C.POINTER(C.c_int32)),
+                ("b1", C.c_double * 3), ("b2", C.c_double * 3), ("eps_area", C.c_double), ("qtf", C.c_double),
+                ("r0", C.c_double), ("q_peak", C.c_double), ("disp", C.POINTER(C.c_double)), ("verbose", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class Pulse(C.Structure):
     _fields_ = [
         ("gaussian", C.c_int32), ("sin2", C.c_int32), ("pw", C.c_int32), ("envelope", C.c_int32),
@@ -83,6 +92,7 @@ for _f in (hip.edus_step_rk4, hip.edus_step_ab, hip.edus_step_taylor):
     _f.argtypes = [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_uint8)]
 hip.edus_step_dyn.argtypes = [C.c_void_p, C.c_double, _dp, C.c_uint8, _dp, C.c_double, _dp]
 hip.edus_observables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+hip.edus_coulomb_coefficients.argtypes = [C.POINTER(CoulombGen), _dp, _dp, _dp, _dp, _dp, _dp]
 hip.edus_eigen_element.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp]
 hip.edus_renorm_energies.argtypes = [C.c_void_p, _dp]
 hip.edus_rhs.argtypes = [C.c_void_p, _dp, C.c_uint8, _dp]
@@ -316,3 +326,26 @@ def write_synthetic_tb(path, nw, rmax, a_ang=2.5, sigma_h_ev=1.5, sigma_r_ang=0.
     rc = host.edus_write_synthetic_tb(os.fsencode(path), nw, rmax, a_ang, sigma_h_ev, sigma_r_ang, decay_ang, seed)
     if rc != 0:
         raise EdusError(host.edus_host_last_error().decode())
+
+
+def coulomb_coefficients(p, verbose=False):
+    """Coulomb mean-field coefficients on the GPU (edus_coulomb_coefficients); p: dict
+    with nb, ncut, n_sum, n_peak, rytova, g [n_g][2], b1, b2, eps_area, qtf, r0,
+    q_peak, disp [nb][3].  Returns A..D [ncut^2][nb^2], screen [nb^2], vh [nb][nb]."""
+    g = np.ascontiguousarray(p["g"], dtype=np.int32).reshape(-1, 2)
+    d = np.ascontiguousarray(p["disp"], dtype=np.float64).reshape(-1, 3)
+    o = CoulombGen()
+    o.nb, o.ncut, o.n_sum, o.n_peak, o.rytova_keldysh, o.n_g = p["nb"], p["ncut"], p["n_sum"], p["n_peak"], p["rytova"], len(g)
+    o.g = g.ctypes.data_as(C.POINTER(C.c_int32))
+    for i in range(3):
+        o.b1[i], o.b2[i] = p["b1"][i], p["b2"][i]
+    o.eps_area, o.qtf, o.r0, o.q_peak = p["eps_area"], p["qtf"], p["r0"], p["q_peak"]
+    o.disp = d.ctypes.data_as(C.POINTER(C.c_double))
+    o.verbose = 1 if verbose else 0
+    nb2, nmm = p["nb"] ** 2, p["ncut"] ** 2
+    out = [np.zeros((nmm, nb2), dtype=np.complex128) for _ in range(4)]
+    sc = np.zeros(nb2, dtype=np.complex128)
+    vh = np.zeros((p["nb"], p["nb"]), dtype=np.complex128)
+    _check_hip(hip.edus_coulomb_coefficients(C.byref(o), *[_dptr(x.view(np.float64)) for x in out], _dptr(sc.view(np.float64)),
+                                             _dptr(vh.view(np.float64))), "edus_coulomb_coefficients")
+    return {"A": out[0], "B": out[1], "C": out[2], "D": out[3], "screen": sc, "vh": vh}

@@ -167,6 +167,32 @@ int edus_set_state(edus_ctx *ctx, const double *rho);
  * with HIP events on the context's stream; *nlaunch receives the kernel count. */
 int edus_last_step_time(edus_ctx *ctx, double *ms, int32_t *nlaunch);
 
+/* Parameters of the Coulomb coefficient generator (Source_Main/Fourier_Coulomb.cpp:95-500
+ * with headers/Coulomb.h:60-267 and integrationWeight.h:5-76), "011" orientation:
+ * the sample lies in the plane of crystal directions 1 and 2, q = b1 q1 + b2 q2. */
+typedef struct edus_coulomb_gen {
+    int32_t nb, ncut;
+    int32_t n_sum;          /* Simpson points per direction of the main integral (reference: 4999) */
+    int32_t n_peak;         /* points per direction of the q ~ 0 correction (reference: 39999)  */
+    int32_t rytova_keldysh; /* 1: V = 2 pi / (eps sqrt(q^2+qTF^2) (|q| r0 + 1)); 0: without (|q| r0 + 1) */
+    int32_t n_g;            /* lattice vectors G of the sum                                        */
+    const int32_t *g;       /* [n_g][2] integer offsets of G along crystal directions 1 and 2     */
+    double b1[3], b2[3];    /* cartesian reciprocal vectors of those directions                     */
+    double eps_area;        /* epsilon_static times the in-plane cell area (Fourier_Coulomb.cpp:2) */
+    double qtf, r0;         /* Thomas-Fermi wave vector, Rytova-Keldysh length (a.u.)              */
+    double q_peak;          /* radius of the q ~ 0 correction: max(qTF, 0.02)                      */
+    const double *disp;     /* [nb][3] orbital positions: real diagonal of D at k = (0, 0.3, 0.1)  */
+    int32_t verbose;        /* progress lines on stderr                                            */
+    int32_t reserved;
+} edus_coulomb_gen;
+
+/* The mean-field coefficients: A..D [ncut^2][nb^2], screen [nb^2] (Screen_const[0])
+ * and vh [nb][nb] (V_Hartree), all complex, as Fourier_Coulomb.cpp:95-500 computes
+ * and then writes them (A_coeff, B_coeff, C_coeff, D_coeff, Screen_const_,
+ * V_Hartree_ files).  The lattice sum over G for each of the n_sum^2 Simpson points
+ * runs on the GPU.  Independent of any context; uses the current HIP device. */
+int edus_coulomb_coefficients(const edus_coulomb_gen *g, double *A, double *B, double *C, double *D, double *screen, double *vh);
+
 /* Wannier phase contraction on the GPU: H(k) = sum_R e^{2 pi i k.R} H(R) and
  * r(k) likewise for the three position components, for every k of a grid
  * k = -0.5 + i/N (crystal units).  Replaces Wannier::energy and dipolex/y/z

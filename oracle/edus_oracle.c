@@ -1272,3 +1272,165 @@ int orc_max_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------ Coulomb coefficient generator
+ * Source_Main/Fourier_Coulomb.cpp:95-500 with Coulomb.h:60-175 (Coloumb_THh_F,
+ * Create_Wq_with_G_sum, Four_integr_Coloumb), Coulomb.h:235-267 (Fourier_series_Coloumb)
+ * and integrationWeight.h:5-18 (Grid_and_Simpson), for the "011" orientation:
+ * crystal (0, q1, q2) with cartesian b1 q1 + b2 q2.  Loops in the reference's order. */
+typedef struct orc_cgen {
+    int32_t nb, ncut, n_sum, n_peak, rytova, n_g;
+    const int32_t *g;      /* [n_g][2] */
+    double b1[3], b2[3];
+    double eps_area, qtf, r0, q_peak;
+    const double *disp;    /* [nb][3] */
+} orc_cgen;
+
+/* Coloumb_THh_F (Coulomb.h:60-84) */
+static double cgen_vq(const orc_cgen *c, double q2, double qtf) {
+    double num = 2 * M_PI, den;
+    if (c->rytova) den = c->eps_area * sqrt(q2 + qtf * qtf) * (sqrt(q2) * c->r0 + 1.0);
+    else den = c->eps_area * sqrt(q2 + qtf * qtf);
+    return num / den;
+}
+
+static void cgen_cart(const orc_cgen *c, double x1, double x2, double cart[3]) {
+    for (int i = 0; i < 3; i++) cart[i] = c->b1[i] * x1 + c->b2[i] * x2;
+}
+
+/* Create_Wq_with_G_sum (Coulomb.h:87-175): W[ic*nb+jc] for ic <= jc. */
+static void cgen_wq(const orc_cgen *c, double qx, double qy, double qtf, int hartree, cplx *W) {
+    const int nb = c->nb;
+    for (int i = 0; i < nb * nb; i++) W[i] = 0;
+    for (int ig = 0; ig < c->n_g; ig++) {
+        const double gx = c->g[2 * ig], gy = c->g[2 * ig + 1];
+        double cart[3];
+        cgen_cart(c, qx + gx, qy + gy, cart);
+        const double q2 = cart[0] * cart[0] + cart[1] * cart[1] + cart[2] * cart[2];
+        const double V = (hartree && (gx + gy) < 1e-16) ? 0.0 : cgen_vq(c, q2, qtf);
+        W[0] += V;
+        for (int ic = 0; ic < nb; ic++)
+            for (int jc = ic + 1; jc < nb; jc++) {
+                double phi = 0;
+                for (int x = 0; x < 3; x++) phi += cart[x] * (c->disp[ic * 3 + x] - c->disp[jc * 3 + x]);
+                W[ic * nb + jc] += V * cexp(I * phi);
+            }
+    }
+    for (int ic = 1; ic < nb; ic++) W[ic * nb + ic] = W[0];
+}
+
+/* Grid_and_Simpson (integrationWeight.h:5-18) on (-0.5, 0.5). */
+static void cgen_simpson(int n, double *q, double *w) {
+    const double h = 2 * 0.5 / (n - 1), hs = 3 * (n - 1);
+    for (int i = 0; i < n; i++) q[i] = -0.5 + i * h;
+    w[0] = 1 / hs;
+    for (int i = 1; i < n; i += 2) w[i] = 4.0 / hs;
+    for (int i = 2; i < n; i += 2) w[i] = 2.0 / hs;
+    w[n - 1] = 1 / hs;
+}
+
+/* Fourier_series_Coloumb (Coulomb.h:235-267) for entry idx. */
+static cplx cgen_series(const orc_cgen *c, const cplx *A, const cplx *B, const cplx *C, const cplx *D, double kx, double ky, int idx) {
+    const int nc = c->ncut, nb2 = c->nb * c->nb;
+    cplx vf = 0;
+    for (int m = 0; m < nc; m++)
+        for (int n = 0; n < nc; n++) {
+            const double cm = cos(2 * M_PI * m * kx), sm = sin(2 * M_PI * m * kx);
+            const double cn = cos(2 * M_PI * n * ky), sn = sin(2 * M_PI * n * ky);
+            const int r = n + nc * m;
+            vf += A[r * nb2 + idx] * cm * cn + B[r * nb2 + idx] * cm * sn + C[r * nb2 + idx] * sm * cn + D[r * nb2 + idx] * sm * sn;
+        }
+    return vf;
+}
+
+/* Four_integr_Coloumb (Coulomb.h:182-232) for one q point and the accumulation of
+ * Fourier_Coulomb.cpp:248-257 (A, D real parts; B, C imaginary parts). */
+static void cgen_accumulate(const orc_cgen *c, const cplx *W, double spxy, double qx, double qy, cplx *A, cplx *B, cplx *C, cplx *D) {
+    const int nb = c->nb, nb2 = nb * nb, nc = c->ncut, nmm = nc * nc;
+    for (int mm = 0; mm < nmm; mm++) {
+        const int m = mm / nc, n = mm % nc;
+        const double mq = 2 * M_PI * m * qx, nq = 2 * M_PI * n * qy;
+        const double cm = cos(mq), sm = sin(mq), cn = cos(nq), sn = sin(nq);
+        const double dA = 4 * spxy * cm * cn, dD = 4 * spxy * sm * sn, dB = 4 * spxy * cm * sn, dC = 4 * spxy * sm * cn;
+        cplx AD[4][ORC_MAXNB * ORC_MAXNB];
+        memset(AD, 0, sizeof AD);
+        for (int ic = 0; ic < nb; ic++)
+            for (int jc = ic; jc < nb; jc++) {
+                const int idx = ic * nb + jc;
+                const cplx Vq = W[idx];
+                if (ic == 0 && jc == 0) { AD[0][0] += Vq * dA; AD[3][0] += Vq * dD; }
+                if (jc != ic) { AD[0][idx] += dA * Vq; AD[1][idx] += dB * Vq; AD[2][idx] += dC * Vq; AD[3][idx] += dD * Vq; }
+            }
+        const double sc = (m == 0 && n == 0) ? 0.25 : ((m == 0 || n == 0) ? 0.5 : 1.0);
+        for (int i = 0; i < nb2; i++) {
+            A[mm * nb2 + i] += creal(AD[0][i] * sc);
+            B[mm * nb2 + i] += I * cimag(AD[1][i] * sc);
+            C[mm * nb2 + i] += I * cimag(AD[2][i] * sc);
+            D[mm * nb2 + i] += creal(AD[3][i] * sc);
+        }
+    }
+}
+
+/* Test hook: the Simpson/Fourier integration alone over a given W grid
+ * [n_sum^2][nb^2] complex (q_x index outer). */
+void orc_cgen_integrate(const orc_cgen *c, const double *Wgrid, double *Ao, double *Bo, double *Co, double *Do) {
+    const int nb2 = c->nb * c->nb, nmm = c->ncut * c->ncut, N = c->n_sum;
+    cplx *A = (cplx *)Ao, *B = (cplx *)Bo, *C = (cplx *)Co, *D = (cplx *)Do;
+    double *q = malloc(sizeof(double) * N), *w = malloc(sizeof(double) * N);
+    cgen_simpson(N, q, w);
+    for (int i = 0; i < nmm * nb2; i++) A[i] = B[i] = C[i] = D[i] = 0;
+    for (long iq = 0; iq < (long)N * N; iq++)
+        cgen_accumulate(c, (const cplx *)Wgrid + iq * nb2, w[iq / N] * w[iq % N], q[iq / N], q[iq % N], A, B, C, D);
+    free(q); free(w);
+}
+
+/* A..D [ncut^2][nb^2], screen [nb^2] (Screen_const[0]), vh [nb][nb], all complex. */
+void orc_coulomb_coefficients(const orc_cgen *c, double *Ao, double *Bo, double *Co, double *Do, double *screen, double *vh) {
+    const int nb = c->nb, nb2 = nb * nb, nc = c->ncut, nmm = nc * nc;
+    cplx *A = (cplx *)Ao, *B = (cplx *)Bo, *C = (cplx *)Co, *D = (cplx *)Do, *S = (cplx *)screen, *VH = (cplx *)vh;
+    cplx W[ORC_MAXNB * ORC_MAXNB];
+    /* Hartree term at q = 0 (Fourier_Coulomb.cpp:173-193) */
+    cgen_wq(c, 0.0, 0.0, c->qtf, 1, W);
+    for (int i = 0; i < nb2; i++) VH[i] = W[i];
+    for (int ic = 0; ic < nb; ic++) {
+        if (ic > 0) VH[ic * nb + ic] = VH[0];
+        for (int jc = ic; jc < nb; jc++) {
+            VH[jc * nb + ic] = creal(VH[ic * nb + jc]);
+            VH[ic * nb + jc] = VH[jc * nb + ic];
+        }
+    }
+    /* Simpson integral of W(q) cos/sin (Fourier_Coulomb.cpp:230-265, Coulomb.h:182-232) */
+    const int N = c->n_sum;
+    double *q = malloc(sizeof(double) * N), *w = malloc(sizeof(double) * N);
+    cgen_simpson(N, q, w);
+    for (int i = 0; i < nmm * nb2; i++) A[i] = B[i] = C[i] = D[i] = 0;
+    for (long iq = 0; iq < (long)N * N; iq++) {
+        const int ix = (int)(iq / N), iy = (int)(iq % N);
+        cgen_wq(c, q[ix], q[iy], c->qtf, 0, W);
+        cgen_accumulate(c, W, w[ix] * w[iy], q[ix], q[iy], A, B, C, D);
+    }
+    free(q); free(w);
+    /* correction around q = 0 (Fourier_Coulomb.cpp:375-456): unscreened W minus the
+     * series, on an n_peak^2 grid with weight 1/n_peak^2 */
+    const int Np = c->n_peak;
+    q = malloc(sizeof(double) * Np); w = malloc(sizeof(double) * Np);
+    cgen_simpson(Np, q, w);
+    const double dk2 = 1. / ((double)Np * Np);
+    for (int i = 0; i < nb2; i++) S[i] = 0;
+    for (long iq = 0; iq < (long)Np * Np; iq++) {
+        const double qx = q[iq / Np], qy = q[iq % Np];
+        double cart[3];
+        cgen_cart(c, qx, qy, cart);
+        const double q2 = cart[0] * cart[0] + cart[1] * cart[1] + cart[2] * cart[2];
+        if (q2 < c->q_peak * c->q_peak && q2 > pow(10, -17)) {
+            cgen_wq(c, qx, qy, 0.0, 0, W);
+            for (int ic = 0; ic < nb; ic++)
+                for (int jc = ic; jc < nb; jc++) {
+                    const int idx = ic * nb + jc;
+                    const cplx vf = cgen_series(c, A, B, C, D, qx, qy, idx);
+                    S[idx] += (W[idx] - vf) * dk2;
+                }
+        }
+    }
+    free(q); free(w);
+}

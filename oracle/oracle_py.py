@@ -57,6 +57,52 @@ class ORun(C.Structure):
                 ("probe", OPulse), ("time_au_fs", C.c_double)]
 
 
+class OCGen(C.Structure):
+    """orc_cgen: parameters of the Coulomb coefficient generator."""
+    _fields_ = [("nb", C.c_int32), ("ncut", C.c_int32), ("n_sum", C.c_int32), ("n_peak", C.c_int32),
+                ("rytova", C.c_int32), ("n_g", C.c_int32), ("g", C.POINTER(C.c_int32)),
+                ("b1", C.c_double * 3), ("b2", C.c_double * 3), ("eps_area", C.c_double), ("qtf", C.c_double),
+                ("r0", C.c_double), ("q_peak", C.c_double), ("disp", C.POINTER(C.c_double))]
+
+
+def _cgen_struct(p, keep):
+    g = np.ascontiguousarray(p["g"], dtype=np.int32).reshape(-1, 2)
+    d = np.ascontiguousarray(p["disp"], dtype=np.float64).reshape(-1, 3)
+    keep += [g, d]
+    o = OCGen()
+    o.nb, o.ncut, o.n_sum, o.n_peak, o.rytova, o.n_g = p["nb"], p["ncut"], p["n_sum"], p["n_peak"], p["rytova"], len(g)
+    o.g = g.ctypes.data_as(C.POINTER(C.c_int32))
+    for i in range(3):
+        o.b1[i], o.b2[i] = p["b1"][i], p["b2"][i]
+    o.eps_area, o.qtf, o.r0, o.q_peak = p["eps_area"], p["qtf"], p["r0"], p["q_peak"]
+    o.disp = d.ctypes.data_as(C.POINTER(C.c_double))
+    return o
+
+
+def coulomb_coefficients(p):
+    """Reference-order restatement of the coefficient generator; p: dict with nb, ncut,
+    n_sum, n_peak, rytova, g [n_g][2], b1, b2, eps_area, qtf, r0, q_peak, disp [nb][3]."""
+    keep = []
+    o = _cgen_struct(p, keep)
+    nb2, nmm = p["nb"] ** 2, p["ncut"] ** 2
+    out = [np.zeros((nmm, nb2), dtype=np.complex128) for _ in range(4)]
+    sc = np.zeros(nb2, dtype=np.complex128)
+    vh = np.zeros((p["nb"], p["nb"]), dtype=np.complex128)
+    lib.orc_coulomb_coefficients(C.byref(o), *[_d(x.view(np.float64)) for x in out], _d(sc.view(np.float64)),
+                                 _d(vh.view(np.float64)))
+    return {"A": out[0], "B": out[1], "C": out[2], "D": out[3], "screen": sc, "vh": vh}
+
+
+def cgen_integrate(p, Wgrid):
+    keep = []
+    o = _cgen_struct(p, keep)
+    nb2, nmm = p["nb"] ** 2, p["ncut"] ** 2
+    Wg = np.ascontiguousarray(Wgrid, dtype=np.complex128)
+    out = [np.zeros((nmm, nb2), dtype=np.complex128) for _ in range(4)]
+    lib.orc_cgen_integrate(C.byref(o), _d(Wg.view(np.float64)), *[_d(x.view(np.float64)) for x in out])
+    return out
+
+
 def _load():
     if not os.path.exists(LIB):
         build()
@@ -78,6 +124,8 @@ def _load():
     lib.orc_coulomb_x.argtypes = [C.POINTER(OTables), _dp, _dp, _dp]
     lib.orc_rhs_coulomb.argtypes = [C.POINTER(OTables), _dp, _dp, _dp, _dp, C.c_int, C.c_double]
     lib.orc_taylor_update_hist.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_long, _dp]
+    lib.orc_coulomb_coefficients.argtypes = [C.POINTER(OCGen), _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_cgen_integrate.argtypes = [C.POINTER(OCGen), _dp, _dp, _dp, _dp, _dp]
     lib.orc_set_threads.argtypes = [C.c_int]
     lib.orc_max_threads.restype = C.c_int
     return lib

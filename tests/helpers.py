@@ -62,6 +62,28 @@ def coulomb_input(d, nk=24, stepper="", dt=0.1, nsteps=400, res=2, pump_int=5e12
     return p
 
 
+def coulomb_gen_params(setup, g_distance=400.0, n_sum=4999, n_peak=39999, ncut=20, qtf=0.01, eps=1.0, r0=18.89726,
+                       rytova=1, g_scan=4999):
+    """Generator parameters of Fourier_Coulomb.cpp:1-120 for a setup ("011"): the G
+    shell |G| < g_distance sqrt(det_b) scanned first offset outer, orbital positions
+    from the real dipole diagonal at k = (0, 0.3, 0.1), eps * in-plane cell area."""
+    info = setup.info
+    b = setup.array("b").reshape(3, 3)
+    gd = g_distance * np.sqrt(info["det_b"])
+    gl = []
+    iy = np.arange(-g_scan, g_scan + 1)
+    for ix in range(-g_scan, g_scan + 1):
+        cart = ix * b[1][None, :] + iy[:, None] * b[2][None, :]
+        m = np.sqrt(np.sum(cart * cart, axis=1)) < gd
+        if m.any():
+            gl.append(np.stack([np.full(m.sum(), ix), iy[m]], axis=1))
+    g = np.concatenate(gl).astype(np.int32)
+    _, _, D = setup.eval_tb(np.array([[0.0, 0.3, 0.1]]))
+    disp = np.real(np.einsum("iix->ix", D[0]))
+    return {"nb": info["nb"], "ncut": ncut, "n_sum": n_sum, "n_peak": n_peak, "rytova": rytova, "g": g, "b1": b[1].copy(),
+            "b2": b[2].copy(), "eps_area": eps * info["det_a"], "qtf": qtf, "r0": r0, "q_peak": max(qtf, 0.02), "disp": disp}
+
+
 def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, coulomb=False):
     """Oracle tables for a setup: k-grid, shells, neighbours and model from the oracle's
     own restatement (tight binding), the setup's parameters and pulses."""

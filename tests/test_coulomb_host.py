@@ -1,6 +1,8 @@
 """Coulomb mean-field input path on the CPU: the host reads the reference's
 coefficient files (Fourier_Coulomb.cpp:505-622) exactly as numpy parses them,
-and rejects the Coulomb variants this build does not carry."""
+and rejects the Coulomb variants this build does not carry.  (Without
+Read_Coulomb_from_files the host computes the coefficients on the GPU; that path
+is tested in test_gpu_coulomb_gen.py.)"""
 import os
 
 import numpy as np
@@ -23,14 +25,6 @@ def test_coefficient_files_read_like_numpy(tmp_path):
     # the files' own structure: V_Hartree symmetric, A diagonal entries real
     assert np.allclose(ref["vh"], ref["vh"].T)
     assert np.max(np.abs(ref["A"][:, [0, 4, 8]].imag)) == 0.0
-
-
-def test_coulomb_requires_files(tmp_path):
-    p = Hh.coulomb_input(str(tmp_path), nk=12)
-    text = open(p).read().replace("Read_Coulomb_from_files\n", "")
-    open(p, "w").write(text)
-    with pytest.raises(RuntimeError, match="read from files"):
-        N.Setup(p)
 
 
 def test_coulomb_diag_basis_rejected(tmp_path):
