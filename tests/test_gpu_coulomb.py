@@ -18,6 +18,9 @@ def test_rhs_with_mean_field_matches_oracle(tmp_path):
     eng = N.Engine(s.problem())
     P0 = s.array("rho0", True).reshape(-1, 3, 3)
     P = random_hermitian_state(P0, 0.02, 5)
+    # plus 1% population in the conduction band, so that the peak-correction gate is open
+    Uc = s.array("U", True).reshape(-1, 3, 3)[:, 2, :]
+    P = P + 0.01 * np.einsum("ki,kj->kij", Uc, np.conj(Uc))
     eng.set_state(P)
     # the engine's peak-correction gate is the largest printed population of P
     ob = eng.observables()
