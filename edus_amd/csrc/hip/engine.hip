@@ -10,7 +10,8 @@
 //   * nb <= 4: "SoA" -- element e of k at e*nkp + k (one thread per k, coalesced).
 //     nb >= 5: "AoS" -- k-major, nb*nb contiguous doubles (one wavefront per k).
 //   * H is packed like the state (upper triangle, as the RHS reads it);
-//     D components are packed symmetric (D_ji = D_ij); U is full complex.
+//     D components are packed by their upper triangle, the lower one being D_ij
+//     (symmetric tables) or conj(D_ij) (Hermitian tables); U is full complex.
 //
 // Kernels: fused RHS + stage update per RK4 stage / Adams-Bashforth step /
 // Taylor step (RungeKutta.h:7-355, RungeKutta.cpp, Linear_Multistep_ODE_solvers.cpp,
@@ -54,11 +55,11 @@ constexpr int kMaxTinv = (EDUS_MAX_ORDER - 1) * (EDUS_MAX_ORDER - 1);
 // Per-context constants of the right-hand side.
 struct ConstArgs {
     const double *H;     // packed (layout of the state)
-    const double *D;     // 3 packed symmetric components
+    const double *D;     // 3 packed components (upper triangle; lower = D_ij or conj(D_ij), see dherm)
     const double *U;     // full complex AoS [k][nb][nb] (dissipation)
     const double *T;     // [nb][nb]
     const int *nbr;      // [n_nbr][nkp]
-    int nk, nkp, nb, nch, ncc, n_nbr, diss, pad;
+    int nk, nkp, nb, nch, ncc, n_nbr, diss, dherm;  // dherm: D_ji = conj(D_ij) (else D_ji = D_ij)
     double wx2;
     // Coulomb mean field (small-nb kernel only): per-k reconstruction from the
     // per-column partial sums Tt, peak correction against PW0, renormalised H out.
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
 #pragma unroll
             for (int x = 0; x < 3; x++) {
                 mij = cadd(mij, cscale(dx[x], E[x]));
-                mji = cadd(mji, cscale(dx[x], E[x]));
+                mji = cadd(mji, cscale(c.dherm ? cconj(dx[x]) : dx[x], E[x]));
             }
             M[i][j] = mij;
             M[j][i] = mji;
@@ -484,7 +485,7 @@ __global__ void k_coul_s3(CoulArgs a) {
 // H_ren D^+ for Current2.  One thread per k, nb <= 4.
 template <int NB>
 __global__ __launch_bounds__(256) void k_coul_obs(const double *Hr, const double *Dp, const int *nbr, const double *wb, int n_nbr,
-                                                   int nk, int nkp, double *Uexc, double *GE, double *Q, double *Ek) {
+                                                   int nk, int nkp, int dherm, double *Uexc, double *GE, double *Q, double *Ek) {
     constexpr int NP = NB * NB;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
@@ -579,6 +580,7 @@ __global__ __launch_bounds__(256) void k_coul_obs(const double *Hr, const double
                     else {
                         const int p = pair_index(min(j, kc), max(j, kc), NB);
                         d = mk(Dp[(x * NP + NB + 2 * p) * (size_t)nkp + k], Dp[(x * NP + NB + 2 * p + 1) * (size_t)nkp + k]);
+                        if (dherm && j > kc) d = cconj(d);
                     }
                     acc = cadd(acc, cmul(h, cconj(d)));
                 }
@@ -708,7 +710,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
             Ad[2 * (i * LD + j) + 1] = y[m];
             Ad[2 * (j * LD + i) + 1] = -y[m];
             Bd[2 * (i * LD + j) + 1] = dsum;
-            Bd[2 * (j * LD + i) + 1] = -hv + dd;
+            Bd[2 * (j * LD + i) + 1] = c.dherm ? -dsum : -hv + dd;
         }
     }
     __syncthreads();
@@ -1247,7 +1249,7 @@ struct edus_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0;
+    int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
     bool soa = true;
     double dt = 0, wx2 = 0, j_norm = 0, tabs_norm = 0, nktot = 0;
     std::vector<double> tinv, coef;
@@ -1363,7 +1365,7 @@ int launch_stage(edus_ctx *c, const StageArgs &a) {
     ConstArgs ca;
     ca.H = c->H; ca.D = c->D; ca.U = c->U; ca.T = c->T; ca.nbr = c->nbr;
     ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
-    ca.diss = c->diss; ca.pad = 0; ca.wx2 = c->wx2;
+    ca.diss = c->diss; ca.dherm = c->dherm; ca.wx2 = c->wx2;
     ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
     ca.cxm = c->cxm; ca.sxm = c->sxm; ca.Tt = c->cT; ca.PW0 = c->cPW0; ca.EH = c->cEH; ca.screen = c->cscreen; ca.Hr = c->cHr;
     if (c->coul) {
@@ -1449,7 +1451,9 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
     // Neighbour indices must address the grid: the gradient gather trusts them.
     for (size_t i = 0; i < (size_t)p->nk * p->n_nbr; i++)
         if (p->nbr[i] < 0 || p->nbr[i] >= p->nk) return set_err(EDUS_ERR_INVALID, "neighbour index outside the grid");
-    // The packed operator layout needs D symmetric with a real diagonal.
+    // The packed operator layout needs D symmetric (tight binding, the Wannier
+    // (D + D^T)/2) or Hermitian (the crystal interface), with a real diagonal.
+    bool dsym = true, dhermit = true;
     for (int k = 0; k < p->nk; k++)
         for (int x = 0; x < 3; x++)
             for (int i = 0; i < nb; i++) {
@@ -1458,13 +1462,16 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
                 for (int j = i + 1; j < nb; j++) {
                     const double *a = p->D + (((size_t)k * nb + i) * nb + j) * 6 + 2 * x;
                     const double *b = p->D + (((size_t)k * nb + j) * nb + i) * 6 + 2 * x;
-                    if (a[0] != b[0] || a[1] != b[1]) return set_err(EDUS_ERR_UNSUPPORTED, "dipole table must be symmetric (D_ij == D_ji)");
+                    if (a[0] != b[0] || a[1] != b[1]) dsym = false;
+                    if (a[0] != b[0] || a[1] != -b[1]) dhermit = false;
                 }
             }
+    if (!dsym && !dhermit) return set_err(EDUS_ERR_UNSUPPORTED, "dipole table must be symmetric or Hermitian");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(EDUS_ERR_NODEVICE, "no HIP device visible");
 
     auto *c = new edus_ctx();
+    c->dherm = (!dsym && dhermit) ? 1 : 0;
     auto fail = [&](int rc) { edus_destroy(c); return rc; };
     if (hipGetDevice(&c->dev) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "stream create failed"));
@@ -1850,10 +1857,10 @@ int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *t
     if (c->coul) {
         const int blocks = (c->nk + 255) / 256;
         switch (c->nb) {
-            case 1: hipLaunchKernelGGL(k_coul_obs<1>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
-            case 2: hipLaunchKernelGGL(k_coul_obs<2>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
-            case 3: hipLaunchKernelGGL(k_coul_obs<3>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
-            default: hipLaunchKernelGGL(k_coul_obs<4>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->cUexc, c->cGE, c->cQ, c->cEk); break;
+            case 1: hipLaunchKernelGGL(k_coul_obs<1>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->dherm, c->cUexc, c->cGE, c->cQ, c->cEk); break;
+            case 2: hipLaunchKernelGGL(k_coul_obs<2>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->dherm, c->cUexc, c->cGE, c->cQ, c->cEk); break;
+            case 3: hipLaunchKernelGGL(k_coul_obs<3>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->dherm, c->cUexc, c->cGE, c->cQ, c->cEk); break;
+            default: hipLaunchKernelGGL(k_coul_obs<4>, dim3(blocks), dim3(256), 0, c->stream, (const double *)c->cHr, (const double *)c->D, (const int *)c->nbr, (const double *)c->dwb, c->n_nbr, c->nk, c->nkp, c->dherm, c->cUexc, c->cGE, c->cQ, c->cEk); break;
         }
         HIP_TRY(hipGetLastError());
         c->cEk_valid = true;

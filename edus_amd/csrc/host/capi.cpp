@@ -99,7 +99,7 @@ int edus_setup_get_info(const edus_setup *h, edus_setup_info *info) {
     info->stepper = s.stepper; info->order = s.order;
     info->it_resolution = s.in.it_resolution; info->i_current = s.in.iCurrent; info->i_tabs = s.in.iTAbs;
     info->n_pumps = (int32_t)s.in.pumps.size();
-    info->model = (s.in.iMode == "W") ? 1 : 0;
+    info->model = (s.in.iMode == "W") ? 1 : (s.in.iMode == "CY") ? 2 : 0;
     info->n_diss = s.n_diss;
     info->itfi = s.itfi;
     info->dt = s.in.dt; info->t_fin = s.in.t_fin; info->t_start = s.t_start;

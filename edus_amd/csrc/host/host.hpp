@@ -151,9 +151,22 @@ struct WannierModel {
     void read_tb(const std::string &path);
 };
 
+// Crystal interface (Crystal_interface.h): tabulated H and Berry connections,
+// interpolated by periodic bicubic convolution (Interpolation.h, 2D case).
+struct CrystalModel {
+    int norb = 0, nkf = 0;
+    std::vector<cplx> h;   // [norb][norb][nkf][nkf], entries jc <= ic
+    std::vector<cplx> bc;  // [norb][norb][2][nkf][nkf], entries jc <= ic
+    void read(const std::string &dir, double a[3][3], double b[3][3]);
+    cplx interp(const cplx *grid, const double k[3]) const;
+    cplx energy(int ic, int jc, const double k[3]) const;
+    cplx dipole(int ic, int jc, int x, const double k[3]) const;
+};
+
 // ---------------------------------------------------------------- setup
 struct Setup {
     Input in;
+    CrystalModel cy;   // iMode "CY"
     Lattice L;
     int nk = 0, nb = 0, nch = 0, ncc = 0;
     std::vector<double> kpt, kcart;

@@ -355,7 +355,7 @@ Input parse_input(const std::string &path) {
                 if (!t.empty()) {
                     const std::string &k = t[0];
                     if (ieq(k, "Wannier")) in.iMode = "W";
-                    else if (ieq(k, "Crystal")) throw Error("Crystal (CY) model is not supported by this build");
+                    else if (ieq(k, "Crystal")) in.iMode = "CY";
                     else if (ieq(k, "tightbinding")) {
                         if (t.size() < 2) fail("Please specify the TightBinding type.%s%s");
                         in.TBtype = t[1];

@@ -9,7 +9,9 @@
 // seeded synthetic generator writing the same format.
 #include "host.hpp"
 
+#include <cctype>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -288,6 +290,158 @@ void write_synthetic_tb(const std::string &path, int nw, int rmax, double a_ang,
                 }
         }
     std::fclose(fp);
+}
+
+}  // namespace edus
+
+namespace edus {
+
+// ---------------------------------------------------------------- crystal interface
+// Crystal_interface.h:19-304 with Interpolation.h (FittedData, 2D case): tabulated
+// H(k) and Berry connections on an nkf x nkf grid of crystal (k1, k2) in
+// [-0.5, 0.5] (spacing 1/(nkf-1)), interpolated by periodic bicubic convolution
+// (Catmull-Rom weights in each direction) at any k.
+namespace {
+std::vector<std::string> split_ws(const std::string &s) {
+    std::vector<std::string> out;
+    std::string cur;
+    for (char ch : s) {
+        if (ch != ' ' && ch != '\t' && ch != '\r') cur += ch;
+        else if (!cur.empty()) { out.push_back(cur); cur.clear(); }
+    }
+    if (!cur.empty()) out.push_back(cur);
+    return out;
+}
+bool last_is(const std::string &line, const char *word) {
+    auto t = split_ws(line);
+    if (t.empty()) return false;
+    std::string a = t.back(), b = word;
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); i++)
+        if (std::tolower((unsigned char)a[i]) != std::tolower((unsigned char)b[i])) return false;
+    return true;
+}
+}  // namespace
+
+void CrystalModel::read(const std::string &dir, double a[3][3], double b[3][3]) {
+    auto open = [&](const std::string &name, std::ifstream &f) {
+        f.open(dir + "/" + name);
+        if (!f) throw Error("cannot open " + dir + "/" + name + " (Crystal interface)");
+    };
+    std::ifstream f;
+    open("kmesh_data.dat", f);
+    std::string line;
+    auto next = [&](std::ifstream &s) {
+        if (!std::getline(s, line)) throw Error("unexpected end of a Crystal interface file");
+        return split_ws(line);
+    };
+    auto two = [&](double *row) {
+        auto t = next(f);
+        if (t.size() < 2) throw Error("kmesh_data.dat: expected two lattice components");
+        row[1] = std::atof(t[0].c_str());
+        row[2] = std::atof(t[1].c_str());
+    };
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[i][j] = b[i][j] = 0.0;
+    next(f); two(a[1]);
+    next(f); two(a[2]);
+    a[0][0] = 15.;
+    next(f); two(b[1]);
+    next(f); two(b[2]);
+    b[0][0] = 1. / 15.;
+    do next(f); while (!last_is(line, "basis"));
+    norb = std::atoi(next(f).at(0).c_str());
+    do next(f); while (!last_is(line, "direction"));
+    nkf = std::atoi(next(f).at(0).c_str());
+    if (norb < 1 || nkf < 2) throw Error("kmesh_data.dat: bad orbital or k-point count");
+    f.close();
+
+    const size_t g = (size_t)nkf * nkf;
+    h.assign((size_t)norb * norb * g, 0.0);
+    bc.assign((size_t)norb * norb * 2 * g, 0.0);
+    std::ifstream fh;
+    open("hk_orthonormal.dat", fh);
+    for (int i1 = 0; i1 < nkf; i1++)
+        for (int i2 = 0; i2 < nkf; i2++) {
+            const size_t kk = (size_t)i1 * nkf + i2;
+            for (int part = 0; part < 2; part++) {
+                for (int ic = 0; ic < norb; ic++) {
+                    auto t = next(fh);
+                    for (int jc = 0; jc <= ic; jc++) {
+                        const double v = std::atof(t.at(jc).c_str());
+                        cplx &e = h[((size_t)ic * norb + jc) * g + kk];
+                        e += part ? cplx(0.0, v) : cplx(v, 0.0);
+                    }
+                }
+                next(fh);
+                if (part) next(fh);
+            }
+        }
+    std::ifstream fr, fi;
+    open("bc_betabetap_real.dat", fr);
+    open("bc_betabetap_imag.dat", fi);
+    for (int i1 = 0; i1 < nkf; i1++)
+        for (int i2 = 0; i2 < nkf; i2++) {
+            const size_t kk = (size_t)i1 * nkf + i2;
+            for (int x = 0; x < 2; x++) {
+                for (int ic = 0; ic < norb; ic++) {
+                    auto tr = next(fr);
+                    auto ti = next(fi);
+                    for (int jc = 0; jc <= ic; jc++)
+                        bc[(((size_t)ic * norb + jc) * 2 + x) * g + kk] = cplx(std::atof(tr.at(jc).c_str()), std::atof(ti.at(jc).c_str()));
+                }
+                next(fr); next(fi);
+                if (x) { next(fr); next(fi); }
+            }
+        }
+}
+
+// Periodic bicubic convolution of a [nkf][nkf] table at crystal (k1, k2).
+cplx CrystalModel::interp(const cplx *grid, const double k[3]) const {
+    const double spacing = 1. / (nkf - 1), xmin = -0.5;
+    double dy = std::fmod((k[1] - xmin) / spacing, nkf), dz = std::fmod((k[2] - xmin) / spacing, nkf);
+    if (dy < 0) dy += nkf;
+    if (dz < 0) dz += nkf;
+    const int yi = (int)std::floor(dy), zi = (int)std::floor(dz);
+    dy -= yi;
+    dz -= zi;
+    auto weights = [](double t, double w[4]) {
+        const double t2 = t * t, t3 = t2 * t;
+        w[0] = 0.5 * (-t3 + 2 * t2 - t);
+        w[1] = 0.5 * (3 * t3 - 5 * t2 + 2);
+        w[2] = 0.5 * (-3 * t3 + 4 * t2 + t);
+        w[3] = 0.5 * (t3 - t2);
+    };
+    double wy[4], wz[4];
+    weights(dy, wy);
+    weights(dz, wz);
+    auto wrap = [&](int i) { return ((i % nkf) + nkf) % nkf; };
+    cplx v = 0.0;
+    for (int r = 0; r < 4; r++) {
+        cplx row = 0.0;
+        const int iy = wrap(yi - 1 + r);
+        for (int q = 0; q < 4; q++) row += wz[q] * grid[(size_t)iy * nkf + wrap(zi - 1 + q)];
+        v += wy[r] * row;
+    }
+    return v;
+}
+
+cplx CrystalModel::energy(int ic, int jc, const double k[3]) const {
+    const size_t g = (size_t)nkf * nkf;
+    if (jc <= ic) return interp(&h[((size_t)ic * norb + jc) * g], k);
+    return std::conj(interp(&h[((size_t)jc * norb + ic) * g], k));
+}
+
+// Component x of the reference's dipole (Crystal::dipole): 0 <- the third Berry
+// component (not in the files: zero), 1 <- the first, 2 <- the second; real diagonal.
+cplx CrystalModel::dipole(int ic, int jc, int x, const double k[3]) const {
+    if (x == 0) return 0.0;
+    const size_t g = (size_t)nkf * nkf;
+    const int c = x - 1;
+    cplx v = (jc <= ic) ? interp(&bc[(((size_t)ic * norb + jc) * 2 + c) * g], k)
+                        : std::conj(interp(&bc[(((size_t)jc * norb + ic) * 2 + c) * g], k));
+    if (ic == jc) v.imag(0.0);
+    return v;
 }
 
 }  // namespace edus

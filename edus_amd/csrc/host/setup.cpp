@@ -26,7 +26,10 @@ void Setup::build(const std::string &input_path, int flags) {
     in = parse_input(input_path);
     if (in.coulomb && in.coulomb_diag_basis)
         throw Error("Coulomb_diag_basis is not supported by this build (Wannier basis only)");
-    if (in.iMode != "TB" && in.iMode != "W") throw Error("iMode = " + in.iMode + " is incorrect");
+    if (in.iMode != "TB" && in.iMode != "W" && in.iMode != "CY") throw Error("iMode = " + in.iMode + " is incorrect");
+    // The crystal interface replaces the lattice with the one of its k-mesh file
+    // (input.cpp:57, Crystal_interface.h:58-96).
+    if (in.iMode == "CY") cy.read(in.base_dir, in.a, in.b);
 
     L.set_from_b(in.b);
     // BZ_parameters re-applies the polarisation of every pulse (InitParameters.h:4-11).
@@ -95,6 +98,22 @@ void Setup::build(const std::string &input_path, int flags) {
                     U[mi(nb, ik, i, j)] = tb.unitary(i, j, k);
                     for (int x = 0; x < 3; x++) D[mi(nb, ik, i, j) * 3 + x] = tb.dipole(i, j, x, k);
                 }
+        }
+    } else if (in.iMode == "CY") {
+        if (cy.norb < nb) throw Error("the crystal interface files hold fewer orbitals than bands in the input");
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int ik = 0; ik < nk; ik++) {
+            const double *k = &kpt[(size_t)ik * 3];
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) {
+                    H[mi(nb, ik, i, j)] = cy.energy(i, j, k);
+                    for (int x = 0; x < 3; x++) D[mi(nb, ik, i, j) * 3 + x] = cy.dipole(i, j, x, k);
+                }
+            std::vector<double> ev(nb);
+            std::vector<cplx> V((size_t)nb * nb);
+            eigh_jacobi(nb, &H[mi(nb, ik, 0, 0)], ev.data(), V.data());
+            for (int a = 0; a < nb; a++)
+                for (int j = 0; j < nb; j++) U[mi(nb, ik, a, j)] = std::conj(V[(size_t)j * nb + a]);
         }
     } else {
         WannierModel w;
@@ -218,6 +237,9 @@ std::vector<double> Setup::disp_at_reference_point() const {
         tb.init(in.TBtype, nch, nb, in.hopping);
         for (int i = 0; i < nb; i++)
             for (int x = 0; x < 3; x++) d[(size_t)i * 3 + x] = tb.dipole(i, i, x, k).real();
+    } else if (in.iMode == "CY") {
+        for (int i = 0; i < nb; i++)
+            for (int x = 0; x < 3; x++) d[(size_t)i * 3 + x] = cy.dipole(i, i, x, k).real();
     } else {
         WannierModel w;
         std::string seed = in.ref_coord;

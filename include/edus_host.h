@@ -33,7 +33,7 @@ typedef struct edus_setup_info {
     int32_t n_nbr, n_shells;        /* gradient neighbours, shells (CalculateWeigths.h) */
     int32_t stepper, order;         /* EDUS_STEPPER_*, AB/Taylor order                    */
     int32_t it_resolution, i_current, i_tabs, n_pumps;
-    int32_t model;                  /* 0 tight binding, 1 Wannier                         */
+    int32_t model;                  /* 0 tight binding, 1 Wannier, 2 crystal interface     */
     int32_t n_diss;                 /* nonzero entries of T (main_MPI.cpp:66-79)          */
     int64_t itfi;                   /* last step index, loop runs it = 0..itfi            */
     double dt, t_fin, t_start;      /* a.u.                                               */

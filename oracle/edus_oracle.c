@@ -1434,3 +1434,49 @@ void orc_coulomb_coefficients(const orc_cgen *c, double *Ao, double *Bo, double 
     }
     free(q); free(w);
 }
+
+/* ------------------------------------------------------------------ crystal interface interpolation
+ * FittedData (Interpolation.h:32-183 construct, 186-251 evaluation), 2D case: the
+ * table [n][n] on crystal (k1, k2) in [-0.5, 0.5] with spacing 1/(n-1), wrapped
+ * periodically by index; per cell the 16 coefficients a_ab = sum_rs C[a][r] C[b][s]
+ * p[r][s] of the Catmull-Rom basis C, evaluated as sum_ab a_ab dy^a dz^b in the
+ * reference's grouping. */
+static const double cy_basis[4][4] = {{0, 1, 0, 0}, {-0.5, 0, 0.5, 0}, {1, -2.5, 2, -0.5}, {-0.5, 1.5, -1.5, 0.5}};
+
+static cplx cy_interp(int n, const cplx *grid, double k1, double k2) {
+    const double spacing = 1. / (n - 1), xmin = -0.5;
+    double dy = fmod((k1 - xmin) / spacing, n), dz = fmod((k2 - xmin) / spacing, n);
+    if (dy < 0) dy += n;
+    if (dz < 0) dz += n;
+    const int yi = (int)floor(dy), zi = (int)floor(dz);
+    dy -= yi;
+    dz -= zi;
+    cplx p[4][4], a[16];
+    for (int r = 0; r < 4; r++)
+        for (int s = 0; s < 4; s++) {
+            int iy = yi - 1 + r, iz = zi - 1 + s;
+            iy = (iy >= 0) ? iy % n : n + iy % n;
+            iz = (iz >= 0) ? iz % n : n + iz % n;
+            p[r][s] = grid[(long)iy * n + iz];
+        }
+    for (int aa = 0; aa < 4; aa++)
+        for (int bb = 0; bb < 4; bb++) {
+            cplx v = 0;
+            for (int r = 0; r < 4; r++)
+                for (int s = 0; s < 4; s++) v += cy_basis[aa][r] * cy_basis[bb][s] * p[r][s];
+            a[4 * aa + bb] = v;
+        }
+    const double y2 = dy * dy, y3 = y2 * dy, z2 = dz * dz, z3 = z2 * dz;
+    return (a[0] + a[1] * dz + a[2] * z2 + a[3] * z3) + (a[4] + a[5] * dz + a[6] * z2 + a[7] * z3) * dy +
+           (a[8] + a[9] * dz + a[10] * z2 + a[11] * z3) * y2 + (a[12] + a[13] * dz + a[14] * z2 + a[15] * z3) * y3;
+}
+
+/* Interpolated values of tables [ntab][n][n] complex at crystal points k [nk][3]
+ * (components 1 and 2): out [nk][ntab]. */
+void orc_cy_interp(int n, int ntab, const double *tabs, long nk, const double *k, double *out) {
+    const cplx *t = (const cplx *)tabs;
+    cplx *o = (cplx *)out;
+#pragma omp parallel for schedule(static)
+    for (long ik = 0; ik < nk; ik++)
+        for (int q = 0; q < ntab; q++) o[ik * ntab + q] = cy_interp(n, t + (long)q * n * n, k[ik * 3 + 1], k[ik * 3 + 2]);
+}

@@ -126,6 +126,7 @@ def _load():
     lib.orc_taylor_update_hist.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_long, _dp]
     lib.orc_coulomb_coefficients.argtypes = [C.POINTER(OCGen), _dp, _dp, _dp, _dp, _dp, _dp]
     lib.orc_cgen_integrate.argtypes = [C.POINTER(OCGen), _dp, _dp, _dp, _dp, _dp]
+    lib.orc_cy_interp.argtypes = [C.c_int, C.c_int, _dp, C.c_long, _dp, _dp]
     lib.orc_set_threads.argtypes = [C.c_int]
     lib.orc_max_threads.restype = C.c_int
     return lib
@@ -316,6 +317,15 @@ class System:
         n = lib.orc_run_dyn(C.byref(self.t), C.byref(o), C.byref(r), float(t_fin), float(eps), _d(P.view(np.float64)),
                             _d(rows), nrows, _d(dts), C.byref(nst))
         return rows[:n], P, dts[:nst.value]
+
+
+def cy_interp(tabs, kpt):
+    """FittedData 2D interpolation of tables [ntab][n][n] at crystal points kpt [nk][3]."""
+    t = np.ascontiguousarray(tabs, dtype=np.complex128)
+    k = np.ascontiguousarray(kpt, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(k), t.shape[0]), dtype=np.complex128)
+    lib.orc_cy_interp(t.shape[1], t.shape[0], _d(t.view(np.float64)), len(k), _d(k), _d(out.view(np.float64)))
+    return out
 
 
 def set_threads(n):

@@ -84,6 +84,79 @@ def coulomb_gen_params(setup, g_distance=400.0, n_sum=4999, n_peak=39999, ncut=2
             "b2": b[2].copy(), "eps_area": eps * info["det_a"], "qtf": qtf, "r0": r0, "q_peak": max(qtf, 0.02), "disp": disp}
 
 
+def crystal_tables(nkf, norb=3, seed=5):
+    """Smooth periodic H(k) and Berry connections (x, y) on the crystal interface's
+    grid k = -0.5 + i/(nkf-1): [norb][norb][nkf][nkf] (H, lower triangle used) and
+    [norb][norb][2][nkf][nkf]."""
+    rng = np.random.default_rng(seed)
+    k = -0.5 + np.arange(nkf) / (nkf - 1)
+    K1, K2 = np.meshgrid(k, k, indexing="ij")
+    e1, e2 = np.exp(2j * np.pi * K1), np.exp(2j * np.pi * K2)
+    H = np.zeros((norb, norb, nkf, nkf), dtype=np.complex128)
+    B = np.zeros((norb, norb, 2, nkf, nkf), dtype=np.complex128)
+    for i in range(norb):
+        H[i, i] = -0.3 + 0.25 * i + 0.02 * (np.cos(2 * np.pi * K1) + np.cos(2 * np.pi * K2))
+        for x in range(2):
+            B[i, i, x] = 0.1 * (i + 1) * np.sin(2 * np.pi * (K1 if x == 0 else K2))
+        for j in range(i):
+            c = rng.normal(size=3) + 1j * rng.normal(size=3)
+            H[i, j] = 0.03 * (c[0] + c[1] * e1 + c[2] * e2)
+            for x in range(2):
+                d = rng.normal(size=2) + 1j * rng.normal(size=2)
+                B[i, j, x] = 0.2 * (d[0] * e1 + d[1] * np.conj(e2))
+    return H, B
+
+
+def write_crystal_input(d, nkf=25, nk=24, norb=3, valence=1, stepper="", nsteps=400, dt=0.1, a=4.7243):
+    """The crystal interface's four files (Crystal_interface.h:58-200 layout) for
+    crystal_tables(), and an input.txt that selects them."""
+    os.makedirs(d, exist_ok=True)
+    H, B = crystal_tables(nkf, norb)
+    c30, s30 = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    a1, a2 = [float(a * c30), float(a * s30)], [float(a * c30), float(-a * s30)]
+    det = a1[0] * a2[1] - a1[1] * a2[0]
+    b1 = [2 * np.pi * a2[1] / det, -2 * np.pi * a2[0] / det]
+    b2 = [-2 * np.pi * a1[1] / det, 2 * np.pi * a1[0] / det]
+    with open(os.path.join(d, "kmesh_data.dat"), "w") as f:
+        f.write(f"a0\n{a1[0]!r} {a1[1]!r}\na1\n{a2[0]!r} {a2[1]!r}\nb0\n{b1[0]!r} {b1[1]!r}\nb1\n{b2[0]!r} {b2[1]!r}\n")
+        f.write("atomic positions\n0.0 0.0\nnumber of orbitals in the basis\n")
+        f.write(f"{norb}\nnumber of k points in each direction\n{nkf}\nk points (crystal)\n")
+        for i in range(nkf * nkf):
+            f.write(f"{i} {-0.5 + (i // nkf) / (nkf - 1)!r} {-0.5 + (i % nkf) / (nkf - 1)!r}\n")
+    with open(os.path.join(d, "hk_orthonormal.dat"), "w") as f:
+        for i1 in range(nkf):
+            for i2 in range(nkf):
+                for part in (np.real, np.imag):
+                    for ic in range(norb):
+                        f.write(" ".join(repr(float(part(H[ic, jc, i1, i2]))) for jc in range(ic + 1)) + "\n")
+                    f.write("\n")
+                f.write("\n")
+    with open(os.path.join(d, "bc_betabetap_real.dat"), "w") as fr, open(os.path.join(d, "bc_betabetap_imag.dat"), "w") as fi:
+        for i1 in range(nkf):
+            for i2 in range(nkf):
+                for x in range(2):
+                    for ic in range(norb):
+                        fr.write(" ".join(repr(float(B[ic, jc, x, i1, i2].real)) for jc in range(ic + 1)) + "\n")
+                        fi.write(" ".join(repr(float(B[ic, jc, x, i1, i2].imag)) for jc in range(ic + 1)) + "\n")
+                    fr.write("\n")
+                    fi.write("\n")
+                fr.write("\n")
+                fi.write("\n")
+    t0 = -0.21 / O.TIME_AU_FS
+    t_fin = t0 + (nsteps - 0.5) * dt
+    text = ("NEWINPUT\ntdse{\nCrystal\n"
+            f"dt {dt} au\nt_fin {t_fin!r} au\n{stepper}num_pump_pulses 1\n}}\n"
+            f"bands{{\nvalence {valence}\nconduction {norb - valence}\n}}\n"
+            f"nkPT{{\n1 {nk} {nk}\n}}\n"
+            "laserpump sin2 {\ncycles 3\nwavelength 800 nm\nintensity 5e12 wcm2\npolarization 0 1 1\n}\n"
+            "laserprobe gaussian {\nsigma 0.3 fs\nintensity 0\nfrequency 15.06\npolarization 0 1 0\n}\n"
+            "observables{\nit_resolution 4\nCurrent\n}\n")
+    p = os.path.join(d, "input.txt")
+    with open(p, "w") as f:
+        f.write(text)
+    return p, H, B, (a1, a2, b1, b2)
+
+
 def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, coulomb=False):
     """Oracle tables for a setup: k-grid, shells, neighbours and model from the oracle's
     own restatement (tight binding), the setup's parameters and pulses."""

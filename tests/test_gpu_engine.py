@@ -5,6 +5,7 @@ Bars: right-hand side and states 1e-12 relative (FP64, different summation order
 observable time series 1e-10 relative to the series' max; J2 (a sum that cancels
 ~1e7-fold in these models) within 2e-15 of the sum of |terms|.
 """
+import ctypes as C
 import os
 
 import numpy as np
@@ -224,3 +225,32 @@ def test_ab5_run_reproduces_reference_outputs(tmp_path):
     assert [len(x) for x in j1[:5]] == [len(x) for x in ref[:5]]
     kg = open(os.path.join(out, "k_grid.txt")).read()
     assert kg == open(os.path.join(Hh.AB5, "k_grid.txt")).read()
+
+
+@pytest.mark.parametrize("model", ["tb", "wannier8"])
+def test_rhs_with_hermitian_dipoles_matches_oracle(tmp_path, model):
+    """Hermitian dipole tables (D_ji = conj(D_ij), the crystal interface's form):
+    the table from the setup plus i times a real antisymmetric matrix per k."""
+    if model == "tb":
+        s = N.Setup(tb_input(str(tmp_path), nk=16, t2=0.01))
+        osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
+    else:
+        s = _wannier_setup(str(tmp_path), 8, t2=10.0)
+        osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
+    nb = s.info["nb"]
+    D = s.array("D", True).reshape(-1, nb, nb, 3)
+    rng = np.random.default_rng(17)
+    Aa = rng.normal(size=D.shape) * 0.05
+    Aa = Aa - np.swapaxes(Aa, 1, 2)
+    Dh = np.ascontiguousarray(0.5 * (D + np.conj(np.swapaxes(D, 1, 2))) + 1j * Aa)
+    p = s.problem()
+    p.D = Dh.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    eng = N.Engine(p)
+    sysd = O.System(osys["H"], Dh, osys["U"], osys["T"], osys["nbr"], np.repeat(osys["w"], osys["sizes"]), osys["bvec"],
+                    s.info["nch"], s.info["ncc"], s.info["wx2"])
+    P = random_hermitian_state(s.array("rho0", True).reshape(-1, nb, nb), 0.05, 23)
+    eng.set_state(P)
+    for field6, on in (([1e-3, 2e-3, -3e-3, 4e-4, 1e-4, 0], True), ([2e-3, -1e-3, 5e-3, 0, 0, 0], False)):
+        pv = eng.rhs(np.array(field6), on)
+        ref = sysd.rhs(P, np.array(field6), on)
+        assert np.max(np.abs(pv - ref)) <= 1e-12 * np.max(np.abs(ref))
