@@ -646,8 +646,16 @@ __device__ __forceinline__ void unpack_index(int e, int nb, int &i, int &j, int 
 }
 
 template <int NBT>
-constexpr int waves_per_block() { return NBT >= 32 ? 2 : 4; }
+constexpr int waves_per_block() { return NBT >= 32 ? 2 : 1; }
 
+// Number of LDS matrices of one wave: P, M (the product X = P M^T overwrites M)
+// and, with dephasing, U.
+__host__ __device__ constexpr int wave_mats(int diss) { return diss ? 3 : 2; }
+
+// One k per wavefront.  Every global operand of the stage (own state, H, D, the
+// stage's accumulator and base state, the 12 stencil neighbours) is requested at
+// the start, so their latencies overlap each other and the LDS work; with one
+// wave per workgroup the barriers only order the wave's own LDS traffic.
 template <int NBT, int MODE>
 __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(ConstArgs c, StageArgs a) {
     constexpr int WPB = waves_per_block<NBT>();
@@ -657,27 +665,50 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = xcd_block() * WPB + wave;
     const bool valid = k < c.nk;
-    const int nb = c.nb, NP = nb * nb;
-    cd *A = smem + (size_t)wave * 3 * MAT, *B = A + MAT, *C = B + MAT;
+    const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
+    cd *A = smem + (size_t)wave * nmat * MAT, *B = A + MAT, *C = c.diss ? B + MAT : B;
     const double *tab = a.tab;
     const double Ep[3] = {tab[0], tab[1], tab[2]}, Eq[3] = {tab[3], tab[4], tab[5]};
     const size_t base = (size_t)k * NP;
+    const int nn = c.n_nbr;
+    const bool grad = tab[6 + nn] != 0.0;
 
-    if (nb < NBT)
-        for (int t = lane; t < 3 * MAT; t += 64) A[t] = mk(0.0, 0.0);
-    double y[PER], pv[PER];
+    double y[PER], pv[PER], hv[PER], dv[PER][3], g[PER], pre0[PER], pre1[PER];
     int ei[PER], ej[PER], ec[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const int e = lane + 64 * m;
-        y[m] = 0.0;
-        pv[m] = 0.0;
+        y[m] = pv[m] = hv[m] = g[m] = pre0[m] = pre1[m] = 0.0;
+        dv[m][0] = dv[m][1] = dv[m][2] = 0.0;
         ei[m] = ej[m] = ec[m] = 0;
         if (valid && e < NP) {
             unpack_index(e, nb, ei[m], ej[m], ec[m]);
             y[m] = a.Y[base + e];
+            hv[m] = ldnt(c.H + base + e);
+#pragma unroll
+            for (int x = 0; x < 3; x++) dv[m][x] = ldnt(c.D + (size_t)x * c.nk * NP + base + e);
+            if constexpr (MODE == RK2 || MODE == RK3) {
+                pre0[m] = ldnt(a.ACC + base + e);
+                pre1[m] = ldnt(a.P0 + base + e);
+            } else if constexpr (MODE == RK4S) {
+                pre0[m] = ldnt(a.ACC + base + e);
+            }
         }
     }
+    // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).
+    if (valid && grad) {
+        for (int n = 0; n < nn; n++) {
+            const size_t kb = (size_t)c.nbr[(size_t)n * c.nkp + k] * NP;
+            const double cn = tab[6 + n];
+#pragma unroll
+            for (int m = 0; m < PER; m++) {
+                const int e = lane + 64 * m;
+                if (e < NP) g[m] += cn * a.Y[kb + e];
+            }
+        }
+    }
+    if (nb < NBT)
+        for (int t = lane; t < nmat * MAT; t += 64) A[t] = mk(0.0, 0.0);
     __syncthreads();
     // A <- P (full Hermitian), B <- M = H + E.D
 #pragma unroll
@@ -687,13 +718,9 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         const int i = ei[m], j = ej[m], comp = ec[m];
         const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
         const double *E = w ? Eq : Ep;
-        const double hv = ldnt(c.H + base + e);
-        double dv[3];
+        double dsum = hv[m];
 #pragma unroll
-        for (int x = 0; x < 3; x++) dv[x] = ldnt(c.D + (size_t)x * c.nk * NP + base + e);
-        double dsum = hv;
-#pragma unroll
-        for (int x = 0; x < 3; x++) dsum += E[x] * dv[x];
+        for (int x = 0; x < 3; x++) dsum += E[x] * dv[m][x];
         double *Ad = reinterpret_cast<double *>(A), *Bd = reinterpret_cast<double *>(B);
         if (i == j) {
             A[i * LD + i] = mk(y[m], 0.0);
@@ -706,11 +733,11 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         } else {
             double dd = 0.0;
 #pragma unroll
-            for (int x = 0; x < 3; x++) dd += E[x] * dv[x];
+            for (int x = 0; x < 3; x++) dd += E[x] * dv[m][x];
             Ad[2 * (i * LD + j) + 1] = y[m];
             Ad[2 * (j * LD + i) + 1] = -y[m];
             Bd[2 * (i * LD + j) + 1] = dsum;
-            Bd[2 * (j * LD + i) + 1] = c.dherm ? -dsum : -hv + dd;
+            Bd[2 * (j * LD + i) + 1] = c.dherm ? -dsum : -hv[m] + dd;
         }
     }
     __syncthreads();
@@ -718,6 +745,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
     {
         cd acc[RB][RB];
         gemm_lds<NBT, 0, 0>(A, B, nb, lane, acc);
+        if (C == B) __syncthreads();  // every lane has finished reading M
         store_block<NBT>(C, lane, acc);
     }
     __syncthreads();
@@ -786,27 +814,23 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         }
     }
     if (!valid) return;
-    const int nn = c.n_nbr;
-    if (tab[6 + nn] != 0.0) {
-        double g[PER];
-#pragma unroll
-        for (int m = 0; m < PER; m++) g[m] = 0.0;
-        for (int n = 0; n < nn; n++) {
-            const size_t kb = (size_t)c.nbr[(size_t)n * c.nkp + k] * NP;
-            const double cn = tab[6 + n];
-#pragma unroll
-            for (int m = 0; m < PER; m++) {
-                const int e = lane + 64 * m;
-                if (e < NP) g[m] += cn * a.Y[kb + e];
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < PER; m++) pv[m] += g[m];
-    }
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const int e = lane + 64 * m;
-        if (e < NP) stage_update<MODE>(a, base + e, (size_t)c.nk * NP, pv[m], y[m]);
+        if (e >= NP) continue;
+        const double kv = pv[m] + g[m];
+        const size_t o = base + e;
+        if constexpr (MODE == RK1) {
+            stnt(a.ACC + o, y[m] + kv * a.ca);
+            stnt(a.OUT + o, y[m] + kv * a.cb);
+        } else if constexpr (MODE == RK2 || MODE == RK3) {
+            stnt(a.ACC + o, pre0[m] + kv * a.ca);
+            stnt(a.OUT + o, pre1[m] + kv * a.cb);
+        } else if constexpr (MODE == RK4S) {
+            stnt(a.OUT + o, pre0[m] + kv * a.ca);
+        } else {
+            stage_update<MODE>(a, o, (size_t)c.nk * NP, kv, y[m]);
+        }
     }
 }
 
@@ -1386,7 +1410,7 @@ int launch_stage(edus_ctx *c, const StageArgs &a) {
             constexpr int NBT = decltype(tag)::value;
             constexpr int WPB = waves_per_block<NBT>();
             const int blocks = (c->nk + WPB - 1) / WPB;
-            const size_t lds = (size_t)WPB * 3 * NBT * (NBT + 1) * sizeof(cd);
+            const size_t lds = (size_t)WPB * wave_mats(c->diss) * NBT * (NBT + 1) * sizeof(cd);
             hipLaunchKernelGGL((k_stage_wave<NBT, MODE>), dim3(blocks), dim3(64 * WPB), lds, c->stream, ca, a);
         };
         if (c->nb <= 8) go(std::integral_constant<int, 8>{});
