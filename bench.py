@@ -30,7 +30,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--nk", type=int, default=200)
+    ap.add_argument("--workload", choices=["wannier", "hbn"], default="wannier",
+                    help="wannier: synthetic 16-band Wannier model, 200x200 (default); hbn: 3-band hBN tight binding, "
+                         "60x60, T2 = 10 fs (the CPU-runnable case)")
+    ap.add_argument("--nk", type=int, default=0, help="grid points per direction (0: the workload's own)")
     ap.add_argument("--nw", type=int, default=16)
     ap.add_argument("--rmax", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -54,8 +57,15 @@ def main():
     wd = args.workdir or tempfile.mkdtemp(prefix="edus_bench_")
     nsteps_total = args.steps + args.warmup
     t_setup = time.perf_counter()
-    inp = workloads.wannier_synthetic(wd, nw=args.nw, rmax=args.rmax, nk=args.nk, nsteps=max(4000, nsteps_total),
-                                      dt=0.05, stepper="RK4", e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0)
+    if args.workload == "wannier":
+        nkd = args.nk or 200
+        inp = workloads.wannier_synthetic(wd, nw=args.nw, rmax=args.rmax, nk=nkd, nsteps=max(4000, nsteps_total),
+                                          dt=0.05, stepper="RK4", e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0)
+        wname = f"wannier_nw{args.nw}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{nkd}x{nkd}_rk4_dt0.05"
+    else:
+        nkd = args.nk or 60
+        inp = workloads.hbn_tb(wd, nk=nkd, nsteps=max(8000, nsteps_total), dt=0.05, stepper="RK4")
+        wname = f"hbn_gen2d_3band_{nkd}x{nkd}_T2_10fs_rk4_dt0.05"
     s = N.Setup(inp)
     info = s.info
     eng = N.Engine(s.problem())
@@ -91,7 +101,8 @@ def main():
         traffic = json.load(open(tfile)).get("bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"k_stage_wave<{16 if nb <= 16 else 32}> (RK4 stages)", "bytes_per_launch": bytes_launch,
+            "kernel": (f"k_stage_wave<{8 if nb <= 8 else 16 if nb <= 16 else 32}>" if nb > 4 else f"k_stage_small<{nb}>")
+                      + " (RK4 stages)", "bytes_per_launch": bytes_launch,
             "stored_bytes_per_launch": stored_launch, "achieved_stored": round(stored, 2),
             "frac_stored": round(stored / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_launch_s * 1e6, 3)}
 
@@ -112,8 +123,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"wannier_nw{nb}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{args.nk}x{args.nk}_rk4_dt0.05",
-                   "nk": nk, "bands": nb, "n_R": (2 * args.rmax + 1) ** 2, "stepper": "RK4", "precision": "fp64",
+        "config": {"workload": wname,
+                   "nk": nk, "bands": nb, "stepper": "RK4", "precision": "fp64",
                    "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
                    "device_ms_per_step": dev_ms / args.steps},
         "roofline": roof,
