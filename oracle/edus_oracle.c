@@ -21,6 +21,25 @@
  *                        Observables_MPI.h          AB5 J1/J2/Losses/TransientAbs (AB5 + core-hole decay);
  *                                                  RK4 / Taylor / dephasing: free precession, T2 decay,
  *                                                  trace preservation, scalar-ODE order tests
+ *   Coulomb mean field   Coulomb.h:273-541, RungeKutta.h:29-138
+ *                                                  brute-force convolution with the separable kernel of
+ *                                                  the A..D coefficients (example files, Ncut 20);
+ *                                                  RHS reduces to the bare RHS at equilibrium
+ *   orc_run_dyn          dyn_dt_evolution.cpp      reduces bit for bit to orc_run when the bound is never
+ *                                                  reached; its dt sequence replayed step by step in numpy;
+ *                                                  Taylor with a dt history = the polynomial identity
+ *   orc_coulomb_coefficients  Fourier_Coulomb.cpp:95-500
+ *                                                  V_Hartree of the reference's example files (502657 G);
+ *                                                  orc_cgen_integrate: trigonometric modes with closed-form
+ *                                                  coefficients (A..D factors and m/n orientation)
+ *   orc_cy_interp        Interpolation.h (2D)      node values, exact quadratics away from the wrap
+ *                                                  (no reference data ship for the crystal interface)
+ *
+ * Parity unpinned: the eigenvector phases of Armadillo's eig_sym (Wannier and crystal U,
+ * which only matter for dephasing and for off-diagonal snapshot elements), and the
+ * reference's own trajectories for RK4, Taylor, Wannier, Coulomb and adaptive dt (its
+ * build needs Armadillo/LAPACK, which the image lacks): those are pinned through the
+ * invariants above, not against reference output.
  */
 #include <complex.h>
 #include <math.h>
