@@ -30,9 +30,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["wannier", "hbn"], default="wannier",
+    ap.add_argument("--workload", choices=["wannier", "hbn", "wannier32"], default="wannier",
                     help="wannier: synthetic 16-band Wannier model, 200x200 (default); hbn: 3-band hBN tight binding, "
-                         "60x60, T2 = 10 fs (the CPU-runnable case)")
+                         "60x60, T2 = 10 fs (the CPU-runnable case); wannier32: 32 bands, |R| <= 15, 512x512")
     ap.add_argument("--nk", type=int, default=0, help="grid points per direction (0: the workload's own)")
     ap.add_argument("--nw", type=int, default=16)
     ap.add_argument("--rmax", type=int, default=10)
@@ -62,6 +62,11 @@ def main():
         inp = workloads.wannier_synthetic(wd, nw=args.nw, rmax=args.rmax, nk=nkd, nsteps=max(4000, nsteps_total),
                                           dt=0.05, stepper="RK4", e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0)
         wname = f"wannier_nw{args.nw}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{nkd}x{nkd}_rk4_dt0.05"
+    elif args.workload == "wannier32":
+        nkd = args.nk or 512
+        inp = workloads.wannier_synthetic(wd, nw=32, rmax=15, nk=nkd, nsteps=max(5000, nsteps_total), dt=0.04,
+                                          stepper="RK4", e_v_ang=0.4, nvalence=8, seed=4004, decay_ang=3.0)
+        wname = f"wannier_nw32_R31x31_{nkd}x{nkd}_rk4_dt0.04"
     else:
         nkd = args.nk or 60
         inp = workloads.hbn_tb(wd, nk=nkd, nsteps=max(8000, nsteps_total), dt=0.05, stepper="RK4")

@@ -590,14 +590,23 @@ __global__ __launch_bounds__(256) void k_coul_obs(const double *Hr, const double
             }
 }
 
-// ================================================================ nb in 5..32: wavefront per k
-// LDS GEMM of one wavefront: lanes form an 8x8 grid, each owning an RB x RB
-// block of the NBT x NBT result.  OPA: conj(A) if 1.  OPB: 0 -> B[j][l]
-// (result A B^T), 1 -> B[l][j], 2 -> conj(B[l][j]).
+// ================================================================ nb in 5..32: one workgroup per k
+// Threads per k: one wavefront for nb <= 16, four for nb <= 32.  The LDS GEMM
+// runs on a GRID x GRID thread grid, each thread owning an RB x RB block of the
+// NBT x NBT result.  OPA: conj(A) if 1.  OPB: 0 -> B[j][l] (result A B^T),
+// 1 -> B[l][j], 2 -> conj(B[l][j]).
+template <int NBT>
+struct WaveGeom {
+    static constexpr int TPK = NBT >= 32 ? 256 : 64;
+    static constexpr int GRID = TPK == 256 ? 16 : 8;
+    static constexpr int RB = NBT / GRID;
+    static constexpr int PER = (NBT * NBT + TPK - 1) / TPK;  // packed doubles per thread
+};
+
 template <int NBT, int OPA, int OPB>
-__device__ __forceinline__ void gemm_lds(const cd *A, const cd *B, int nb, int lane, cd (&acc)[NBT / 8][NBT / 8]) {
-    constexpr int RB = NBT / 8, LD = NBT + 1;
-    const int r0 = (lane >> 3) * RB, c0 = (lane & 7) * RB;
+__device__ __forceinline__ void gemm_lds(const cd *A, const cd *B, int nb, int t, cd (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
+    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = NBT + 1;
+    const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
@@ -606,13 +615,13 @@ __device__ __forceinline__ void gemm_lds(const cd *A, const cd *B, int nb, int l
         cd av[RB], bv[RB];
 #pragma unroll
         for (int i = 0; i < RB; i++) {
-            cd t = A[(r0 + i) * LD + l];
-            av[i] = OPA ? cconj(t) : t;
+            cd v = A[(r0 + i) * LD + l];
+            av[i] = OPA ? cconj(v) : v;
         }
 #pragma unroll
         for (int j = 0; j < RB; j++) {
-            cd t = (OPB == 0) ? B[(c0 + j) * LD + l] : B[l * LD + c0 + j];
-            bv[j] = (OPB == 2) ? cconj(t) : t;
+            cd v = (OPB == 0) ? B[(c0 + j) * LD + l] : B[l * LD + c0 + j];
+            bv[j] = (OPB == 2) ? cconj(v) : v;
         }
 #pragma unroll
         for (int i = 0; i < RB; i++)
@@ -625,9 +634,9 @@ __device__ __forceinline__ void gemm_lds(const cd *A, const cd *B, int nb, int l
 }
 
 template <int NBT>
-__device__ __forceinline__ void store_block(cd *C, int lane, const cd (&acc)[NBT / 8][NBT / 8]) {
-    constexpr int RB = NBT / 8, LD = NBT + 1;
-    const int r0 = (lane >> 3) * RB, c0 = (lane & 7) * RB;
+__device__ __forceinline__ void store_block(cd *C, int t, const cd (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
+    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = NBT + 1;
+    const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
@@ -645,28 +654,24 @@ __device__ __forceinline__ void unpack_index(int e, int nb, int &i, int &j, int 
     j = r + 1 + p;
 }
 
-template <int NBT>
-constexpr int waves_per_block() { return NBT >= 32 ? 2 : 1; }
-
 // Number of LDS matrices of one wave: P, M (the product X = P M^T overwrites M)
 // and, with dephasing, U.
 __host__ __device__ constexpr int wave_mats(int diss) { return diss ? 3 : 2; }
 
-// One k per wavefront.  Every global operand of the stage (own state, H, D, the
+// One k per workgroup.  Every global operand of the stage (own state, H, D, the
 // stage's accumulator and base state, the 12 stencil neighbours) is requested at
 // the start, so their latencies overlap each other and the LDS work; with one
-// wave per workgroup the barriers only order the wave's own LDS traffic.
+// k per workgroup the barriers only order that k's own LDS traffic.
 template <int NBT, int MODE>
-__global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(ConstArgs c, StageArgs a) {
-    constexpr int WPB = waves_per_block<NBT>();
-    constexpr int LD = NBT + 1, MAT = NBT * LD, RB = NBT / 8;
-    constexpr int PER = (NBT * NBT + 63) / 64;   // packed doubles per lane (upper bound)
+__global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, StageArgs a) {
+    constexpr int TPK = WaveGeom<NBT>::TPK, RB = WaveGeom<NBT>::RB, PER = WaveGeom<NBT>::PER;
+    constexpr int LD = NBT + 1, MAT = NBT * LD;
     extern __shared__ cd smem[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = xcd_block() * WPB + wave;
+    const int lane = threadIdx.x;  // thread of the k's workgroup
+    const int k = xcd_block();
     const bool valid = k < c.nk;
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
-    cd *A = smem + (size_t)wave * nmat * MAT, *B = A + MAT, *C = c.diss ? B + MAT : B;
+    cd *A = smem, *B = A + MAT, *C = c.diss ? B + MAT : B;
     const double *tab = a.tab;
     const double Ep[3] = {tab[0], tab[1], tab[2]}, Eq[3] = {tab[3], tab[4], tab[5]};
     const size_t base = (size_t)k * NP;
@@ -677,7 +682,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
     int ei[PER], ej[PER], ec[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = lane + 64 * m;
+        const int e = lane + TPK * m;
         y[m] = pv[m] = hv[m] = g[m] = pre0[m] = pre1[m] = 0.0;
         dv[m][0] = dv[m][1] = dv[m][2] = 0.0;
         ei[m] = ej[m] = ec[m] = 0;
@@ -702,18 +707,18 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
             const double cn = tab[6 + n];
 #pragma unroll
             for (int m = 0; m < PER; m++) {
-                const int e = lane + 64 * m;
+                const int e = lane + TPK * m;
                 if (e < NP) g[m] += cn * a.Y[kb + e];
             }
         }
     }
     if (nb < NBT)
-        for (int t = lane; t < nmat * MAT; t += 64) A[t] = mk(0.0, 0.0);
+        for (int t = lane; t < nmat * MAT; t += TPK) A[t] = mk(0.0, 0.0);
     __syncthreads();
     // A <- P (full Hermitian), B <- M = H + E.D
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = lane + 64 * m;
+        const int e = lane + TPK * m;
         if (!valid || e >= NP) continue;
         const int i = ei[m], j = ej[m], comp = ec[m];
         const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
@@ -751,7 +756,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = lane + 64 * m;
+        const int e = lane + TPK * m;
         if (!valid || e >= NP) continue;
         const int i = ei[m], j = ej[m];
         if (i == j) {
@@ -770,7 +775,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         __syncthreads();
         // C <- U
         const double *u = c.U + (size_t)k * nb * nb * 2;
-        for (int t = lane; t < nb * nb; t += 64)
+        for (int t = lane; t < nb * nb; t += TPK)
             if (valid) C[(t / nb) * LD + (t % nb)] = mk(ldnt(u + 2 * t), ldnt(u + 2 * t + 1));
         __syncthreads();
         cd acc[RB][RB];
@@ -780,7 +785,8 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         __syncthreads();
         gemm_lds<NBT, 1, 1>(C, B, nb, lane, acc);     // G = conj(U) T1
         {
-            const int r0 = (lane >> 3) * RB, c0 = (lane & 7) * RB;
+            constexpr int GRID = WaveGeom<NBT>::GRID;
+            const int r0 = (lane / GRID) * RB, c0 = (lane % GRID) * RB;
 #pragma unroll
             for (int i = 0; i < RB; i++)
 #pragma unroll
@@ -803,7 +809,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < PER; m++) {
-            const int e = lane + 64 * m;
+            const int e = lane + TPK * m;
             if (!valid || e >= NP) continue;
             const int i = ei[m], j = ej[m];
             cd w = A[i * LD + j];
@@ -816,7 +822,7 @@ __global__ __launch_bounds__(64 * waves_per_block<NBT>()) void k_stage_wave(Cons
     if (!valid) return;
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = lane + 64 * m;
+        const int e = lane + TPK * m;
         if (e >= NP) continue;
         const double kv = pv[m] + g[m];
         const size_t o = base + e;
@@ -1408,10 +1414,8 @@ int launch_stage(edus_ctx *c, const StageArgs &a) {
     } else {
         auto go = [&](auto tag) {
             constexpr int NBT = decltype(tag)::value;
-            constexpr int WPB = waves_per_block<NBT>();
-            const int blocks = (c->nk + WPB - 1) / WPB;
-            const size_t lds = (size_t)WPB * wave_mats(c->diss) * NBT * (NBT + 1) * sizeof(cd);
-            hipLaunchKernelGGL((k_stage_wave<NBT, MODE>), dim3(blocks), dim3(64 * WPB), lds, c->stream, ca, a);
+            const size_t lds = (size_t)wave_mats(c->diss) * NBT * (NBT + 1) * sizeof(cd);
+            hipLaunchKernelGGL((k_stage_wave<NBT, MODE>), dim3(c->nk), dim3(WaveGeom<NBT>::TPK), lds, c->stream, ca, a);
         };
         if (c->nb <= 8) go(std::integral_constant<int, 8>{});
         else if (c->nb <= 16) go(std::integral_constant<int, 16>{});

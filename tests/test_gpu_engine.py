@@ -158,7 +158,7 @@ def test_wannier_contraction_matches_direct_sum(tmp_path, nw):
     assert np.all(np.diff(ev, axis=1) >= -1e-12)
 
 
-@pytest.mark.parametrize("nw,t2", [(5, 0.0), (8, 10.0), (12, 0.0), (16, 0.0), (16, 10.0)])
+@pytest.mark.parametrize("nw,t2", [(5, 0.0), (8, 10.0), (12, 0.0), (16, 0.0), (16, 10.0), (20, 10.0), (32, 0.0), (32, 10.0)])
 def test_rhs_wave_kernel_matches_oracle(tmp_path, nw, t2):
     s = _wannier_setup(str(tmp_path), nw, t2=t2)
     osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
@@ -184,7 +184,7 @@ def test_trajectory_small_tb_matches_oracle(tmp_path, stepper, order):
     assert np.max(np.abs(Pg - Pora)) <= 1e-11 * np.max(np.abs(Pora))
 
 
-@pytest.mark.parametrize("nw,stepper", [(8, "RK4"), (16, "RK4"), (16, "Taylor")])
+@pytest.mark.parametrize("nw,stepper", [(8, "RK4"), (16, "RK4"), (16, "Taylor"), (32, "RK4")])
 def test_trajectory_wannier_matches_oracle(tmp_path, nw, stepper):
     from edus_amd import workloads
     p = workloads.wannier_synthetic(str(tmp_path), nw=nw, rmax=2, nk=12, nsteps=400, dt=0.05, stepper=stepper,
