@@ -30,9 +30,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["wannier", "hbn", "wannier32"], default="wannier",
+    ap.add_argument("--workload", choices=["wannier", "hbn", "wannier32", "ab5"], default="wannier",
                     help="wannier: synthetic 16-band Wannier model, 200x200 (default); hbn: 3-band hBN tight binding, "
-                         "60x60, T2 = 10 fs (the CPU-runnable case); wannier32: 32 bands, |R| <= 15, 512x512")
+                         "60x60, T2 = 10 fs (the CPU-runnable case); wannier32: 32 bands, |R| <= 15, 512x512; "
+                         "ab5: the reference's own AB5 example run end to end through the drop-in (all 16591 steps, "
+                         "observables and files), the workload of its one published time")
     ap.add_argument("--nk", type=int, default=0, help="grid points per direction (0: the workload's own)")
     ap.add_argument("--nw", type=int, default=16)
     ap.add_argument("--rmax", type=int, default=10)
@@ -49,6 +51,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "ab5":
+        bench_ab5()
         return
 
     from edus_amd import native as N
@@ -136,6 +141,35 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+# The reference's published run (example AB5: 70x70 k, 3 bands, AB5, 16591 steps,
+# observables every 5 steps): 787 s wall on 1 MPI rank x 7 OpenMP threads.
+REF_AB5_RATE = 4900 * 16591 / 787.0
+
+
+def bench_ab5():
+    """The drop-in end to end on the reference's AB5 input: setup excluded, the time
+    loop with its observables and output files timed, as the reference's log times it."""
+    from edus_amd import native as N
+
+    src = os.path.join(ROOT, "tests", "golden", "ab5", "input.txt")
+    wd = tempfile.mkdtemp(prefix="edus_ab5_")
+    s = N.Setup(src)
+    info = s.info
+    s.run(wd, max_steps=200)  # warm-up: first launches, file creation
+    t0 = time.perf_counter()
+    n = s.run(wd)
+    el = time.perf_counter() - t0
+    value = info["nk"] * n / el
+    print(json.dumps({
+        "metric": "k-point\u00b7steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %",
+        "value": value, "unit": "k-point\u00b7steps/s", "n_gpus": 1, "steps": n, "warmup": 200,
+        "ms_per_step": el * 1e3 / n, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": value / REF_AB5_RATE, "dtype": "f64", "data": "the reference's AB5 example input",
+        "config": {"workload": "ab5_haldane_corebn_70x70_ab5_dt0.1_16591steps", "nk": info["nk"], "bands": info["nb"],
+                   "stepper": "AB5", "it_resolution": info["it_resolution"], "includes": "observables and output files",
+                   "reference_seconds": 787.0, "seconds": round(el, 3)}}))
 
 
 def copy_calibration(mib):
