@@ -1002,6 +1002,27 @@ __global__ void k_obs_final(const double *part, int nparts, int nv, double *out)
     }
 }
 
+// ================================================================ setup helpers
+// Q_x = H D_x^+ per k (Current2's sum_kc H_ik conj(D_jk), Observables_MPI.h:726-775)
+// from the packed operators: full complex [k][3][nb][nb].  Element e of k sits at
+// k*sk + e*se, component x of D at x*dxs.
+__global__ void k_build_q(const double *Hp, const double *Dp, int nk, int nb, size_t sk, size_t se, size_t dxs, int dherm, double *Q) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)nk * 3 * nb * nb) return;
+    const int j = (int)(id % nb), i = (int)((id / nb) % nb), x = (int)((id / ((long long)nb * nb)) % 3);
+    const long long k = id / (3LL * nb * nb);
+    auto el = [&](const double *base, int a, int b, bool herm) -> cd {
+        if (a == b) return mk(base[k * sk + a * se], 0.0);
+        const int p = pair_index(min(a, b), max(a, b), nb);
+        const cd v = mk(base[k * sk + (nb + 2 * p) * se], base[k * sk + (nb + 2 * p + 1) * se]);
+        return (a < b || !herm) ? v : cconj(v);
+    };
+    cd acc = mk(0.0, 0.0);
+    for (int kc = 0; kc < nb; kc++) acc = cadd(acc, cmul(el(Hp, i, kc, true), cconj(el(Dp + x * dxs, j, kc, dherm != 0))));
+    Q[2 * id] = acc.x;
+    Q[2 * id + 1] = acc.y;
+}
+
 // ================================================================ adaptive dt
 // Diagonal of the state before a step (P_W_prev, RungeKutta.cpp:1-9,139-147 and
 // Taylor_DE_solver_deriv_Polynom.cpp:75-84).  Element e of k sits at k*sk + e*se.
@@ -1560,23 +1581,16 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
                     dt[(((size_t)k * 3 + x) * nblk + t) * 2 + 1] = d[1];
                 }
         if ((rc = dalloc(c, &c->DT, dt.size())) || (rc = upload(c, c->DT, dt))) return fail(rc);
-        // Q_x = H D_x^+ for Current2 (sum_kc H_ik conj(D_jk)), full complex.
-        std::vector<double> q((size_t)p->nk * 3 * nb * nb * 2);
-        for (int k = 0; k < p->nk; k++)
-            for (int x = 0; x < 3; x++)
-                for (int i = 0; i < nb; i++)
-                    for (int j = 0; j < nb; j++) {
-                        double re = 0, im = 0;
-                        for (int kc = 0; kc < nb; kc++) {
-                            const double *h = p->H + (((size_t)k * nb + i) * nb + kc) * 2;
-                            const double *d = p->D + (((size_t)k * nb + j) * nb + kc) * 6 + 2 * x;
-                            re += h[0] * d[0] + h[1] * d[1];
-                            im += h[1] * d[0] - h[0] * d[1];
-                        }
-                        q[((((size_t)k * 3 + x) * nb + i) * nb + j) * 2] = re;
-                        q[((((size_t)k * 3 + x) * nb + i) * nb + j) * 2 + 1] = im;
-                    }
-        if ((rc = dalloc(c, &c->Q, q.size())) || (rc = upload(c, c->Q, q))) return fail(rc);
+        // Q_x = H D_x^+ for Current2, built on the device from the packed operators.
+        const size_t nq = (size_t)p->nk * 3 * nb * nb;
+        if ((rc = dalloc(c, &c->Q, nq * 2))) return fail(rc);
+        {
+            const size_t sk = c->soa ? 1 : (size_t)c->np, sek = c->soa ? (size_t)c->nkp : 1;
+            const size_t dxs = c->soa ? (size_t)c->np * c->nkp : (size_t)c->nk * c->np;
+            hipLaunchKernelGGL(k_build_q, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, (const double *)c->H,
+                               (const double *)c->D, p->nk, nb, sk, sek, dxs, c->dherm, c->Q);
+            if (hipGetLastError() != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "k_build_q launch"));
+        }
     }
     {
         std::vector<double> u(p->U, p->U + (size_t)p->nk * nb * nb * 2), t(p->T, p->T + (size_t)nb * nb);
