@@ -62,6 +62,10 @@ static thread_local std::string g_err;
     catch (const std::exception &e) {       \
         g_err = e.what();                   \
         return ret;                         \
+    }                                       \
+    catch (...) {                           \
+        g_err = "unknown host error";       \
+        return ret;                         \
     }
 
 extern "C" {
@@ -88,6 +92,7 @@ int edus_setup_from_input(const char *input_path, int32_t flags, edus_setup **ou
 void edus_setup_free(edus_setup *s) { delete s; }
 
 int edus_setup_get_info(const edus_setup *h, edus_setup_info *info) {
+    GUARD_BEGIN
     if (!h || !info) return EDUS_ERR_INVALID;
     const edus::Setup &s = h->s;
     std::memset(info, 0, sizeof *info);
@@ -115,6 +120,7 @@ int edus_setup_get_info(const edus_setup *h, edus_setup_info *info) {
     info->print_flags = (s.in.print_population ? 1 : 0) | (s.in.print_band_dispersion ? 2 : 0);
     info->print_start = s.in.print_start; info->print_end = s.in.print_end; info->print_step = s.in.print_step;
     return 0;
+    GUARD_END(EDUS_ERR_INVALID)
 }
 
 const double *edus_setup_array(const edus_setup *h, const char *name, int64_t *count) {
@@ -170,10 +176,15 @@ int edus_setup_pulse(const edus_setup *h, int32_t index, edus_pulse *out) {
 }
 
 int edus_setup_model(const edus_setup *h, char *name, int32_t len, double *params) {
+    GUARD_BEGIN
     if (!h || !name || len < 1) return EDUS_ERR_INVALID;
     const edus::Setup &s = h->s;
     if (s.in.iMode == "W") {
         std::snprintf(name, (size_t)len, "%s", s.in.ref_coord.c_str());
+        return 1;
+    }
+    if (s.in.iMode == "CY") {
+        std::snprintf(name, (size_t)len, "%s", "crystal");
         return 1;
     }
     edus::TBModel tb;
@@ -184,6 +195,7 @@ int edus_setup_model(const edus_setup *h, char *name, int32_t len, double *param
         std::memcpy(params, v, sizeof v);
     }
     return 0;
+    GUARD_END(EDUS_ERR_INVALID)
 }
 
 int edus_setup_eval_tb(const edus_setup *h, const double *kcrys, int32_t n, double *H, double *U, double *D) {
@@ -213,19 +225,23 @@ int edus_setup_eval_tb(const edus_setup *h, const double *kcrys, int32_t n, doub
 }
 
 int edus_setup_problem(const edus_setup *h, edus_problem *p) {
+    GUARD_BEGIN
     if (!h || !p) return EDUS_ERR_INVALID;
     edus::fill_problem(h->s, *p);
     if (h->s.in.coulomb) p->coulomb = &h->coul;
     return 0;
+    GUARD_END(EDUS_ERR_INVALID)
 }
 
 int edus_setup_fields(const edus_setup *h, int64_t it0, int64_t nsteps, double *fields, uint8_t *grad_on) {
+    GUARD_BEGIN
     if (!h || !fields || !grad_on || it0 < 0 || nsteps < 0) return EDUS_ERR_INVALID;
     const edus::Setup &s = h->s;
     if (it0 + nsteps > s.itfi + 1) { g_err = "step range beyond the last step"; return EDUS_ERR_INVALID; }
     const int nfe = s.fields_per_step();
     for (int64_t j = 0; j < nsteps; j++) s.step_fields(it0 + j, fields + (size_t)j * nfe * 6, grad_on + j);
     return 0;
+    GUARD_END(EDUS_ERR_INVALID)
 }
 
 double edus_setup_time(const edus_setup *h, int64_t it) {
