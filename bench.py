@@ -6,7 +6,11 @@ gaussian pump 0.6 eV / 0.5 V/A along z, RK4 with dt = 0.05 a.u. (4000 steps in t
 full run).  A step is one RK4 step (4 fused RHS+stage launches) over all k, with the
 state and every operator resident in HBM before the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--precision fp64|mixed]
+
+With --precision mixed the operators and commutator products run in FP32 (state
+and increments FP64); the line then also states the J(t) error of that form
+against the FP64 path over a window of steps around the pulse peak.
 
 Prints one JSON line.
 """
@@ -39,6 +43,8 @@ def main():
     ap.add_argument("--nw", type=int, default=16)
     ap.add_argument("--rmax", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--acc-steps", type=int, default=400, help="steps of the mixed-vs-FP64 J(t) comparison")
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps of the CPU sample (0: auto)")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -78,7 +84,8 @@ def main():
         wname = f"hbn_gen2d_3band_{nkd}x{nkd}_T2_10fs_rk4_dt0.05"
     s = N.Setup(inp)
     info = s.info
-    eng = N.Engine(s.problem())
+    prec = N.PRECISION_MIXED if args.precision == "mixed" else N.PRECISION_FP64
+    eng = N.Engine(s.problem(), precision=prec)
     t_setup = time.perf_counter() - t_setup
     nk, nb = info["nk"], info["nb"]
 
@@ -101,23 +108,30 @@ def main():
     S = 16 * nb * nb
     n_op = 4 + (1 if info["n_diss"] > 0 else 0)
     bytes_launch = (4 + n_op) * S * nk
-    stored_launch = (4 + 4) * (S // 2) * nk + (S * nk if info["n_diss"] > 0 else 0)
+    # (mixed form: the four operator matrices and U at half width)
+    opw = 1 if args.precision == "fp64" else 2
+    stored_launch = 4 * (S // 2) * nk + 4 * (S // 2 // opw) * nk + (S // opw * nk if info["n_diss"] > 0 else 0)
     avg_launch_s = dev_ms * 1e-3 / nlaunch
     achieved = bytes_launch / avg_launch_s / 1e9
     stored = stored_launch / avg_launch_s / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_stage_wave_nb{nb}_nk{nk}.json")
+    suffix = "" if args.precision == "fp64" else "_mixed"
+    tfile = os.path.join(ROOT, "profiles", f"traffic_stage_wave_nb{nb}_nk{nk}{suffix}.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": (f"k_stage_wave<{8 if nb <= 8 else 16 if nb <= 16 else 32}>" if nb > 4 else f"k_stage_small<{nb}>")
-                      + " (RK4 stages)", "bytes_per_launch": bytes_launch,
+                      + (" (RK4 stages)" if args.precision == "fp64" else " (RK4 stages, FP32 operators)"),
+            "bytes_per_launch": bytes_launch,
             "stored_bytes_per_launch": stored_launch, "achieved_stored": round(stored, 2),
             "frac_stored": round(stored / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_launch_s * 1e6, 3)}
 
     cpu = None
-    if not args.no_cpu:
+    accuracy = None
+    if args.precision == "mixed":
+        accuracy = mixed_j_error(s, eng, it0, args.acc_steps)
+    elif not args.no_cpu:
         cpu = cpu_baseline(s, eng, args.cpu_steps)
 
     out = {
@@ -131,16 +145,49 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64/f32",
         "data": "synthetic",
         "config": {"workload": wname,
-                   "nk": nk, "bands": nb, "stepper": "RK4", "precision": "fp64",
+                   "nk": nk, "bands": nb, "stepper": "RK4",
+                   "precision": "fp64" if args.precision == "fp64" else "mixed (FP32 operators and products, FP64 state)",
                    "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
                    "device_ms_per_step": dev_ms / args.steps},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if accuracy is not None:
+        out["j_error_vs_fp64"] = accuracy
     print(json.dumps(out))
+
+
+def currents(ob):
+    """J(t) per Cartesian component: Re J1 + J2 (J2 as the output file writes it)."""
+    return ob["j1"][0::2] - 2.0 * ob["j2"][1::2]
+
+
+def mixed_j_error(s, eng, it0, nsteps, every=10):
+    """max_t |J_mixed - J_fp64| / max_t |J_fp64| over nsteps from rho0 at the pulse
+    peak window, observables every `every` steps; the FP64 side is the drop-in path."""
+    from edus_amd import native as N
+
+    ref = N.Engine(s.problem(), precision=N.PRECISION_FP64)
+    P0 = s.array("rho0", True).reshape(-1, s.info["nb"], s.info["nb"])
+    f, g = s.fields(it0, nsteps)
+    jm, jr = [], []
+    for e in (eng, ref):
+        e.set_state(P0)
+    for a in range(0, nsteps, every):
+        b = min(a + every, nsteps)
+        for e, acc in ((eng, jm), (ref, jr)):
+            e.step(f[a:b], g[a:b])
+            acc.append(currents(e.observables()))
+    jm, jr = np.array(jm), np.array(jr)
+    scale = max(float(np.max(np.abs(jr))), 1e-300)
+    pm, pr = eng.get_state(), ref.get_state()
+    ref.close()
+    return {"J_rel_err": float(np.max(np.abs(jm - jr)) / scale), "J_max": scale,
+            "rho_rel_err": float(np.max(np.abs(pm - pr)) / np.max(np.abs(pr))),
+            "steps": nsteps, "observed_every": every, "reference": "FP64 path, same steps and fields"}
 
 
 # The reference's published run (example AB5: 70x70 k, 3 bands, AB5, 16591 steps,

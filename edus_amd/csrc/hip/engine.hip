@@ -58,6 +58,7 @@ struct ConstArgs {
     const double *D;     // 3 packed components (upper triangle; lower = D_ij or conj(D_ij), see dherm)
     const double *U;     // full complex AoS [k][nb][nb] (dissipation)
     const double *T;     // [nb][nb]
+    const float *Hf, *Df, *Uf;  // mixed precision: H, D, U in FP32 (same layouts)
     const int *nbr;      // [n_nbr][nkp]
     int nk, nkp, nb, nch, ncc, n_nbr, diss, dherm;  // dherm: D_ji = conj(D_ij) (else D_ji = D_ij)
     double wx2;
@@ -102,6 +103,14 @@ __device__ __forceinline__ cd csub(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}
 __device__ __forceinline__ cd cconj(cd a) { return cd{a.x, -a.y}; }
 __device__ __forceinline__ cd cscale(cd a, double s) { return cd{a.x * s, a.y * s}; }
 
+// Single-precision complex for the mixed-precision stage (operators and LDS products).
+struct cf {
+    float x, y;
+};
+template <typename R> struct CplxOf;
+template <> struct CplxOf<double> { using T = cd; };
+template <> struct CplxOf<float> { using T = cf; };
+
 // Block index remapped so that each of the 8 XCDs (which receive workgroups
 // round-robin) works on one contiguous range of k: the 12-point gradient stencil
 // then finds its neighbours in the XCD's own L2.
@@ -118,6 +127,7 @@ __host__ __device__ __forceinline__ int pair_index(int i, int j, int nb) { retur
 // touched once per launch, so they bypass the caches' normal retention and leave
 // L2 to the stage input, whose 12 stencil neighbours are re-read from it.
 __device__ __forceinline__ double ldnt(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void stnt(double *p, double v) { __builtin_nontemporal_store(v, p); }
 
 // ---------------------------------------------------------------- stage update
@@ -603,25 +613,25 @@ struct WaveGeom {
     static constexpr int PER = (NBT * NBT + TPK - 1) / TPK;  // packed doubles per thread
 };
 
-template <int NBT, int OPA, int OPB>
-__device__ __forceinline__ void gemm_lds(const cd *A, const cd *B, int nb, int t, cd (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
+template <int NBT, int OPA, int OPB, typename T>
+__device__ __forceinline__ void gemm_lds(const T *A, const T *B, int nb, int t, T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
     constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = NBT + 1;
     const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
-        for (int j = 0; j < RB; j++) acc[i][j] = mk(0.0, 0.0);
+        for (int j = 0; j < RB; j++) acc[i][j] = T{0, 0};
     for (int l = 0; l < nb; l++) {
-        cd av[RB], bv[RB];
+        T av[RB], bv[RB];
 #pragma unroll
         for (int i = 0; i < RB; i++) {
-            cd v = A[(r0 + i) * LD + l];
-            av[i] = OPA ? cconj(v) : v;
+            T v = A[(r0 + i) * LD + l];
+            av[i] = OPA ? T{v.x, -v.y} : v;
         }
 #pragma unroll
         for (int j = 0; j < RB; j++) {
-            cd v = (OPB == 0) ? B[(c0 + j) * LD + l] : B[l * LD + c0 + j];
-            bv[j] = (OPB == 2) ? cconj(v) : v;
+            T v = (OPB == 0) ? B[(c0 + j) * LD + l] : B[l * LD + c0 + j];
+            bv[j] = (OPB == 2) ? T{v.x, -v.y} : v;
         }
 #pragma unroll
         for (int i = 0; i < RB; i++)
@@ -633,8 +643,8 @@ __device__ __forceinline__ void gemm_lds(const cd *A, const cd *B, int nb, int t
     }
 }
 
-template <int NBT>
-__device__ __forceinline__ void store_block(cd *C, int t, const cd (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
+template <int NBT, typename T>
+__device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
     constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = NBT + 1;
     const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
 #pragma unroll
@@ -662,36 +672,44 @@ __host__ __device__ constexpr int wave_mats(int diss) { return diss ? 3 : 2; }
 // stage's accumulator and base state, the 12 stencil neighbours) is requested at
 // the start, so their latencies overlap each other and the LDS work; with one
 // k per workgroup the barriers only order that k's own LDS traffic.
-template <int NBT, int MODE>
+// R is the precision of the operators and of the LDS products: double for the
+// drop-in, float for the mixed-precision mode (state, stencil and increments stay
+// FP64 in both).
+template <int NBT, int MODE, typename R>
 __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, StageArgs a) {
+    using CT = typename CplxOf<R>::T;
     constexpr int TPK = WaveGeom<NBT>::TPK, RB = WaveGeom<NBT>::RB, PER = WaveGeom<NBT>::PER;
     constexpr int LD = NBT + 1, MAT = NBT * LD;
-    extern __shared__ cd smem[];
+    extern __shared__ double smem_d[];
     const int lane = threadIdx.x;  // thread of the k's workgroup
     const int k = xcd_block();
     const bool valid = k < c.nk;
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
-    cd *A = smem, *B = A + MAT, *C = c.diss ? B + MAT : B;
+    CT *A = reinterpret_cast<CT *>(smem_d), *B = A + MAT, *C = c.diss ? B + MAT : B;
+    const R *Hop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.H : (const void *)c.Hf);
+    const R *Dop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.D : (const void *)c.Df);
+    const R *Uop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.U : (const void *)c.Uf);
     const double *tab = a.tab;
-    const double Ep[3] = {tab[0], tab[1], tab[2]}, Eq[3] = {tab[3], tab[4], tab[5]};
+    const R Ep[3] = {(R)tab[0], (R)tab[1], (R)tab[2]}, Eq[3] = {(R)tab[3], (R)tab[4], (R)tab[5]};
     const size_t base = (size_t)k * NP;
     const int nn = c.n_nbr;
     const bool grad = tab[6 + nn] != 0.0;
 
-    double y[PER], pv[PER], hv[PER], dv[PER][3], g[PER], pre0[PER], pre1[PER];
+    double y[PER], pv[PER], g[PER], pre0[PER], pre1[PER];
+    R hv[PER], dv[PER][3];
     int ei[PER], ej[PER], ec[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const int e = lane + TPK * m;
-        y[m] = pv[m] = hv[m] = g[m] = pre0[m] = pre1[m] = 0.0;
-        dv[m][0] = dv[m][1] = dv[m][2] = 0.0;
+        y[m] = pv[m] = g[m] = pre0[m] = pre1[m] = 0.0;
+        hv[m] = dv[m][0] = dv[m][1] = dv[m][2] = 0;
         ei[m] = ej[m] = ec[m] = 0;
         if (valid && e < NP) {
             unpack_index(e, nb, ei[m], ej[m], ec[m]);
             y[m] = a.Y[base + e];
-            hv[m] = ldnt(c.H + base + e);
+            hv[m] = ldnt(Hop + base + e);
 #pragma unroll
-            for (int x = 0; x < 3; x++) dv[m][x] = ldnt(c.D + (size_t)x * c.nk * NP + base + e);
+            for (int x = 0; x < 3; x++) dv[m][x] = ldnt(Dop + (size_t)x * c.nk * NP + base + e);
             if constexpr (MODE == RK2 || MODE == RK3) {
                 pre0[m] = ldnt(a.ACC + base + e);
                 pre1[m] = ldnt(a.P0 + base + e);
@@ -713,7 +731,7 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
         }
     }
     if (nb < NBT)
-        for (int t = lane; t < nmat * MAT; t += TPK) A[t] = mk(0.0, 0.0);
+        for (int t = lane; t < nmat * MAT; t += TPK) A[t] = CT{0, 0};
     __syncthreads();
     // A <- P (full Hermitian), B <- M = H + E.D
 #pragma unroll
@@ -722,25 +740,26 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
         if (!valid || e >= NP) continue;
         const int i = ei[m], j = ej[m], comp = ec[m];
         const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
-        const double *E = w ? Eq : Ep;
-        double dsum = hv[m];
+        const R *E = w ? Eq : Ep;
+        R dsum = hv[m];
 #pragma unroll
         for (int x = 0; x < 3; x++) dsum += E[x] * dv[m][x];
-        double *Ad = reinterpret_cast<double *>(A), *Bd = reinterpret_cast<double *>(B);
+        R *Ad = reinterpret_cast<R *>(A), *Bd = reinterpret_cast<R *>(B);
+        const R yr = (R)y[m];
         if (i == j) {
-            A[i * LD + i] = mk(y[m], 0.0);
-            B[i * LD + i] = mk(dsum, 0.0);
+            A[i * LD + i] = CT{yr, 0};
+            B[i * LD + i] = CT{dsum, 0};
         } else if (comp == 0) {
-            Ad[2 * (i * LD + j)] = y[m];
-            Ad[2 * (j * LD + i)] = y[m];
+            Ad[2 * (i * LD + j)] = yr;
+            Ad[2 * (j * LD + i)] = yr;
             Bd[2 * (i * LD + j)] = dsum;
             Bd[2 * (j * LD + i)] = dsum;
         } else {
-            double dd = 0.0;
+            R dd = 0;
 #pragma unroll
             for (int x = 0; x < 3; x++) dd += E[x] * dv[m][x];
-            Ad[2 * (i * LD + j) + 1] = y[m];
-            Ad[2 * (j * LD + i) + 1] = -y[m];
+            Ad[2 * (i * LD + j) + 1] = yr;
+            Ad[2 * (j * LD + i) + 1] = -yr;
             Bd[2 * (i * LD + j) + 1] = dsum;
             Bd[2 * (j * LD + i) + 1] = c.dherm ? -dsum : -hv[m] + dd;
         }
@@ -748,7 +767,7 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
     __syncthreads();
     // X = P M^T; -i[M,rho] (transposed storage) = -i (X - X^+)
     {
-        cd acc[RB][RB];
+        CT acc[RB][RB];
         gemm_lds<NBT, 0, 0>(A, B, nb, lane, acc);
         if (C == B) __syncthreads();  // every lane has finished reading M
         store_block<NBT>(C, lane, acc);
@@ -760,13 +779,14 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
         if (!valid || e >= NP) continue;
         const int i = ei[m], j = ej[m];
         if (i == j) {
-            pv[m] = 2. * C[i * LD + i].y;
+            pv[m] = 2. * (double)C[i * LD + i].y;
         } else {
-            cd beta = csub(C[i * LD + j], cconj(C[j * LD + i]));
+            const CT xij = C[i * LD + j], xji = C[j * LD + i];
+            cd beta = mk((double)xij.x - (double)xji.x, (double)xij.y + (double)xji.y);
             cd v = mk(beta.y, -beta.x);
             if (i < c.nch && j >= c.nch) {
-                cd pij = A[i * LD + j];
-                v = cadd(v, mk(-c.wx2 * pij.y, c.wx2 * pij.x));
+                const CT pij = A[i * LD + j];
+                v = cadd(v, mk(-c.wx2 * (double)pij.y, c.wx2 * (double)pij.x));
             }
             pv[m] = ec[m] ? v.y : v.x;
         }
@@ -774,11 +794,11 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
     if (c.diss) {
         __syncthreads();
         // C <- U
-        const double *u = c.U + (size_t)k * nb * nb * 2;
+        const R *u = Uop + (size_t)k * nb * nb * 2;
         for (int t = lane; t < nb * nb; t += TPK)
-            if (valid) C[(t / nb) * LD + (t % nb)] = mk(ldnt(u + 2 * t), ldnt(u + 2 * t + 1));
+            if (valid) C[(t / nb) * LD + (t % nb)] = CT{ldnt(u + 2 * t), ldnt(u + 2 * t + 1)};
         __syncthreads();
-        cd acc[RB][RB];
+        CT acc[RB][RB];
         gemm_lds<NBT, 0, 0>(A, C, nb, lane, acc);     // T1 = P U^T
         __syncthreads();
         store_block<NBT>(B, lane, acc);
@@ -792,8 +812,8 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
 #pragma unroll
                 for (int j = 0; j < RB; j++) {
                     const int ia = r0 + i, ib = c0 + j;
-                    const double t = (ia < nb && ib < nb) ? c.T[ia * nb + ib] : 0.0;
-                    acc[i][j] = cscale(acc[i][j], t);
+                    const R t = (ia < nb && ib < nb) ? (R)c.T[ia * nb + ib] : (R)0;
+                    acc[i][j] = CT{acc[i][j].x * t, acc[i][j].y * t};
                 }
         }
         __syncthreads();
@@ -812,11 +832,17 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
             const int e = lane + TPK * m;
             if (!valid || e >= NP) continue;
             const int i = ei[m], j = ej[m];
-            cd w = A[i * LD + j];
-            cd hole = mk(0.0, 0.0);
-            for (int v = 0; v < c.ncc; v++) hole = cadd(hole, cscale(cmul(cconj(C[v * LD + j]), C[v * LD + i]), c.T[v * nb + v]));
-            if (i == j || ec[m] == 0) pv[m] = pv[m] - w.x + hole.x;
-            else pv[m] = pv[m] - w.y + hole.y;
+            const CT w = A[i * LD + j];
+            CT hole = CT{0, 0};
+            for (int v = 0; v < c.ncc; v++) {
+                const CT uj = C[v * LD + j], ui = C[v * LD + i];
+                const R tv = (R)c.T[v * nb + v];
+                // conj(U[v][j]) U[v][i] T[v][v]
+                hole.x += (uj.x * ui.x + uj.y * ui.y) * tv;
+                hole.y += (uj.x * ui.y - uj.y * ui.x) * tv;
+            }
+            if (i == j || ec[m] == 0) pv[m] = pv[m] - (double)w.x + (double)hole.x;
+            else pv[m] = pv[m] - (double)w.y + (double)hole.y;
         }
     }
     if (!valid) return;
@@ -1006,6 +1032,11 @@ __global__ void k_obs_final(const double *part, int nparts, int nv, double *out)
 // Q_x = H D_x^+ per k (Current2's sum_kc H_ik conj(D_jk), Observables_MPI.h:726-775)
 // from the packed operators: full complex [k][3][nb][nb].  Element e of k sits at
 // k*sk + e*se, component x of D at x*dxs.
+// FP32 copy of an operator table (mixed-precision mode), grid-stride.
+__global__ __launch_bounds__(256) void k_to_float(const double *src, float *dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = (float)src[i];
+}
+
 __global__ void k_build_q(const double *Hp, const double *Dp, int nk, int nb, size_t sk, size_t se, size_t dxs, int dherm, double *Q) {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)nk * 3 * nb * nb) return;
@@ -1301,6 +1332,8 @@ struct edus_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
+    int precision = EDUS_PRECISION_FP64;
+    float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
     bool soa = true;
     double dt = 0, wx2 = 0, j_norm = 0, tabs_norm = 0, nktot = 0;
     std::vector<double> tinv, coef;
@@ -1415,6 +1448,7 @@ template <int MODE>
 int launch_stage(edus_ctx *c, const StageArgs &a) {
     ConstArgs ca;
     ca.H = c->H; ca.D = c->D; ca.U = c->U; ca.T = c->T; ca.nbr = c->nbr;
+    ca.Hf = c->Hf; ca.Df = c->Df; ca.Uf = c->Uf;
     ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
     ca.diss = c->diss; ca.dherm = c->dherm; ca.wx2 = c->wx2;
     ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
@@ -1436,7 +1470,10 @@ int launch_stage(edus_ctx *c, const StageArgs &a) {
         auto go = [&](auto tag) {
             constexpr int NBT = decltype(tag)::value;
             const size_t lds = (size_t)wave_mats(c->diss) * NBT * (NBT + 1) * sizeof(cd);
-            hipLaunchKernelGGL((k_stage_wave<NBT, MODE>), dim3(c->nk), dim3(WaveGeom<NBT>::TPK), lds, c->stream, ca, a);
+            if (c->precision == EDUS_PRECISION_MIXED)
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float>), dim3(c->nk), dim3(WaveGeom<NBT>::TPK), lds / 2, c->stream, ca, a);
+            else
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double>), dim3(c->nk), dim3(WaveGeom<NBT>::TPK), lds, c->stream, ca, a);
         };
         if (c->nb <= 8) go(std::integral_constant<int, 8>{});
         else if (c->nb <= 16) go(std::integral_constant<int, 16>{});
@@ -1528,6 +1565,11 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
     c->nk = p->nk; c->nb = nb; c->np = nb * nb; c->nch = p->nch; c->ncc = p->ncc; c->n_nbr = p->n_nbr;
     c->stepper = p->stepper; c->order = (p->stepper == EDUS_STEPPER_RK4) ? 4 : p->order;
     c->soa = nb <= 4;
+    c->precision = p->precision;
+    if (c->precision != EDUS_PRECISION_FP64 && c->precision != EDUS_PRECISION_MIXED)
+        return fail(set_err(EDUS_ERR_INVALID, "precision must be EDUS_PRECISION_FP64 or EDUS_PRECISION_MIXED"));
+    if (c->precision == EDUS_PRECISION_MIXED && c->soa)
+        return fail(set_err(EDUS_ERR_UNSUPPORTED, "mixed precision is implemented for nb >= 5"));
     c->nkp = c->soa ? ((p->nk + 255) / 256) * 256 : p->nk;
     c->dt = p->dt; c->wx2 = p->wx2; c->j_norm = p->j_norm; c->tabs_norm = p->tabs_norm; c->nktot = p->nktot;
     c->wb.assign(p->nbr_wb, p->nbr_wb + (size_t)p->n_nbr * 3);
@@ -1596,6 +1638,16 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
         std::vector<double> u(p->U, p->U + (size_t)p->nk * nb * nb * 2), t(p->T, p->T + (size_t)nb * nb);
         if ((rc = dalloc(c, &c->U, u.size())) || (rc = upload(c, c->U, u))) return fail(rc);
         if ((rc = dalloc(c, &c->T, t.size())) || (rc = upload(c, c->T, t))) return fail(rc);
+    }
+    if (c->precision == EDUS_PRECISION_MIXED) {
+        const size_t nh = se, nd = 3 * se, nu = (size_t)p->nk * nb * nb * 2;
+        if ((rc = dalloc(c, &c->Hf, nh)) || (rc = dalloc(c, &c->Df, nd)) || (rc = dalloc(c, &c->Uf, nu))) return fail(rc);
+        const double *src[3] = {c->H, c->D, c->U};
+        float *dst[3] = {c->Hf, c->Df, c->Uf};
+        const size_t n[3] = {nh, nd, nu};
+        for (int q = 0; q < 3; q++)
+            hipLaunchKernelGGL(k_to_float, dim3((unsigned)std::min<size_t>((n[q] + 255) / 256, 65536)), dim3(256), 0, c->stream, src[q], dst[q], n[q]);
+        if (hipGetLastError() != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "k_to_float launch"));
     }
     if (p->gradH) {
         // packed Hermitian gradient of H, [k][3][NP] (always k-major)

@@ -14,6 +14,7 @@ import numpy as np
 LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 
 STEPPER_RK4, STEPPER_AB, STEPPER_TAYLOR = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
 FIELDS_PER_EVAL = 6
 
 _dp = C.POINTER(C.c_double)
@@ -23,7 +24,7 @@ _ip = C.POINTER(C.c_int32)
 class Problem(C.Structure):
     _fields_ = [
         ("nk", C.c_int32), ("nb", C.c_int32), ("nch", C.c_int32), ("ncc", C.c_int32),
-        ("n_nbr", C.c_int32), ("stepper", C.c_int32), ("order", C.c_int32), ("reserved0", C.c_int32),
+        ("n_nbr", C.c_int32), ("stepper", C.c_int32), ("order", C.c_int32), ("precision", C.c_int32),
         ("nbr", _ip), ("nbr_wb", _dp), ("H", _dp), ("D", _dp), ("U", _dp), ("T", _dp),
         ("rho0", _dp), ("gradH", _dp),
         ("wx2", C.c_double), ("dt", C.c_double), ("j_norm", C.c_double), ("tabs_norm", C.c_double),
@@ -234,9 +235,11 @@ class Setup:
 class Engine:
     """One GPU context; the arrays passed to create() are copied to HBM."""
 
-    def __init__(self, problem, stepper=None):
+    def __init__(self, problem, stepper=None, precision=None):
         self.stepper = problem.stepper if stepper is None else stepper
         self.nk, self.nb = problem.nk, problem.nb
+        if precision is not None:
+            problem.precision = precision
         h = C.c_void_p()
         _check_hip(hip.edus_create(C.byref(problem), C.byref(h)), "edus_create")
         self._h = h

@@ -38,6 +38,14 @@ extern "C" {
 #define EDUS_STEPPER_AB      1
 #define EDUS_STEPPER_TAYLOR  2
 
+/* Arithmetic of the stage kernel.  FP64 is the reference's (std::complex<double>
+ * throughout).  MIXED keeps the state, the stencil drift and every increment in
+ * FP64 but stores H, D and U as FP32 and forms the commutator and dephasing
+ * products in FP32 (relative RHS error ~1e-6); band counts 5..32 only.  No
+ * reference counterpart: an option for long runs where FP32 operators suffice. */
+#define EDUS_PRECISION_FP64  0
+#define EDUS_PRECISION_MIXED 1
+
 #define EDUS_MAX_NB          32  /* largest band count with a kernel in this build */
 #define EDUS_MAX_ORDER        8  /* largest Adams-Bashforth / Taylor order */
 
@@ -68,11 +76,11 @@ typedef struct edus_problem {
     int32_t n_nbr;       /* gradient neighbours per k, all shells (12 for 2D hexagonal) */
     int32_t stepper;     /* EDUS_STEPPER_*                                               */
     int32_t order;       /* Adams-Bashforth / Taylor order (ignored for RK4)             */
-    int32_t reserved0;
+    int32_t precision;   /* EDUS_PRECISION_FP64 (the reference's arithmetic) or _MIXED  */
     const int32_t *nbr;  /* [nk][n_nbr]  GradientIndex[ik][shell][b], shells in order    */
     const double  *nbr_wb; /* [n_nbr][3] Weigths[shell] * Bvector[shell][b][xyz]        */
     const double  *H;    /* [nk][nb][nb] complex  Hamiltonian                            */
-    const double  *D;    /* [nk][nb][nb][3] complex Dipole (must be symmetric D_ij=D_ji) */
+    const double  *D;    /* [nk][nb][nb][3] complex Dipole, symmetric or Hermitian in ij */
     const double  *U;    /* [nk][nb][nb] complex  Unitary (eigenbasis rows)              */
     const double  *T;    /* [nb][nb] real decoherence rates (Initialize_T)               */
     const double  *rho0; /* [nk][nb][nb] complex  initial P0                             */

@@ -60,11 +60,11 @@ def check_series(gpu, ora, nb, what=""):
     return g, o
 
 
-def run_engine_rows(setup, nsteps, res=None):
+def run_engine_rows(setup, nsteps, res=None, problem=None):
     """The engine through the C-ABI with the host's field schedule, rows in the oracle's layout."""
     info = setup.info
     res = res or info["it_resolution"]
-    eng = N.Engine(setup.problem())
+    eng = N.Engine(setup.problem() if problem is None else problem)
     nb = info["nb"]
     rows = []
     ef_last = np.zeros(6)
