@@ -60,6 +60,7 @@ struct ConstArgs {
     const double *T;     // [nb][nb]
     const float *Hf, *Df, *Uf;  // mixed precision: H, D, U in FP32 (same layouts)
     const int *nbr;      // [n_nbr][nkp]
+    const int *eidx;     // packed element e -> i | j << 8 | comp << 16 (wave layout)
     int nk, nkp, nb, nch, ncc, n_nbr, diss, dherm;  // dherm: D_ji = conj(D_ij) (else D_ji = D_ij)
     double wx2;
     // Coulomb mean field (small-nb kernel only): per-k reconstruction from the
@@ -338,14 +339,26 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
     // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).
     const int nn = c.n_nbr;
     if (tab[6 + nn] != 0.0) {
+        // neighbours two at a time: both index loads, then both rows, in flight together
         double g[NP];
 #pragma unroll
         for (int e = 0; e < NP; e++) g[e] = 0.0;
-        for (int n = 0; n < nn; n++) {
-            const int kn = c.nbr[(size_t)n * nkp + k];
-            const double cn = tab[6 + n];
+        for (int n0 = 0; n0 < nn; n0 += 2) {
+            const bool two = n0 + 1 < nn;
+            const int k0 = c.nbr[(size_t)n0 * nkp + k], k1 = two ? c.nbr[(size_t)(n0 + 1) * nkp + k] : k0;
+            double v0[NP], v1[NP];
 #pragma unroll
-            for (int e = 0; e < NP; e++) g[e] += cn * a.Y[e * nkp + kn];
+            for (int e = 0; e < NP; e++) {
+                v0[e] = a.Y[e * nkp + k0];
+                v1[e] = a.Y[e * nkp + k1];
+            }
+            const double c0 = tab[6 + n0], c1 = two ? tab[7 + n0] : 0.0;
+#pragma unroll
+            for (int e = 0; e < NP; e++) g[e] += c0 * v0[e];
+            if (two) {
+#pragma unroll
+                for (int e = 0; e < NP; e++) g[e] += c1 * v1[e];
+            }
         }
 #pragma unroll
         for (int e = 0; e < NP; e++) pv[e] += g[e];
@@ -611,17 +624,23 @@ struct WaveGeom {
     static constexpr int GRID = TPK == 256 ? 16 : 8;
     static constexpr int RB = NBT / GRID;
     static constexpr int PER = (NBT * NBT + TPK - 1) / TPK;  // packed doubles per thread
+    static constexpr int GATHER = PER <= 1 ? 12 : NBT <= 16 ? 6 : 4;  // stencil neighbours per load batch
 };
 
+// The matrices are zero beyond nb (filled before use, and every product of
+// zero-padded operands is zero there), so the inner loop has the compile-time
+// trip count NBT; unrolled by 4 (a full unroll hoists every LDS read and
+// costs twice the registers).
 template <int NBT, int OPA, int OPB, typename T>
-__device__ __forceinline__ void gemm_lds(const T *A, const T *B, int nb, int t, T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
+__device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
     constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = NBT + 1;
     const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
         for (int j = 0; j < RB; j++) acc[i][j] = T{0, 0};
-    for (int l = 0; l < nb; l++) {
+#pragma unroll 4
+    for (int l = 0; l < NBT; l++) {
         T av[RB], bv[RB];
 #pragma unroll
         for (int i = 0; i < RB; i++) {
@@ -653,17 +672,6 @@ __device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom
         for (int j = 0; j < RB; j++) C[(r0 + i) * LD + c0 + j] = acc[i][j];
 }
 
-// (i, j, component) of packed double e: diag -> (e, e, 0); pair -> (i<j, re/im).
-__device__ __forceinline__ void unpack_index(int e, int nb, int &i, int &j, int &comp) {
-    if (e < nb) { i = j = e; comp = 0; return; }
-    int q = e - nb, p = q >> 1;
-    comp = q & 1;
-    int r = 0, rowlen = nb - 1;
-    while (p >= rowlen) { p -= rowlen; r++; rowlen--; }
-    i = r;
-    j = r + 1 + p;
-}
-
 // Number of LDS matrices of one wave: P, M (the product X = P M^T overwrites M)
 // and, with dephasing, U.
 __host__ __device__ constexpr int wave_mats(int diss) { return diss ? 3 : 2; }
@@ -682,8 +690,7 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
     constexpr int LD = NBT + 1, MAT = NBT * LD;
     extern __shared__ double smem_d[];
     const int lane = threadIdx.x;  // thread of the k's workgroup
-    const int k = xcd_block();
-    const bool valid = k < c.nk;
+    const int k = xcd_block();  // a bijection of [0, nk): every workgroup owns a k
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
     CT *A = reinterpret_cast<CT *>(smem_d), *B = A + MAT, *C = c.diss ? B + MAT : B;
     const R *Hop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.H : (const void *)c.Hf);
@@ -695,115 +702,116 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
     const int nn = c.n_nbr;
     const bool grad = tab[6 + nn] != 0.0;
 
+    // Lanes past the last packed element (nb < NBT) repeat element NP-1: every
+    // load stays in bounds without a branch, and their LDS and global stores
+    // write the same value to the same address as the lane that owns it.
     double y[PER], pv[PER], g[PER], pre0[PER], pre1[PER];
     R hv[PER], dv[PER][3];
-    int ei[PER], ej[PER], ec[PER];
+    int eo[PER], ei[PER], ej[PER], ec[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = lane + TPK * m;
-        y[m] = pv[m] = g[m] = pre0[m] = pre1[m] = 0.0;
-        hv[m] = dv[m][0] = dv[m][1] = dv[m][2] = 0;
-        ei[m] = ej[m] = ec[m] = 0;
-        if (valid && e < NP) {
-            unpack_index(e, nb, ei[m], ej[m], ec[m]);
-            y[m] = a.Y[base + e];
-            hv[m] = ldnt(Hop + base + e);
+        const int e = min(lane + TPK * m, NP - 1);
+        eo[m] = e;
+        const int code = c.eidx[e];
+        ei[m] = code & 255;
+        ej[m] = (code >> 8) & 255;
+        ec[m] = code >> 16;
+        g[m] = pre0[m] = pre1[m] = 0.0;
+        y[m] = a.Y[base + e];
+        hv[m] = ldnt(Hop + base + e);
 #pragma unroll
-            for (int x = 0; x < 3; x++) dv[m][x] = ldnt(Dop + (size_t)x * c.nk * NP + base + e);
-            if constexpr (MODE == RK2 || MODE == RK3) {
-                pre0[m] = ldnt(a.ACC + base + e);
-                pre1[m] = ldnt(a.P0 + base + e);
-            } else if constexpr (MODE == RK4S) {
-                pre0[m] = ldnt(a.ACC + base + e);
-            }
+        for (int x = 0; x < 3; x++) dv[m][x] = ldnt(Dop + (size_t)x * c.nk * NP + base + e);
+        if constexpr (MODE == RK2 || MODE == RK3) {
+            pre0[m] = ldnt(a.ACC + base + e);
+            pre1[m] = ldnt(a.P0 + base + e);
+        } else if constexpr (MODE == RK4S) {
+            pre0[m] = ldnt(a.ACC + base + e);
         }
     }
-    // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).
-    if (valid && grad) {
-        for (int n = 0; n < nn; n++) {
-            const size_t kb = (size_t)c.nbr[(size_t)n * c.nkp + k] * NP;
-            const double cn = tab[6 + n];
+    // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).  The
+    // neighbours are fetched GB at a time: their index and data loads are all in
+    // flight together, and the sum keeps the stencil order.
+    if (grad) {
+        constexpr int GB = WaveGeom<NBT>::GATHER;
+        for (int n0 = 0; n0 < nn; n0 += GB) {
+            size_t kb[GB];
 #pragma unroll
-            for (int m = 0; m < PER; m++) {
-                const int e = lane + TPK * m;
-                if (e < NP) g[m] += cn * a.Y[kb + e];
-            }
+            for (int j = 0; j < GB; j++) kb[j] = (size_t)c.nbr[(size_t)min(n0 + j, nn - 1) * c.nkp + k] * NP;
+            double v[GB][PER];
+#pragma unroll
+            for (int j = 0; j < GB; j++)
+#pragma unroll
+                for (int m = 0; m < PER; m++) v[j][m] = a.Y[kb[j] + eo[m]];
+#pragma unroll
+            for (int j = 0; j < GB; j++)
+                if (n0 + j < nn) {
+                    const double cn = tab[6 + n0 + j];
+#pragma unroll
+                    for (int m = 0; m < PER; m++) g[m] += cn * v[j][m];
+                }
         }
     }
     if (nb < NBT)
         for (int t = lane; t < nmat * MAT; t += TPK) A[t] = CT{0, 0};
     __syncthreads();
-    // A <- P (full Hermitian), B <- M = H + E.D
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = lane + TPK * m;
-        if (!valid || e >= NP) continue;
-        const int i = ei[m], j = ej[m], comp = ec[m];
-        const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
-        const R *E = w ? Eq : Ep;
-        R dsum = hv[m];
-#pragma unroll
-        for (int x = 0; x < 3; x++) dsum += E[x] * dv[m][x];
+    // A <- P (full Hermitian), B <- M = H + E.D.  Element (i, j, comp) writes its
+    // upper value and its mirror; a diagonal element writes (y, 0).
+    {
         R *Ad = reinterpret_cast<R *>(A), *Bd = reinterpret_cast<R *>(B);
-        const R yr = (R)y[m];
-        if (i == j) {
-            A[i * LD + i] = CT{yr, 0};
-            B[i * LD + i] = CT{dsum, 0};
-        } else if (comp == 0) {
-            Ad[2 * (i * LD + j)] = yr;
-            Ad[2 * (j * LD + i)] = yr;
-            Bd[2 * (i * LD + j)] = dsum;
-            Bd[2 * (j * LD + i)] = dsum;
-        } else {
-            R dd = 0;
 #pragma unroll
-            for (int x = 0; x < 3; x++) dd += E[x] * dv[m][x];
-            Ad[2 * (i * LD + j) + 1] = yr;
-            Ad[2 * (j * LD + i) + 1] = -yr;
-            Bd[2 * (i * LD + j) + 1] = dsum;
-            Bd[2 * (j * LD + i) + 1] = c.dherm ? -dsum : -hv[m] + dd;
+        for (int m = 0; m < PER; m++) {
+            const int i = ei[m], j = ej[m], comp = ec[m];
+            const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
+            const R *E = w ? Eq : Ep;
+            R dsum = hv[m], dd = 0;
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                dsum += E[x] * dv[m][x];
+                dd += E[x] * dv[m][x];
+            }
+            const bool diag = i == j;
+            const R yr = (R)y[m];
+            const int up = 2 * (i * LD + j) + comp;
+            const int lo = diag ? up + 1 : 2 * (j * LD + i) + comp;
+            const R mlo = comp ? (c.dherm ? -dsum : -hv[m] + dd) : dsum;
+            Ad[up] = yr;
+            Ad[lo] = diag ? (R)0 : (comp ? -yr : yr);
+            Bd[up] = dsum;
+            Bd[lo] = diag ? (R)0 : mlo;
         }
     }
     __syncthreads();
     // X = P M^T; -i[M,rho] (transposed storage) = -i (X - X^+)
     {
         CT acc[RB][RB];
-        gemm_lds<NBT, 0, 0>(A, B, nb, lane, acc);
+        gemm_lds<NBT, 0, 0>(A, B, lane, acc);
         if (C == B) __syncthreads();  // every lane has finished reading M
         store_block<NBT>(C, lane, acc);
     }
     __syncthreads();
+    // pv = -i (X_ij - conj(X_ji)) plus the RWA term on core -> non-core pairs; on
+    // the diagonal this is 2 Im X_ii.
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = lane + TPK * m;
-        if (!valid || e >= NP) continue;
         const int i = ei[m], j = ej[m];
-        if (i == j) {
-            pv[m] = 2. * (double)C[i * LD + i].y;
-        } else {
-            const CT xij = C[i * LD + j], xji = C[j * LD + i];
-            cd beta = mk((double)xij.x - (double)xji.x, (double)xij.y + (double)xji.y);
-            cd v = mk(beta.y, -beta.x);
-            if (i < c.nch && j >= c.nch) {
-                const CT pij = A[i * LD + j];
-                v = cadd(v, mk(-c.wx2 * (double)pij.y, c.wx2 * (double)pij.x));
-            }
-            pv[m] = ec[m] ? v.y : v.x;
-        }
+        const CT xij = C[i * LD + j], xji = C[j * LD + i], pij = A[i * LD + j];
+        const double bre = (double)xij.x - (double)xji.x, bim = (double)xij.y + (double)xji.y;
+        const double wr = (i < c.nch && j >= c.nch) ? c.wx2 : 0.0;
+        const double vr = bim + -wr * (double)pij.y, vi = -bre + wr * (double)pij.x;
+        pv[m] = ec[m] ? vi : vr;
     }
     if (c.diss) {
         __syncthreads();
         // C <- U
         const R *u = Uop + (size_t)k * nb * nb * 2;
-        for (int t = lane; t < nb * nb; t += TPK)
-            if (valid) C[(t / nb) * LD + (t % nb)] = CT{ldnt(u + 2 * t), ldnt(u + 2 * t + 1)};
+        for (int t = lane; t < nb * nb; t += TPK) C[(t / nb) * LD + (t % nb)] = CT{ldnt(u + 2 * t), ldnt(u + 2 * t + 1)};
         __syncthreads();
         CT acc[RB][RB];
-        gemm_lds<NBT, 0, 0>(A, C, nb, lane, acc);     // T1 = P U^T
+        gemm_lds<NBT, 0, 0>(A, C, lane, acc);     // T1 = P U^T
         __syncthreads();
         store_block<NBT>(B, lane, acc);
         __syncthreads();
-        gemm_lds<NBT, 1, 1>(C, B, nb, lane, acc);     // G = conj(U) T1
+        gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // G = conj(U) T1
         {
             constexpr int GRID = WaveGeom<NBT>::GRID;
             const int r0 = (lane / GRID) * RB, c0 = (lane % GRID) * RB;
@@ -819,18 +827,16 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
         __syncthreads();
         store_block<NBT>(A, lane, acc);               // A <- K = T o G
         __syncthreads();
-        gemm_lds<NBT, 0, 2>(A, C, nb, lane, acc);     // T2 = K conj(U)
+        gemm_lds<NBT, 0, 2>(A, C, lane, acc);     // T2 = K conj(U)
         __syncthreads();
         store_block<NBT>(B, lane, acc);
         __syncthreads();
-        gemm_lds<NBT, 1, 1>(C, B, nb, lane, acc);     // W = conj(U) T2
+        gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // W = conj(U) T2
         __syncthreads();
         store_block<NBT>(A, lane, acc);
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < PER; m++) {
-            const int e = lane + TPK * m;
-            if (!valid || e >= NP) continue;
             const int i = ei[m], j = ej[m];
             const CT w = A[i * LD + j];
             CT hole = CT{0, 0};
@@ -841,17 +847,13 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
                 hole.x += (uj.x * ui.x + uj.y * ui.y) * tv;
                 hole.y += (uj.x * ui.y - uj.y * ui.x) * tv;
             }
-            if (i == j || ec[m] == 0) pv[m] = pv[m] - (double)w.x + (double)hole.x;
-            else pv[m] = pv[m] - (double)w.y + (double)hole.y;
+            pv[m] = ec[m] ? pv[m] - (double)w.y + (double)hole.y : pv[m] - (double)w.x + (double)hole.x;
         }
     }
-    if (!valid) return;
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = lane + TPK * m;
-        if (e >= NP) continue;
         const double kv = pv[m] + g[m];
-        const size_t o = base + e;
+        const size_t o = base + eo[m];
         if constexpr (MODE == RK1) {
             stnt(a.ACC + o, y[m] + kv * a.ca);
             stnt(a.OUT + o, y[m] + kv * a.cb);
@@ -1334,6 +1336,7 @@ struct edus_ctx {
     int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
     int precision = EDUS_PRECISION_FP64;
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
+    int *eidx = nullptr;  // packed element -> (i, j, comp) of the wave layout
     bool soa = true;
     double dt = 0, wx2 = 0, j_norm = 0, tabs_norm = 0, nktot = 0;
     std::vector<double> tinv, coef;
@@ -1448,7 +1451,7 @@ template <int MODE>
 int launch_stage(edus_ctx *c, const StageArgs &a) {
     ConstArgs ca;
     ca.H = c->H; ca.D = c->D; ca.U = c->U; ca.T = c->T; ca.nbr = c->nbr;
-    ca.Hf = c->Hf; ca.Df = c->Df; ca.Uf = c->Uf;
+    ca.Hf = c->Hf; ca.Df = c->Df; ca.Uf = c->Uf; ca.eidx = c->eidx;
     ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
     ca.diss = c->diss; ca.dherm = c->dherm; ca.wx2 = c->wx2;
     ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
@@ -1595,6 +1598,20 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
         if ((rc = dalloc(c, &c->HIST, se * c->order))) return fail(rc);
         if (hipMemsetAsync(c->HIST, 0, se * c->order * sizeof(double), c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "memset history"));
+    }
+    if (!c->soa) {
+        std::vector<int> ex(c->np);
+        for (int i = 0; i < nb; i++) ex[i] = i | (i << 8);
+        for (int i = 0; i < nb; i++)
+            for (int j = i + 1; j < nb; j++) {
+                const int pp = pair_index(i, j, nb);
+                ex[nb + 2 * pp] = i | (j << 8);
+                ex[nb + 2 * pp + 1] = i | (j << 8) | (1 << 16);
+            }
+        if ((rc = dalloc(c, &c->eidx, ex.size())) ||
+            hipMemcpyAsync(c->eidx, ex.data(), ex.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail(rc ? rc : set_err(EDUS_ERR_HIP, "element table upload"));
     }
     // Operators in the state layout.
     std::vector<double> hp;

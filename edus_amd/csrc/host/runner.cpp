@@ -99,7 +99,7 @@ struct Writers {
         std::ofstream f(od + label + "_" + std::to_string(ic) + std::to_string(jc) + "_" + std::to_string(icont) + ".txt");
         for (int ik = 0; ik < s.nk; ik++) {
             const int q = print_order[ik];
-            f << v[2 * (size_t)q] << " " << v[2 * (size_t)q + 1] << std::endl;
+            f << v[2 * (size_t)q] << " " << v[2 * (size_t)q + 1] << '\n';
         }
     }
 
@@ -108,13 +108,13 @@ struct Writers {
         double j1[6], j2[6], ta[6], pm[2];
         fE << std::setw(15) << std::setprecision(8) << tl * tfs;
         for (int i = 0; i < 3; i++) fE << std::setw(15) << std::setprecision(8) << ef_last[i];
-        fE << std::endl;
+        fE << '\n';
         bool print_k = false;
         if ((s.in.print_band_dispersion || s.in.print_population) && tl > s.in.print_start + icont * s.in.print_step &&
             tl < s.in.print_end) {
             print_k = true;
             icont++;
-            fTEk << tl * tfs << std::endl;
+            fTEk << tl * tfs << '\n';
         }
         int rc = edus_observables(ctx, pop.data(), j1, j2, ta, pm);
         if (rc != 0) throw Error(std::string("edus_observables: ") + edus_last_error());
@@ -128,13 +128,13 @@ struct Writers {
             if (v > n_cond) n_cond = v;
             fL << std::setprecision(16) << v << "   ";
         }
-        fL << std::endl;
+        fL << '\n';
         p_cond_max = pm[1];
         if (verbose && std::fabs(trace) > 1e-13) std::cout << "Preservation of particle num fail P_trace = " << trace << std::endl;
         if (!k_printed) {
             std::ofstream fk(od + "k_grid.txt");
             for (int ik = 0; ik < s.nk; ik++)
-                fk << s.kpt[(size_t)ik * 3] << " " << s.kpt[(size_t)ik * 3 + 1] << " " << s.kpt[(size_t)ik * 3 + 2] << std::endl;
+                fk << s.kpt[(size_t)ik * 3] << " " << s.kpt[(size_t)ik * 3 + 1] << " " << s.kpt[(size_t)ik * 3 + 2] << '\n';
             k_printed = true;
         }
         if (s.in.iCurrent) {
@@ -143,10 +143,10 @@ struct Writers {
                 fJ1 << std::setw(25) << std::setprecision(15) << std::scientific << j1[2 * x];
                 fJ1 << std::setw(25) << std::setprecision(15) << std::scientific << j1[2 * x + 1];
             }
-            fJ1 << std::endl;
+            fJ1 << '\n';
             fJ2 << std::setw(15) << std::setprecision(8) << tl * tfs;
             for (int x = 0; x < 3; x++) fJ2 << std::setw(25) << std::setprecision(15) << std::scientific << -2. * j2[2 * x + 1];
-            fJ2 << std::endl;
+            fJ2 << '\n';
         }
         if (s.in.iTAbs) {
             fT << std::setw(15) << std::setprecision(8) << tl * tfs;
@@ -155,7 +155,7 @@ struct Writers {
                 fT << std::setw(15) << std::setprecision(8) << ta[2 * x] / (double)s.nk;
                 fT << std::setw(15) << std::setprecision(8) << ta[2 * x + 1] / (double)s.nk;
             }
-            fT << std::endl;
+            fT << '\n';
         }
         if (print_k && s.in.coulomb) {  // Edia_<icont>.txt (if_resolution.cpp:40-55, Observables_MPI.h:440-477)
             std::vector<double> ek((size_t)s.nk * s.nb);
@@ -163,7 +163,7 @@ struct Writers {
             std::ofstream f(od + "Edia_" + std::to_string(icont) + ".txt");
             for (int ik = 0; ik < s.nk; ik++) {
                 for (int ic = 0; ic < s.nb; ic++) f << std::setprecision(20) << ek[(size_t)ik * s.nb + ic] << " ";
-                f << std::endl;
+                f << '\n';
             }
         }
         if (print_k && s.in.print_population) {
