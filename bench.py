@@ -34,7 +34,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["wannier", "hbn", "wannier32", "ab5"], default="wannier",
+    ap.add_argument("--workload", choices=["wannier", "hbn", "hbn_coulomb", "wannier32", "ab5", "batch"], default="wannier",
                     help="wannier: synthetic 16-band Wannier model, 200x200 (default); hbn: 3-band hBN tight binding, "
                          "60x60, T2 = 10 fs (the CPU-runnable case); wannier32: 32 bands, |R| <= 15, 512x512; "
                          "ab5: the reference's own AB5 example run end to end through the drop-in (all 16591 steps, "
@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--acc-steps", type=int, default=400, help="steps of the mixed-vs-FP64 J(t) comparison")
+    ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
+    ap.add_argument("--end-to-end", action="store_true",
+                    help="time the drop-in's own loop (observables and output files included) for --steps steps")
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps of the CPU sample (0: auto)")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -60,6 +63,9 @@ def main():
         return
     if args.workload == "ab5":
         bench_ab5()
+        return
+    if args.workload == "batch":
+        bench_batch(args)
         return
 
     from edus_amd import native as N
@@ -78,12 +84,32 @@ def main():
         inp = workloads.wannier_synthetic(wd, nw=32, rmax=15, nk=nkd, nsteps=max(5000, nsteps_total), dt=0.04,
                                           stepper="RK4", e_v_ang=0.4, nvalence=8, seed=4004, decay_ang=3.0)
         wname = f"wannier_nw32_R31x31_{nkd}x{nkd}_rk4_dt0.04"
+    elif args.workload == "hbn_coulomb":
+        nkd = args.nk or 300
+        inp = workloads.hbn_coulomb(wd, os.path.join(ROOT, "tests", "golden", "coulomb"), "G_num502657_Ncut20_", nk=nkd,
+                                    nsteps=max(6000, nsteps_total), dt=0.03, order=4)
+        wname = f"hbn_coulomb_haldane_corebn_{nkd}x{nkd}_ncut20_ab4_dt0.03"
     else:
         nkd = args.nk or 60
         inp = workloads.hbn_tb(wd, nk=nkd, nsteps=max(8000, nsteps_total), dt=0.05, stepper="RK4")
         wname = f"hbn_gen2d_3band_{nkd}x{nkd}_T2_10fs_rk4_dt0.05"
     s = N.Setup(inp)
     info = s.info
+    if args.end_to_end:
+        s.run(os.path.join(wd, "warm"), max_steps=args.warmup)
+        t0 = time.perf_counter()
+        n = s.run(os.path.join(wd, "run"), max_steps=args.steps)
+        el = time.perf_counter() - t0
+        value = info["nk"] * n / el
+        print(json.dumps({
+            "metric": "k-point\u00b7steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %",
+            "value": value, "unit": "k-point\u00b7steps/s", "n_gpus": 1, "steps": n, "warmup": args.warmup,
+            "ms_per_step": el * 1e3 / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": wname, "nk": info["nk"], "bands": info["nb"], "it_resolution": info["it_resolution"],
+                       "includes": "the drop-in's time loop: steps, observables and output files from t0",
+                       "seconds": round(el, 3)}}))
+        return
     prec = N.PRECISION_MIXED if args.precision == "mixed" else N.PRECISION_FP64
     eng = N.Engine(s.problem(), precision=prec)
     t_setup = time.perf_counter() - t_setup
@@ -105,12 +131,17 @@ def main():
     # n_op operator matrices per RHS (H, Dx, Dy, Dz; +U with dephasing).  The engine
     # stores each Hermitian matrix packed (8 nb^2 bytes), so the bytes it has to move
     # are half of that (U stays full complex).
+    # Adams-Bashforth s: S * (s + 2 + n_op) per step (read y_n, write f_n, read s-1
+    # old f, write y_n+1), one launch per step, plus S for the Coulomb pass.
     S = 16 * nb * nb
     n_op = 4 + (1 if info["n_diss"] > 0 else 0)
-    bytes_launch = (4 + n_op) * S * nk
+    rk4 = info["stepper"] == N.STEPPER_RK4
+    transfers = 4 if rk4 else info["order"] + 2 + (1 if info["ncut"] else 0)
+    bytes_launch = (transfers + n_op) * S * nk
     # (mixed form: the four operator matrices and U at half width)
     opw = 1 if args.precision == "fp64" else 2
-    stored_launch = 4 * (S // 2) * nk + 4 * (S // 2 // opw) * nk + (S // opw * nk if info["n_diss"] > 0 else 0)
+    stored_launch = transfers * (S // 2) * nk + 4 * (S // 2 // opw) * nk + (S // opw * nk if info["n_diss"] > 0 else 0)
+    stepper = "RK4" if rk4 else f"AB{info['order']}"
     avg_launch_s = dev_ms * 1e-3 / nlaunch
     achieved = bytes_launch / avg_launch_s / 1e9
     stored = stored_launch / avg_launch_s / 1e9
@@ -122,7 +153,8 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": (f"k_stage_wave<{8 if nb <= 8 else 16 if nb <= 16 else 32}>" if nb > 4 else f"k_stage_small<{nb}>")
-                      + (" (RK4 stages)" if args.precision == "fp64" else " (RK4 stages, FP32 operators)"),
+                      + (" (RK4 stages)" if rk4 else f" ({stepper} steps)")
+                      + ("" if args.precision == "fp64" else ", FP32 operators"),
             "bytes_per_launch": bytes_launch,
             "stored_bytes_per_launch": stored_launch, "achieved_stored": round(stored, 2),
             "frac_stored": round(stored / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_launch_s * 1e6, 3)}
@@ -131,7 +163,7 @@ def main():
     accuracy = None
     if args.precision == "mixed":
         accuracy = mixed_j_error(s, eng, it0, args.acc_steps)
-    elif not args.no_cpu:
+    elif not args.no_cpu and rk4 and not info["ncut"]:  # the CPU sample runs the plain RK4 oracle
         cpu = cpu_baseline(s, eng, args.cpu_steps)
 
     out = {
@@ -148,7 +180,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64/f32",
         "data": "synthetic",
         "config": {"workload": wname,
-                   "nk": nk, "bands": nb, "stepper": "RK4",
+                   "nk": nk, "bands": nb, "stepper": stepper,
                    "precision": "fp64" if args.precision == "fp64" else "mixed (FP32 operators and products, FP64 state)",
                    "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
                    "device_ms_per_step": dev_ms / args.steps},
@@ -217,6 +249,70 @@ def bench_ab5():
         "config": {"workload": "ab5_haldane_corebn_70x70_ab5_dt0.1_16591steps", "nk": info["nk"], "bands": info["nb"],
                    "stepper": "AB5", "it_resolution": info["it_resolution"], "includes": "observables and output files",
                    "reference_seconds": 787.0, "seconds": round(el, 3)}}))
+
+
+def bench_batch(args):
+    """The batched scan: independent RK4 instances of uneven grids and band counts,
+    each with its own pulse, advanced by one launch per stage and kernel class.
+    The same instances stepped one context at a time are timed beside it."""
+    from edus_amd import native as N
+    from edus_amd import workloads
+
+    wd = args.workdir or tempfile.mkdtemp(prefix="edus_batch_")
+    t_setup = time.perf_counter()
+    paths = workloads.batch_scan(wd, n=args.instances, nsteps=max(3000, args.steps + args.warmup + 10))
+    setups = []
+    for i, p in enumerate(paths):
+        setups.append(N.Setup(p))
+        if i % 16 == 15 or i == len(paths) - 1:
+            print(f"setup {i + 1}/{len(paths)} instances, {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+    prec = N.PRECISION_MIXED if args.precision == "mixed" else None
+    problems = [s.problem() for s in setups]
+    if prec is not None:  # mixed precision applies to the wave-kernel members
+        for p, s in zip(problems, setups):
+            if s.info["nb"] >= 5:
+                p.precision = prec
+    batch = N.Batch(problems)
+    t_setup = time.perf_counter() - t_setup
+    nk = np.array([s.info["nk"] for s in setups])
+    nb = np.array([s.info["nb"] for s in setups])
+    n_op = np.array([4 + (1 if s.info["n_diss"] > 0 else 0) for s in setups])
+    total = int(args.warmup + args.steps)
+    it0 = 0
+    fg = [s.fields(it0, total) for s in setups]
+    batch.step([f[:args.warmup] for f, _ in fg], [g[:args.warmup] for _, g in fg])
+    t0 = time.perf_counter()
+    batch.step([f[args.warmup:] for f, _ in fg], [g[args.warmup:] for _, g in fg])
+    dt_wall = time.perf_counter() - t0
+    dev_ms, nlaunch = batch.last_step_time()
+    value = float(nk.sum()) * args.steps / dt_wall
+    # the same instances, one context after another (per-instance launches)
+    seq_steps = max(1, min(args.steps, 20))
+    t1 = time.perf_counter()
+    for m, (f, g) in zip(batch.members, fg):
+        m.step(f[:seq_steps], g[:seq_steps])
+    seq_wall = time.perf_counter() - t1
+    seq_value = float(nk.sum()) * seq_steps / seq_wall
+    S = 16 * nb * nb
+    bytes_step = float(np.sum((16 + 4 * n_op) * S * nk))   # RK4: S (16 + 4 n_op) per k and step
+    achieved = bytes_step * args.steps / (dev_ms * 1e-3) / 1e9
+    classes = sorted({(int(b) if b <= 4 else (8 if b <= 8 else 16 if b <= 16 else 32)) for b in nb})
+    print(json.dumps({
+        "metric": "k-point\u00b7steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %",
+        "value": value, "unit": "k-point\u00b7steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt_wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec is None else "f64/f32", "data": "synthetic",
+        "config": {"workload": f"batch_scan_{len(setups)}_instances_seed3003_rk4_dt0.05", "instances": len(setups),
+                   "total_nk": int(nk.sum()), "bands": {str(b): int(np.sum(nb == b)) for b in sorted(set(nb.tolist()))},
+                   "kernel_classes": classes, "launches_per_step": nlaunch // args.steps,
+                   "precision": "fp64" if prec is None else "mixed for nb >= 5", "setup_s": round(t_setup, 1),
+                   "device_ms_per_step": dev_ms / args.steps,
+                   "unbatched": {"value": seq_value, "steps": seq_steps, "speedup_of_batch": value / seq_value}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_stage_small_batch / k_stage_wave_batch (RK4 stages)",
+                     "bytes_per_step": bytes_step},
+        "cpu_baseline": None}))
 
 
 def copy_calibration(mib):

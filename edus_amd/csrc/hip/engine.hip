@@ -63,12 +63,13 @@ struct ConstArgs {
     const int *eidx;     // packed element e -> i | j << 8 | comp << 16 (wave layout)
     int nk, nkp, nb, nch, ncc, n_nbr, diss, dherm;  // dherm: D_ji = conj(D_ij) (else D_ji = D_ij)
     double wx2;
-    // Coulomb mean field (small-nb kernel only): per-k reconstruction from the
-    // per-column partial sums Tt, peak correction against PW0, renormalised H out.
+    // Coulomb mean field (small-nb kernel only): the mean-field matrix per k,
+    // peak correction against PW0, renormalised H out.
     int coul, N2, ncut, nu;
     double n_cond;
     const double *cxm, *sxm;   // [ncut][N1] cos/sin(2 pi m k1)
-    const double *Tt;          // [2][N2][ncut][nu] complex
+    const double *Tt;          // [2][ncut][nu][N2] complex
+    const double *Xw;          // [2 nu][nkp] mean-field matrix (upper elements, re/im)
     const double *PW0;         // packed state at t = 0
     const double *EH;          // [nb] Hartree shift (minus equilibrium)
     const double *screen;      // [nb*nb] complex
@@ -146,9 +147,16 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_
     } else if constexpr (MODE == RK4S) {
         stnt(a.OUT + o, ldnt(a.ACC + o) + kv * a.ca);
     } else if constexpr (MODE == ABM) {
+        // the order-1 history loads are issued together (unrolled to the largest
+        // order, 5) before the sum, which keeps the order of the coefficients
         stnt(a.HIST + (size_t)a.slot_out * h + o, kv);
+        double hv[5];
+#pragma unroll
+        for (int s = 1; s < 5; s++) hv[s] = (s < a.order) ? ldnt(a.HIST + (size_t)a.slots[s] * h + o) : 0.0;
         double delta = a.coef[0] * kv;
-        for (int s = 1; s < a.order; s++) delta += a.coef[s] * ldnt(a.HIST + (size_t)a.slots[s] * h + o);
+#pragma unroll
+        for (int s = 1; s < 5; s++)
+            if (s < a.order) delta += a.coef[s] * hv[s];
         stnt(a.OUT + o, y + a.dt * delta);
     } else if constexpr (MODE == RHSO) {
         a.OUT[o] = kv;
@@ -156,7 +164,8 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_
         stnt(a.HIST + (size_t)a.slot_out * h + o, kv);
         const int n = a.order - 1;
         double dp[EDUS_MAX_ORDER];
-        for (int i = 0; i < n; i++) dp[i] = kv - ldnt(a.HIST + (size_t)a.slots[i + 1] * h + o);
+#pragma unroll
+        for (int i = 0; i < EDUS_MAX_ORDER - 1; i++) dp[i] = (i < n) ? kv - ldnt(a.HIST + (size_t)a.slots[i + 1] * h + o) : 0.0;
         double delta = kv * a.dt;
         for (int i = 1; i <= n; i++) {
             double al = 0.0;
@@ -169,13 +178,22 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_
 
 // ================================================================ nb <= 4: thread per k
 template <int NB, int MODE>
-__global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
+__device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs &a, const int k) {
     constexpr int NP = NB * NB;
-    const int k = xcd_block() * 256 + threadIdx.x;
     if (k >= c.nk) return;
     const size_t nkp = c.nkp;
     const double *tab = a.tab;
     const double Ep[3] = {tab[0], tab[1], tab[2]}, Eq[3] = {tab[3], tab[4], tab[5]};
+    const int nn = c.n_nbr;
+    const bool grad = nn > 0 && tab[6 + nn] != 0.0;
+    // The first stencil indices are requested with the state, so the gather at
+    // the end waits for one round trip, not two.
+    constexpr int kPreNbr = NB <= 3 ? 12 : 0;  // a multiple of the batch size below
+    int kpre[kPreNbr > 0 ? kPreNbr : 1];
+    if (grad) {
+#pragma unroll
+        for (int n = 0; n < kPreNbr; n++) kpre[n] = c.nbr[(size_t)min(n, nn - 1) * nkp + k];
+    }
 
     double y[NP];
 #pragma unroll
@@ -196,20 +214,12 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
     // when n_cond > 1e-20, real diagonal, plus the Hartree shift.
     cd Hren[NB][NB];
     if (c.coul) {
-        const int i1 = k / c.N2, i2 = k % c.N2, ncut = c.ncut, N1 = c.nk / c.N2;
         int u = 0;
 #pragma unroll
         for (int i = 0; i < NB; i++)
 #pragma unroll
             for (int j = i; j < NB; j++, u++) {
-                cd xk = mk(0.0, 0.0);
-                for (int m = 0; m < ncut; m++) {
-                    const double cm = c.cxm[m * N1 + i1], sm = c.sxm[m * N1 + i1];
-                    const double *tc = c.Tt + (((size_t)0 * c.N2 + i2) * ncut + m) * c.nu * 2 + 2 * u;
-                    const double *ts = c.Tt + (((size_t)1 * c.N2 + i2) * ncut + m) * c.nu * 2 + 2 * u;
-                    xk.x -= cm * tc[0] + sm * ts[0];
-                    xk.y -= cm * tc[1] + sm * ts[1];
-                }
+                cd xk = mk(c.Xw[(size_t)(2 * u) * nkp + k], c.Xw[(size_t)(2 * u + 1) * nkp + k]);
                 if (c.n_cond > 1e-20) {
                     const int idx = (i != j) ? i * NB + j : 0;
                     cd scr = mk(c.screen[2 * idx], -c.screen[2 * idx + 1]);  // conj(Screen_const)
@@ -241,7 +251,7 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
 #pragma unroll
         for (int j = i; j < NB; j++) {
             const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
-            const double *E = w ? Eq : Ep;
+            const double E[3] = {w ? Eq[0] : Ep[0], w ? Eq[1] : Ep[1], w ? Eq[2] : Ep[2]};
             cd h, dx[3];
             if (i == j) {
                 h = mk(ldnt(c.H + i * nkp + k), 0.0);
@@ -336,35 +346,52 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
                 }
             }
     }
-    // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).
-    const int nn = c.n_nbr;
-    if (tab[6 + nn] != 0.0) {
-        // neighbours two at a time: both index loads, then both rows, in flight together
+    // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).  The
+    // neighbour indices were requested at the start; the rows come GB neighbours
+    // at a time, all in flight together, and the sum keeps the stencil order.
+    if (grad) {
+        constexpr int GB = NB <= 2 ? 6 : NB == 3 ? 4 : 2;
         double g[NP];
 #pragma unroll
         for (int e = 0; e < NP; e++) g[e] = 0.0;
-        for (int n0 = 0; n0 < nn; n0 += 2) {
-            const bool two = n0 + 1 < nn;
-            const int k0 = c.nbr[(size_t)n0 * nkp + k], k1 = two ? c.nbr[(size_t)(n0 + 1) * nkp + k] : k0;
-            double v0[NP], v1[NP];
+        auto batch = [&](int n0, const int (&kk)[GB]) {
+            double v[GB][NP];
 #pragma unroll
-            for (int e = 0; e < NP; e++) {
-                v0[e] = a.Y[e * nkp + k0];
-                v1[e] = a.Y[e * nkp + k1];
-            }
-            const double c0 = tab[6 + n0], c1 = two ? tab[7 + n0] : 0.0;
+            for (int j = 0; j < GB; j++)
 #pragma unroll
-            for (int e = 0; e < NP; e++) g[e] += c0 * v0[e];
-            if (two) {
+                for (int e = 0; e < NP; e++) v[j][e] = a.Y[e * nkp + kk[j]];
 #pragma unroll
-                for (int e = 0; e < NP; e++) g[e] += c1 * v1[e];
-            }
+            for (int j = 0; j < GB; j++)
+                if (n0 + j < nn) {
+                    const double cn = tab[6 + n0 + j];
+#pragma unroll
+                    for (int e = 0; e < NP; e++) g[e] += cn * v[j][e];
+                }
+        };
+#pragma unroll
+        for (int n0 = 0; n0 < kPreNbr; n0 += GB) {
+            if (n0 >= nn) break;
+            int kk[GB];
+#pragma unroll
+            for (int j = 0; j < GB; j++) kk[j] = kpre[n0 + j];
+            batch(n0, kk);
+        }
+        for (int n0 = kPreNbr; n0 < nn; n0 += GB) {
+            int kk[GB];
+#pragma unroll
+            for (int j = 0; j < GB; j++) kk[j] = c.nbr[(size_t)min(n0 + j, nn - 1) * nkp + k];
+            batch(n0, kk);
         }
 #pragma unroll
         for (int e = 0; e < NP; e++) pv[e] += g[e];
     }
 #pragma unroll
     for (int e = 0; e < NP; e++) stage_update<MODE>(a, e * nkp + k, (size_t)NP * nkp, pv[e], y[e]);
+}
+
+template <int NB, int MODE>
+__global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
+    stage_small<NB, MODE>(c, a, xcd_block() * 256 + threadIdx.x);
 }
 
 // ================================================================ Coulomb mean field
@@ -386,46 +413,83 @@ struct CoulArgs {
     const double *A, *B, *C, *D, *vh;  // complex
     double *EH;
     const double *EH0;
-    double *Tt;                // [2][N2][ncut][nu] complex
+    double *Tt;                // [2][ncut][nu][N2] complex
+    double *Xw;                // [2 nu][nkp]: mean-field matrix per k (upper elements, re/im)
 };
 
-__global__ void k_coul_s1(CoulArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int total = a.N1 * a.ncut * a.NP;
-    if (t >= total) return;
-    const int e = t % a.NP, n = (t / a.NP) % a.ncut, i1 = t / (a.NP * a.ncut);
-    const double *row = a.Y + (size_t)e * a.nkp + (size_t)i1 * a.N2;
-    double sc = 0.0, ss = 0.0;
-    for (int i2 = 0; i2 < a.N2; i2++) {
-        const double p = row[i2];
-        sc += a.cyn[n * a.N2 + i2] * p;
-        ss += a.syn[n * a.N2 + i2] * p;
+constexpr int kCoulThreads = 256;
+
+// S_y[i1][n][e] = sum_i2 y_n(i2) P_e(i1, i2): one workgroup of 1024 threads per
+// row i1.  Thread t owns output (n, e) = t % 256 and the quarter t / 256 of the
+// columns; the row and the trig tables are read through L1 (every output of the
+// row shares them), and the four quarter sums are added in order.
+constexpr int kCoulS1Threads = 1024;
+__global__ __launch_bounds__(kCoulS1Threads) void k_coul_s1(CoulArgs a) {
+    __shared__ double red[2][kCoulS1Threads];
+    const int i1 = blockIdx.x, t = threadIdx.x, NP = a.NP, ncut = a.ncut;
+    const int quarter = t >> 8, nout = ncut * NP;
+    const int q0 = (a.N2 * quarter) / 4, q1 = (a.N2 * (quarter + 1)) / 4;
+    for (int o0 = 0; o0 < nout; o0 += 256) {  // at most two passes (ncut * NP <= 512)
+        const int o = o0 + (t & 255);
+        double sc = 0.0, ss = 0.0;
+        if (o < nout) {
+            const int n = o / NP, e = o % NP;
+            const double *row = a.Y + (size_t)e * a.nkp + (size_t)i1 * a.N2;
+            const double *cy = a.cyn + (size_t)n * a.N2, *sy = a.syn + (size_t)n * a.N2;
+#pragma unroll 8
+            for (int i2 = q0; i2 < q1; i2++) {
+                const double p = row[i2];
+                sc += cy[i2] * p;
+                ss += sy[i2] * p;
+            }
+        }
+        __syncthreads();  // the previous pass has read red
+        red[0][t] = sc;
+        red[1][t] = ss;
+        __syncthreads();
+        if (t < 256 && o0 + t < nout) {
+            const int n = (o0 + t) / NP, e = (o0 + t) % NP;
+            const double c0 = ((red[0][t] + red[0][t + 256]) + red[0][t + 512]) + red[0][t + 768];
+            const double c1 = ((red[1][t] + red[1][t + 256]) + red[1][t + 512]) + red[1][t + 768];
+            a.S[(((size_t)0 * a.N1 + i1) * ncut + n) * NP + e] = c0;
+            a.S[(((size_t)1 * a.N1 + i1) * ncut + n) * NP + e] = c1;
+        }
     }
-    a.S[(((size_t)0 * a.N1 + i1) * a.ncut + n) * a.NP + e] = sc;
-    a.S[(((size_t)1 * a.N1 + i1) * a.ncut + n) * a.NP + e] = ss;
 }
 
-__global__ void k_coul_s2(CoulArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nmm = a.ncut * a.ncut;
-    if (t >= nmm * a.NP) return;
-    const int e = t % a.NP, mm = t / a.NP, m = mm / a.ncut, n = mm % a.ncut;
+// F_xy[mm][e] = w (1 + corr_mn) sum_i1 x_m(i1) S_y[i1][n][e]: one workgroup per
+// mm; thread t takes element e = t % NP and every (256/NP)-th term of the sum,
+// and the parts are added in a fixed order per element.
+__global__ __launch_bounds__(kCoulThreads) void k_coul_s2(CoulArgs a) {
+    __shared__ double red[4][kCoulThreads];
+    const int nmm = a.ncut * a.ncut, mm = blockIdx.x, m = mm / a.ncut, n = mm % a.ncut, t = threadIdx.x;
+    const int NP = a.NP, nparts = kCoulThreads / NP, e = t % NP, part = t / NP;
     double cc = 0, cs = 0, sc = 0, ss = 0;
-    for (int i1 = 0; i1 < a.N1; i1++) {
-        const double xc = a.cxm[m * a.N1 + i1], xs = a.sxm[m * a.N1 + i1];
-        const double yc = a.S[(((size_t)0 * a.N1 + i1) * a.ncut + n) * a.NP + e];
-        const double ys = a.S[(((size_t)1 * a.N1 + i1) * a.ncut + n) * a.NP + e];
-        cc += xc * yc;
-        cs += xc * ys;
-        sc += xs * yc;
-        ss += xs * ys;
+    if (part < nparts) {
+        for (int i1 = part; i1 < a.N1; i1 += nparts) {
+            const double xc = a.cxm[m * a.N1 + i1], xs = a.sxm[m * a.N1 + i1];
+            const double yc = a.S[(((size_t)0 * a.N1 + i1) * a.ncut + n) * NP + e];
+            const double ys = a.S[(((size_t)1 * a.N1 + i1) * a.ncut + n) * NP + e];
+            cc += xc * yc;
+            cs += xc * ys;
+            sc += xs * yc;
+            ss += xs * ys;
+        }
     }
-    const double corr = -4 * M_PI * M_PI * (m * m + n * n) / ((double)a.nk * 27);
-    const double f = a.w * (1. + corr);
-    a.F[((size_t)0 * nmm + mm) * a.NP + e] = cc * f;
-    a.F[((size_t)1 * nmm + mm) * a.NP + e] = cs * f;
-    a.F[((size_t)2 * nmm + mm) * a.NP + e] = sc * f;
-    a.F[((size_t)3 * nmm + mm) * a.NP + e] = ss * f;
+    red[0][t] = cc;
+    red[1][t] = cs;
+    red[2][t] = sc;
+    red[3][t] = ss;
+    __syncthreads();
+    if (t < NP) {
+        const double corr = -4 * M_PI * M_PI * (m * m + n * n) / ((double)a.nk * 27);
+        const double f = a.w * (1. + corr);
+        for (int qd = 0; qd < 4; qd++) {
+            double v = 0.0;
+            for (int q = 0; q < nparts; q++) v += red[qd][q * NP + t];
+            a.F[((size_t)qd * nmm + mm) * NP + t] = v * f;
+        }
+    }
 }
 
 __global__ void k_coul_x(CoulArgs a) {
@@ -484,7 +548,7 @@ __global__ void k_coul_s3(CoulArgs a) {
     const int nb = a.nb, nmm = a.ncut * a.ncut, nb2 = nb * nb;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.N2 * a.ncut * a.nu) return;
-    const int u = t % a.nu, m = (t / a.nu) % a.ncut, i2 = t / (a.nu * a.ncut);
+    const int i2 = t % a.N2, u = (t / a.N2) % a.nu, m = t / (a.N2 * a.nu);
     int i = 0, j = 0;
     for (int q = u, r = 0; ; r++) {
         if (q < nb - r) { i = r; j = r + q; break; }
@@ -499,8 +563,41 @@ __global__ void k_coul_s3(CoulArgs a) {
         ts = cadd(ts, cadd(cscale(X[2 * (size_t)nmm * nb2 + d], yc), cscale(X[3 * (size_t)nmm * nb2 + d], ys)));
     }
     cd *T = reinterpret_cast<cd *>(a.Tt);
-    T[((size_t)0 * a.N2 + i2) * a.ncut * a.nu + m * a.nu + u] = tc;
-    T[((size_t)1 * a.N2 + i2) * a.ncut * a.nu + m * a.nu + u] = ts;
+    T[(((size_t)0 * a.ncut + m) * a.nu + u) * a.N2 + i2] = tc;
+    T[(((size_t)1 * a.ncut + m) * a.nu + u) * a.N2 + i2] = ts;
+}
+
+// Mean-field matrix of every k from the column sums (Coulomb.h:510-541):
+// X_W(i1, i2)_u = -sum_m [cos_m(i1) Tc(m, u, i2) + sin_m(i1) Ts(m, u, i2)], stored
+// SoA as Xw[2u + re/im][k].  A workgroup serves 8 rows i1, so each table value
+// read serves 8 k.
+constexpr int kCoulRows = 8;
+__global__ __launch_bounds__(256) void k_coul_w(CoulArgs a, int N1) {
+    const int v = blockIdx.y * 256 + threadIdx.x;
+    if (v >= a.nu * a.N2) return;
+    const int u = v / a.N2, i2 = v % a.N2, r0 = blockIdx.x * kCoulRows, ncut = a.ncut;
+    const cd *T = reinterpret_cast<const cd *>(a.Tt);
+    cd acc[kCoulRows];
+#pragma unroll
+    for (int r = 0; r < kCoulRows; r++) acc[r] = mk(0.0, 0.0);
+    for (int m = 0; m < ncut; m++) {
+        const cd tc = T[(((size_t)0 * ncut + m) * a.nu + u) * a.N2 + i2];
+        const cd ts = T[(((size_t)1 * ncut + m) * a.nu + u) * a.N2 + i2];
+#pragma unroll
+        for (int r = 0; r < kCoulRows; r++) {
+            const int i1 = min(r0 + r, N1 - 1);
+            const double cm = a.cxm[m * N1 + i1], sm = a.sxm[m * N1 + i1];
+            acc[r].x -= cm * tc.x + sm * ts.x;
+            acc[r].y -= cm * tc.y + sm * ts.y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kCoulRows; r++) {
+        if (r0 + r >= N1) break;
+        const size_t k = (size_t)(r0 + r) * a.N2 + i2;
+        a.Xw[(size_t)(2 * u) * a.nkp + k] = acc[r].x;
+        a.Xw[(size_t)(2 * u + 1) * a.nkp + k] = acc[r].y;
+    }
 }
 
 // Observables with the mean field (Observables_MPI.h:286-304, if_resolution.cpp:125-192):
@@ -624,6 +721,7 @@ struct WaveGeom {
     static constexpr int GRID = TPK == 256 ? 16 : 8;
     static constexpr int RB = NBT / GRID;
     static constexpr int PER = (NBT * NBT + TPK - 1) / TPK;  // packed doubles per thread
+    static constexpr int LD = NBT + 1;  // LDS row length in complex elements (one pad)
     static constexpr int GATHER = PER <= 1 ? 12 : NBT <= 16 ? 6 : 4;  // stencil neighbours per load batch
 };
 
@@ -633,7 +731,7 @@ struct WaveGeom {
 // costs twice the registers).
 template <int NBT, int OPA, int OPB, typename T>
 __device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
-    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = NBT + 1;
+    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = WaveGeom<NBT>::LD;
     const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
 #pragma unroll
     for (int i = 0; i < RB; i++)
@@ -664,7 +762,7 @@ __device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)
 
 template <int NBT, typename T>
 __device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
-    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = NBT + 1;
+    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = WaveGeom<NBT>::LD;
     const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
 #pragma unroll
     for (int i = 0; i < RB; i++)
@@ -684,13 +782,12 @@ __host__ __device__ constexpr int wave_mats(int diss) { return diss ? 3 : 2; }
 // drop-in, float for the mixed-precision mode (state, stencil and increments stay
 // FP64 in both).
 template <int NBT, int MODE, typename R>
-__global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, StageArgs a) {
+__device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &a, const int k) {
     using CT = typename CplxOf<R>::T;
     constexpr int TPK = WaveGeom<NBT>::TPK, RB = WaveGeom<NBT>::RB, PER = WaveGeom<NBT>::PER;
-    constexpr int LD = NBT + 1, MAT = NBT * LD;
+    constexpr int LD = WaveGeom<NBT>::LD, MAT = NBT * LD;
     extern __shared__ double smem_d[];
     const int lane = threadIdx.x;  // thread of the k's workgroup
-    const int k = xcd_block();  // a bijection of [0, nk): every workgroup owns a k
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
     CT *A = reinterpret_cast<CT *>(smem_d), *B = A + MAT, *C = c.diss ? B + MAT : B;
     const R *Hop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.H : (const void *)c.Hf);
@@ -762,7 +859,7 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
         for (int m = 0; m < PER; m++) {
             const int i = ei[m], j = ej[m], comp = ec[m];
             const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
-            const R *E = w ? Eq : Ep;
+            const R E[3] = {w ? Eq[0] : Ep[0], w ? Eq[1] : Ep[1], w ? Eq[2] : Ep[2]};  // selects, no pointer
             R dsum = hv[m], dd = 0;
 #pragma unroll
             for (int x = 0; x < 3; x++) {
@@ -868,6 +965,43 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, 
     }
 }
 
+template <int NBT, int MODE, typename R>
+__global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, StageArgs a) {
+    stage_wave<NBT, MODE, R>(c, a, xcd_block());  // xcd_block is a bijection of [0, nk): every workgroup owns a k
+}
+
+// ---------------------------------------------------------------- batched instances
+// One launch per RK4 stage and kernel class advances several independent
+// problems: workgroup b serves (instance, block of that instance) from a device
+// table, and every instance's arguments sit in a device array.  The stage body
+// is the single-problem kernel's, so a batched step is bit-identical to
+// stepping each problem on its own.
+struct BatchEntry {
+    ConstArgs c;
+    StageArgs a[4];  // RK4 stages; tab is the instance's row of the call's first step
+};
+
+template <int MODE>
+__host__ __device__ constexpr int rk4_stage_of() { return MODE == RK1 ? 0 : MODE == RK2 ? 1 : MODE == RK3 ? 2 : 3; }
+
+template <int NB, int MODE>
+__global__ __launch_bounds__(256) void k_stage_small_batch(const BatchEntry *E, const int2 *blk, size_t tab_step) {
+    const int2 w = blk[xcd_block()];
+    const BatchEntry &be = E[w.x];
+    StageArgs a = be.a[rk4_stage_of<MODE>()];
+    a.tab += tab_step;
+    stage_small<NB, MODE>(be.c, a, w.y * 256 + threadIdx.x);
+}
+
+template <int NBT, int MODE, typename R>
+__global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave_batch(const BatchEntry *E, const int2 *blk, size_t tab_step) {
+    const int2 w = blk[xcd_block()];
+    const BatchEntry &be = E[w.x];
+    StageArgs a = be.a[rk4_stage_of<MODE>()];
+    a.tab += tab_step;
+    stage_wave<NBT, MODE, R>(be.c, a, w.y);
+}
+
 // ================================================================ observables
 // Values per partial: pop[nb], j1 re x3, j2 (re,im) x3, tabs (re,im) x3 -> nb + 15,
 // plus min and max of the population diagnostics.
@@ -914,23 +1048,38 @@ __device__ __forceinline__ cd load_p(const ObsArgs &o, int k, int i, int j) {
     return (i < j) ? v : cconj(v);
 }
 
-constexpr int kObsWaves = 4;
+// One wave per workgroup; the wave walks its k points and stages each k's full
+// P (from the packed state) and U in its LDS, so the population quadratic form
+// and the current / absorption sums read their matrix operands from LDS.
+constexpr int kObsMaxBlocks = 8192;
 
-__global__ __launch_bounds__(64 * kObsWaves) void k_observables(ObsArgs o) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nb = o.nb, NP = nb * nb, nch = o.nch;
-    const int gw = blockIdx.x * kObsWaves + wave, nw = gridDim.x * kObsWaves;
+__global__ __launch_bounds__(64) void k_observables(ObsArgs o) {
+    extern __shared__ double obs_lds[];
+    const int lane = threadIdx.x;
+    const int nb = o.nb, NP = nb * nb, NN = nb * nb, nch = o.nch;
+    // U rows padded to nb + 1: lane c reads row c, so consecutive lanes start
+    // 4 banks apart instead of on the same bank
+    const int LU = nb + 1;
+    cd *Pl = reinterpret_cast<cd *>(obs_lds), *Ul = Pl + NN;
     Kahan pop, j1[3], j2[6], ta[6];
     double pmin = 1.0, pmax = 0.0;
-    for (int k = gw; k < o.nk; k += nw) {
+    for (int k = blockIdx.x; k < o.nk; k += gridDim.x) {
+        __syncthreads();  // the previous k's LDS reads are done
+        for (int t = lane; t < NN; t += 64) {
+            Pl[t] = load_p(o, k, t / nb, t % nb);
+            const double *u = o.U + ((size_t)k * NN + t) * 2;
+            Ul[(t / nb) * LU + t % nb] = mk(u[0], u[1]);
+        }
+        __syncthreads();
         // population of band c = Re (conj(U) P_sym U^T)_cc (InitParameters.h:264-335)
         if (lane < nb) {
-            const double *u = o.U + ((size_t)k * nb + lane) * nb * 2;
+            const cd *uc = Ul + lane * LU;
             cd s = mk(0.0, 0.0);
             for (int ii = 0; ii < nb; ii++) {
                 cd t = mk(0.0, 0.0);
-                for (int jj = 0; jj < nb; jj++) t = cadd(t, cmul(load_p(o, k, ii, jj), mk(u[2 * jj], u[2 * jj + 1])));
-                s = cadd(s, cmul(mk(u[2 * ii], -u[2 * ii + 1]), t));
+#pragma unroll 4
+                for (int jj = 0; jj < nb; jj++) t = cadd(t, cmul(Pl[ii * nb + jj], uc[jj]));
+                s = cadd(s, cmul(cconj(uc[ii]), t));
             }
             const double pik = s.x;
             pop.add(pik);
@@ -943,20 +1092,16 @@ __global__ __launch_bounds__(64 * kObsWaves) void k_observables(ObsArgs o) {
             for (int e = lane; e < NP; e += 64) {
                 const double pe = o.P[k * o.sk + e * o.se];
                 const double wgt = (e < nb) ? 1.0 : 2.0;   // a pair carries (i,j) and (j,i)
-                for (int x = 0; x < 3; x++) {
-                    const double g = o.GE[((size_t)k * 3 + x) * NP + e];
-                    // Re(P_ij G_ij): re*re - im*im over the (re, im) doubles of the pair
-                    const double sgn = (e >= nb && ((e - nb) & 1)) ? -1.0 : 1.0;
-                    j1[x].add(wgt * sgn * pe * g);
-                }
+                // Re(P_ij G_ij): re*re - im*im over the (re, im) doubles of the pair
+                const double sgn = (e >= nb && ((e - nb) & 1)) ? -1.0 : 1.0;
+                for (int x = 0; x < 3; x++) j1[x].add(wgt * sgn * pe * o.GE[((size_t)k * 3 + x) * NP + e]);
             }
         }
         // Current2: sum_ij P_ij (H D^+)_ij
-        for (int t = lane; t < nb * nb; t += 64) {
-            const int i = t / nb, j = t % nb;
-            const cd pij = load_p(o, k, i, j);
+        for (int t = lane; t < NN; t += 64) {
+            const cd pij = Pl[t];
             for (int x = 0; x < 3; x++) {
-                const double *q = o.Q + (((size_t)k * 3 + x) * nb * nb + t) * 2;
+                const double *q = o.Q + (((size_t)k * 3 + x) * NN + t) * 2;
                 cd v = cmul(pij, mk(q[0], q[1]));
                 j2[2 * x].add(v.x);
                 j2[2 * x + 1].add(v.y);
@@ -966,7 +1111,7 @@ __global__ __launch_bounds__(64 * kObsWaves) void k_observables(ObsArgs o) {
         const int nblk = nch * (nb - nch);
         for (int t = lane; t < nblk; t += 64) {
             const int i = t / (nb - nch), j = nch + t % (nb - nch);
-            const cd pij = load_p(o, k, i, j);
+            const cd pij = Pl[i * nb + j];
             for (int x = 0; x < 3; x++) {
                 const double *d = o.DT + (((size_t)k * 3 + x) * nblk + t) * 2;
                 cd v = cmul(mk(d[0], d[1]), pij);
@@ -975,58 +1120,70 @@ __global__ __launch_bounds__(64 * kObsWaves) void k_observables(ObsArgs o) {
             }
         }
     }
-    // Combine lanes and waves deterministically: lane sums by shuffle, waves through LDS.
-    __shared__ double red[kObsWaves][EDUS_MAX_NB + 17];
+    // Lane sums by shuffle, in a fixed order.
     const int nv = nb + 15;
+    double *out = o.part + (size_t)blockIdx.x * (nv + 2);
     for (int c = 0; c < nb; c++) {
-        double v = (lane == c) ? pop.s : 0.0;
-        v = wave_sum(v);
-        if (lane == 0) red[wave][c] = v;
+        const double v = wave_sum((lane == c) ? pop.s : 0.0);
+        if (lane == 0) out[c] = v;
     }
     for (int x = 0; x < 3; x++) {
-        double v = wave_sum(j1[x].s);
-        if (lane == 0) red[wave][nb + x] = v;
+        const double v = wave_sum(j1[x].s);
+        if (lane == 0) out[nb + x] = v;
     }
     for (int x = 0; x < 6; x++) {
-        double v = wave_sum(j2[x].s);
-        if (lane == 0) red[wave][nb + 3 + x] = v;
-        v = wave_sum(ta[x].s);
-        if (lane == 0) red[wave][nb + 9 + x] = v;
+        const double v = wave_sum(j2[x].s), w = wave_sum(ta[x].s);
+        if (lane == 0) { out[nb + 3 + x] = v; out[nb + 9 + x] = w; }
     }
-    double mn = wave_min(pmin), mx = wave_max(pmax);
-    if (lane == 0) { red[wave][nv] = mn; red[wave][nv + 1] = mx; }
-    __syncthreads();
-    if (threadIdx.x < nv + 2) {
-        const int c = threadIdx.x;
-        double v;
-        if (c < nv) { v = 0.0; for (int w = 0; w < kObsWaves; w++) v += red[w][c]; }
-        else if (c == nv) { v = 1.0; for (int w = 0; w < kObsWaves; w++) v = fmin(v, red[w][c]); }
-        else { v = 0.0; for (int w = 0; w < kObsWaves; w++) v = fmax(v, red[w][c]); }
-        o.part[(size_t)blockIdx.x * (nv + 2) + c] = v;
-    }
+    const double mn = wave_min(pmin), mx = wave_max(pmax);
+    if (lane == 0) { out[nv] = mn; out[nv + 1] = mx; }
 }
 
-// Sum of the block partials in a fixed order with Kahan compensation.
-__global__ void k_obs_final(const double *part, int nparts, int nv, double *out) {
-    const int c = threadIdx.x;
-    if (c >= nv + 2) return;
+// Final reduction of the per-block partials: one workgroup per value.  Each
+// thread sums a strided subset with Kahan compensation, and the 256 (sum,
+// compensation) pairs are combined pairwise with an error-free two-sum, so the
+// result keeps the compensated accuracy; the last two values are the
+// population diagnostics' min and max.
+__global__ __launch_bounds__(256) void k_obs_final(const double *part, int nparts, int nv, double *out) {
+    __shared__ double shi[256], slo[256];
+    const int c = blockIdx.x, t = threadIdx.x;
+    const size_t stride = (size_t)nv + 2;
     if (c < nv) {
         double s = 0.0, comp = 0.0;
-        for (int b = 0; b < nparts; b++) {
-            double yv = part[(size_t)b * (nv + 2) + c] - comp;
-            double t = s + yv;
-            comp = (t - s) - yv;
-            s = t;
+        for (int b = t; b < nparts; b += 256) {
+            const double yv = part[(size_t)b * stride + c] - comp;
+            const double q = s + yv;
+            comp = (q - s) - yv;
+            s = q;
         }
-        out[c] = s;
-    } else if (c == nv) {
-        double v = 1.0;
-        for (int b = 0; b < nparts; b++) v = fmin(v, part[(size_t)b * (nv + 2) + c]);
-        out[c] = v;
+        shi[t] = s;
+        slo[t] = -comp;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (t < h) {
+                const double a = shi[t], b = shi[t + h];
+                const double sum = a + b, bp = sum - a;
+                const double err = (a - (sum - bp)) + (b - bp);
+                shi[t] = sum;
+                slo[t] = slo[t] + slo[t + h] + err;
+            }
+            __syncthreads();
+        }
+        if (t == 0) out[c] = shi[0] + slo[0];
     } else {
-        double v = 0.0;
-        for (int b = 0; b < nparts; b++) v = fmax(v, part[(size_t)b * (nv + 2) + c]);
-        out[c] = v;
+        const bool mn = c == nv;
+        double v = mn ? 1.0 : 0.0;
+        for (int b = t; b < nparts; b += 256) {
+            const double x = part[(size_t)b * stride + c];
+            v = mn ? fmin(v, x) : fmax(v, x);
+        }
+        shi[t] = v;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (t < h) shi[t] = mn ? fmin(shi[t], shi[t + h]) : fmax(shi[t], shi[t + h]);
+            __syncthreads();
+        }
+        if (t == 0) out[c] = shi[0];
     }
 }
 
@@ -1338,6 +1495,7 @@ struct edus_ctx {
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
     int *eidx = nullptr;  // packed element -> (i, j, comp) of the wave layout
     bool soa = true;
+    bool own_stream = true;  // false when the stream belongs to a batch
     double dt = 0, wx2 = 0, j_norm = 0, tabs_norm = 0, nktot = 0;
     std::vector<double> tinv, coef;
     std::vector<double> wb;  // [n_nbr][3]
@@ -1359,7 +1517,7 @@ struct edus_ctx {
     double n_cond = 0.0;
     double *cxm = nullptr, *sxm = nullptr, *cyn = nullptr, *syn = nullptr, *cS = nullptr, *cF = nullptr;
     double *cX = nullptr, *cX0 = nullptr, *cA = nullptr, *cB = nullptr, *cC = nullptr, *cD = nullptr, *cvh = nullptr;
-    double *cEH = nullptr, *cEH0 = nullptr, *cT = nullptr, *cPW0 = nullptr, *cHr = nullptr, *cscreen = nullptr;
+    double *cEH = nullptr, *cEH0 = nullptr, *cT = nullptr, *cPW0 = nullptr, *cHr = nullptr, *cscreen = nullptr, *cXw = nullptr;
     double *cUexc = nullptr, *cGE = nullptr, *cQ = nullptr, *dwb = nullptr, *cEk = nullptr;
     bool cEk_valid = false;
     double *snap = nullptr;  // [nk] complex, k-resolved snapshot element
@@ -1431,31 +1589,40 @@ CoulArgs coul_args(edus_ctx *c, const double *Y) {
     a.w = 1.0 / (double)c->nk;
     a.cxm = c->cxm; a.sxm = c->sxm; a.cyn = c->cyn; a.syn = c->syn; a.S = c->cS; a.F = c->cF; a.X = c->cX; a.X0 = c->cX0;
     a.A = c->cA; a.B = c->cB; a.C = c->cC; a.D = c->cD; a.vh = c->cvh; a.EH = c->cEH; a.EH0 = c->cEH0; a.Tt = c->cT;
+    a.Xw = c->cXw;
     return a;
 }
 
 // Mean-field coefficients of state Y: reduction, X_mn, column sums (4 launches).
 int launch_coulomb(edus_ctx *c, const double *Y) {
     CoulArgs a = coul_args(c, Y);
-    const int t1 = c->N1 * c->ncut * c->np, t2 = c->ncut * c->ncut * c->np;
     const int tx = std::max(c->ncut * c->ncut * c->nu, c->nb), t3 = c->N2 * c->ncut * c->nu;
-    hipLaunchKernelGGL(k_coul_s1, dim3((t1 + 255) / 256), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_coul_s2, dim3((t2 + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_s1, dim3(c->N1), dim3(kCoulS1Threads), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_s2, dim3(c->ncut * c->ncut), dim3(kCoulThreads), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_x, dim3((tx + 255) / 256), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_s3, dim3((t3 + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_w, dim3((c->N1 + kCoulRows - 1) / kCoulRows, (c->nu * c->N2 + 255) / 256), dim3(256), 0, c->stream, a,
+                       c->N1);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-template <int MODE>
-int launch_stage(edus_ctx *c, const StageArgs &a) {
+ConstArgs const_args(const edus_ctx *c) {
     ConstArgs ca;
+    std::memset(&ca, 0, sizeof ca);
     ca.H = c->H; ca.D = c->D; ca.U = c->U; ca.T = c->T; ca.nbr = c->nbr;
     ca.Hf = c->Hf; ca.Df = c->Df; ca.Uf = c->Uf; ca.eidx = c->eidx;
     ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
     ca.diss = c->diss; ca.dherm = c->dherm; ca.wx2 = c->wx2;
     ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
     ca.cxm = c->cxm; ca.sxm = c->sxm; ca.Tt = c->cT; ca.PW0 = c->cPW0; ca.EH = c->cEH; ca.screen = c->cscreen; ca.Hr = c->cHr;
+    ca.Xw = c->cXw;
+    return ca;
+}
+
+template <int MODE>
+int launch_stage(edus_ctx *c, const StageArgs &a) {
+    const ConstArgs ca = const_args(c);
     if (c->coul) {
         int rc = launch_coulomb(c, a.Y);
         if (rc) return rc;
@@ -1472,7 +1639,7 @@ int launch_stage(edus_ctx *c, const StageArgs &a) {
     } else {
         auto go = [&](auto tag) {
             constexpr int NBT = decltype(tag)::value;
-            const size_t lds = (size_t)wave_mats(c->diss) * NBT * (NBT + 1) * sizeof(cd);
+            const size_t lds = (size_t)wave_mats(c->diss) * NBT * WaveGeom<NBT>::LD * sizeof(cd);
             if (c->precision == EDUS_PRECISION_MIXED)
                 hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float>), dim3(c->nk), dim3(WaveGeom<NBT>::TPK), lds / 2, c->stream, ca, a);
             else
@@ -1692,7 +1859,7 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
         if (hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "upload nbr"));
     }
-    c->obs_blocks = std::max(1, std::min(2048, (p->nk + kObsWaves - 1) / kObsWaves));
+    c->obs_blocks = std::max(1, std::min(kObsMaxBlocks, p->nk));
     if ((rc = dalloc(c, &c->part, (size_t)c->obs_blocks * (nb + 17))) || (rc = dalloc(c, &c->obs, nb + 17))) return fail(rc);
     {
         std::vector<double> st;
@@ -1712,6 +1879,7 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
             return fail(set_err(EDUS_ERR_UNSUPPORTED, "the Coulomb term needs a 1 x n1 x n2 grid (sample in yz)"));
         if (q->ncut < 1 || !q->A || !q->B || !q->C || !q->D || !q->screen || !q->v_hartree)
             return fail(set_err(EDUS_ERR_INVALID, "incomplete Coulomb tables"));
+        if (q->ncut * nb * nb > 512) return fail(set_err(EDUS_ERR_UNSUPPORTED, "Ncut * nb^2 above 512"));
         c->coul = 1; c->N1 = p->nk_dims[1]; c->N2 = p->nk_dims[2]; c->ncut = q->ncut; c->nu = nb * (nb + 1) / 2;
         const int nmm = c->ncut * c->ncut, nb2 = nb * nb;
         // trig tables cos/sin(2 pi m k) with k = -0.5 + i/N (integrationWeight.h:139-175)
@@ -1744,7 +1912,8 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
             (rc = dalloc(c, &c->cX, (size_t)4 * nmm * nb2 * 2)) || (rc = dalloc(c, &c->cT, (size_t)2 * c->N2 * c->ncut * c->nu * 2)) ||
             (rc = dalloc(c, &c->cEH, nb)) || (rc = dalloc(c, &c->cPW0, se)) || (rc = dalloc(c, &c->cHr, se)) ||
             (rc = dalloc(c, &c->cUexc, (size_t)p->nk * nb2 * 2)) || (rc = dalloc(c, &c->cGE, (size_t)p->nk * 3 * c->np)) ||
-            (rc = dalloc(c, &c->cQ, (size_t)p->nk * 3 * nb2 * 2)) || (rc = dalloc(c, &c->cEk, (size_t)p->nk * nb)))
+            (rc = dalloc(c, &c->cQ, (size_t)p->nk * 3 * nb2 * 2)) || (rc = dalloc(c, &c->cEk, (size_t)p->nk * nb)) ||
+            (rc = dalloc(c, &c->cXw, (size_t)2 * c->nu * c->nkp)))
             return fail(rc);
         if ((rc = up(&c->cX0, std::vector<double>((size_t)4 * nmm * nb2 * 2, 0.0))) || (rc = up(&c->cEH0, std::vector<double>(nb, 0.0))) ||
             (rc = upc(&c->dwb, p->nbr_wb, (size_t)std::max(p->n_nbr, 1) * 3)))
@@ -1774,7 +1943,7 @@ int edus_destroy(edus_ctx *c) {
     if (c->tab) (void)hipFree(c->tab);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }
@@ -1789,22 +1958,31 @@ static int finish_step(edus_ctx *c, int launches) {
     return 0;
 }
 
-// One RK4 step of size dt from the four table rows at `tab` (RungeKutta.cpp:11-150).
-static int rk4_step(edus_ctx *c, const double *tab, int rs, double dt) {
-    const double dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
+// Arguments of RK4 stage st (0..3) of size dt reading the field row `row`
+// (RungeKutta.cpp:11-150): y_n in P0, stages through YA/YB, k-sum in ACC.
+static StageArgs rk4_stage_args(const edus_ctx *c, int st, const double *row, double dt) {
     StageArgs a;
     std::memset(&a, 0, sizeof a);
     a.dt = dt;
-    a.tab = tab;
-    a.Y = c->P0; a.P0 = c->P0; a.ACC = c->ACC; a.OUT = c->YA; a.ca = dt6; a.cb = dt2;
+    a.tab = row;
+    a.P0 = c->P0; a.ACC = c->ACC;
+    const double dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
+    switch (st) {
+        case 0: a.Y = c->P0; a.OUT = c->YA; a.ca = dt6; a.cb = dt2; break;
+        case 1: a.Y = c->YA; a.OUT = c->YB; a.ca = dt3; a.cb = dt2; break;
+        case 2: a.Y = c->YB; a.OUT = c->YA; a.ca = dt3; a.cb = dt; break;
+        default: a.Y = c->YA; a.OUT = c->P0; a.ca = dt6; break;
+    }
+    return a;
+}
+
+// One RK4 step of size dt from the four table rows at `tab`.
+static int rk4_step(edus_ctx *c, const double *tab, int rs, double dt) {
     int rc;
-    if ((rc = launch_stage<RK1>(c, a))) return rc;
-    a.tab += rs; a.Y = c->YA; a.OUT = c->YB; a.ca = dt3; a.cb = dt2;
-    if ((rc = launch_stage<RK2>(c, a))) return rc;
-    a.tab += rs; a.Y = c->YB; a.OUT = c->YA; a.ca = dt3; a.cb = dt;
-    if ((rc = launch_stage<RK3>(c, a))) return rc;
-    a.tab += rs; a.Y = c->YA; a.OUT = c->P0; a.ca = dt6;
-    return launch_stage<RK4S>(c, a);
+    if ((rc = launch_stage<RK1>(c, rk4_stage_args(c, 0, tab, dt)))) return rc;
+    if ((rc = launch_stage<RK2>(c, rk4_stage_args(c, 1, tab + rs, dt)))) return rc;
+    if ((rc = launch_stage<RK3>(c, rk4_stage_args(c, 2, tab + 2 * rs, dt)))) return rc;
+    return launch_stage<RK4S>(c, rk4_stage_args(c, 3, tab + 3 * rs, dt));
 }
 
 // Fields [nsteps][3][6] -> table rows [nsteps][4] (stage 3 reuses the midpoint field).
@@ -1982,10 +2160,10 @@ int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *t
     o.nk = c->nk; o.nb = c->nb; o.nch = c->nch; o.ncc = c->ncc;
     o.sk = c->soa ? 1 : c->np;
     o.se = c->soa ? c->nkp : 1;
-    hipLaunchKernelGGL(k_observables, dim3(c->obs_blocks), dim3(64 * kObsWaves), 0, c->stream, o);
+    hipLaunchKernelGGL(k_observables, dim3(c->obs_blocks), dim3(64), (size_t)c->nb * (2 * c->nb + 1) * sizeof(cd), c->stream, o);
     HIP_TRY(hipGetLastError());
     const int nv = c->nb + 15;
-    hipLaunchKernelGGL(k_obs_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->part, c->obs_blocks, nv, c->obs);
+    hipLaunchKernelGGL(k_obs_final, dim3(nv + 2), dim3(256), 0, c->stream, (const double *)c->part, c->obs_blocks, nv, c->obs);
     HIP_TRY(hipGetLastError());
     std::vector<double> h(nv + 2);
     HIP_TRY(hipMemcpyAsync(h.data(), c->obs, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2059,6 +2237,222 @@ int edus_last_step_time(edus_ctx *c, double *ms, int32_t *nlaunch) {
     if (ms) *ms = c->last_ms;
     if (nlaunch) *nlaunch = c->last_launches;
     return 0;
+}
+
+// ---------------------------------------------------------------- batched instances
+struct edus_batch {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<edus_ctx *> inst;
+    struct Cls {
+        bool soa = true;
+        int nb_small = 0, nbt = 0, precision = EDUS_PRECISION_FP64, diss = 0;
+        std::vector<int> members;
+        int2 *blk = nullptr;
+        int nblocks = 0;
+    };
+    std::vector<Cls> cls;
+    BatchEntry *entries = nullptr;
+    double *tab = nullptr;  // [step][stage][R_tot]; instance i's row at roff[i]
+    size_t tab_steps = 0;
+    int R_tot = 0;
+    std::vector<int> roff;
+    std::vector<void *> allocs;
+    double last_ms = 0.0;
+    int last_launches = 0;
+};
+
+extern "C++" {
+namespace {
+
+int batch_write_entries(edus_batch *b) {
+    std::vector<BatchEntry> e(b->inst.size());
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        const edus_ctx *c = b->inst[i];
+        e[i].c = const_args(c);
+        for (int st = 0; st < 4; st++) e[i].a[st] = rk4_stage_args(c, st, b->tab + (size_t)st * b->R_tot + b->roff[i], c->dt);
+    }
+    HIP_TRY(hipMemcpyAsync(b->entries, e.data(), e.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int batch_table_capacity(edus_batch *b, size_t nsteps) {
+    if (nsteps <= b->tab_steps) return 0;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->tab) HIP_TRY(hipFree(b->tab));
+    b->tab = nullptr;
+    const size_t cap = std::max<size_t>(nsteps, 16);
+    HIP_TRY(hipMalloc(&b->tab, cap * 4 * (size_t)b->R_tot * sizeof(double)));
+    b->tab_steps = cap;
+    return batch_write_entries(b);
+}
+
+template <int MODE>
+int batch_launch(edus_batch *b, const edus_batch::Cls &k, size_t tab_step) {
+    const dim3 grid(k.nblocks);
+    if (k.soa) {
+        switch (k.nb_small) {
+            case 1: hipLaunchKernelGGL((k_stage_small_batch<1, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+            case 2: hipLaunchKernelGGL((k_stage_small_batch<2, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+            case 3: hipLaunchKernelGGL((k_stage_small_batch<3, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+            default: hipLaunchKernelGGL((k_stage_small_batch<4, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+        }
+    } else {
+        auto go = [&](auto tag) {
+            constexpr int NBT = decltype(tag)::value;
+            const size_t lds = (size_t)wave_mats(k.diss) * NBT * WaveGeom<NBT>::LD * sizeof(cd);
+            if (k.precision == EDUS_PRECISION_MIXED)
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, float>), grid, dim3(WaveGeom<NBT>::TPK), lds / 2, b->stream, b->entries, k.blk, tab_step);
+            else
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, double>), grid, dim3(WaveGeom<NBT>::TPK), lds, b->stream, b->entries, k.blk, tab_step);
+        };
+        if (k.nbt == 8) go(std::integral_constant<int, 8>{});
+        else if (k.nbt == 16) go(std::integral_constant<int, 16>{});
+        else go(std::integral_constant<int, 32>{});
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int edus_batch_destroy(edus_batch *b) {
+    if (!b) return 0;
+    (void)hipSetDevice(b->dev);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (edus_ctx *c : b->inst) edus_destroy(c);
+    for (auto &k : b->cls)
+        if (k.blk) (void)hipFree(k.blk);
+    for (void *p : b->allocs) (void)hipFree(p);
+    if (b->tab) (void)hipFree(b->tab);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch **out) {
+    if (!problems || !out || n < 1) return set_err(EDUS_ERR_INVALID, "null argument or empty batch");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!problems[i]) return set_err(EDUS_ERR_INVALID, "null problem in batch");
+        if (problems[i]->stepper != EDUS_STEPPER_RK4) return set_err(EDUS_ERR_UNSUPPORTED, "batches run RK4 problems");
+        if (problems[i]->coulomb) return set_err(EDUS_ERR_UNSUPPORTED, "batches do not carry the Coulomb term");
+    }
+    auto *b = new edus_batch();
+    auto fail = [&](int rc) { edus_batch_destroy(b); return rc; };
+    if (hipGetDevice(&b->dev) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "hipGetDevice failed"));
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "stream create failed"));
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "event create failed"));
+    for (int i = 0; i < n; i++) {
+        edus_ctx *c = nullptr;
+        const int rc = edus_create(problems[i], &c);
+        if (rc) return fail(rc);
+        // every instance works on the batch's stream
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        c->stream = b->stream;
+        c->own_stream = false;
+        b->inst.push_back(c);
+        b->roff.push_back(b->R_tot);
+        b->R_tot += 7 + c->n_nbr;
+    }
+    // kernel classes: small kernels by band count, wave kernels by tile and precision
+    for (int i = 0; i < n; i++) {
+        const edus_ctx *c = b->inst[i];
+        const int nbt = c->soa ? 0 : (c->nb <= 8 ? 8 : c->nb <= 16 ? 16 : 32);
+        edus_batch::Cls *hit = nullptr;
+        for (auto &k : b->cls)
+            if (k.soa == c->soa && k.nb_small == (c->soa ? c->nb : 0) && k.nbt == nbt && k.precision == c->precision) hit = &k;
+        if (!hit) {
+            b->cls.emplace_back();
+            hit = &b->cls.back();
+            hit->soa = c->soa; hit->nb_small = c->soa ? c->nb : 0; hit->nbt = nbt; hit->precision = c->precision;
+        }
+        hit->members.push_back(i);
+        hit->diss |= c->diss;
+    }
+    for (auto &k : b->cls) {
+        std::vector<int2> map;
+        for (int i : k.members) {
+            const edus_ctx *c = b->inst[i];
+            const int nblk = c->soa ? (c->nk + 255) / 256 : c->nk;
+            for (int q = 0; q < nblk; q++) map.push_back(make_int2(i, q));
+        }
+        k.nblocks = (int)map.size();
+        if (hipMalloc(&k.blk, map.size() * sizeof(int2)) != hipSuccess ||
+            hipMemcpy(k.blk, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(EDUS_ERR_NOMEM, "batch block table"));
+    }
+    void *ent = nullptr;
+    if (hipMalloc(&ent, (size_t)n * sizeof(BatchEntry)) != hipSuccess) return fail(set_err(EDUS_ERR_NOMEM, "batch entries"));
+    b->allocs.push_back(ent);
+    b->entries = static_cast<BatchEntry *>(ent);
+    int rc = batch_table_capacity(b, 16);
+    if (rc) return fail(rc);
+    *out = b;
+    g_err.clear();
+    return 0;
+}
+
+int edus_batch_size(const edus_batch *b) { return b ? (int)b->inst.size() : 0; }
+
+int edus_batch_step_rk4(edus_batch *b, int32_t nsteps, const double *const *fields, const uint8_t *const *grad_on) {
+    if (!b) return set_err(EDUS_ERR_INVALID, "null batch");
+    HIP_TRY(hipSetDevice(b->dev));
+    const int n = (int)b->inst.size();
+    if (nsteps < 0 || (nsteps > 0 && (!fields || !grad_on))) return set_err(EDUS_ERR_INVALID, "bad step arguments");
+    if (nsteps == 0) return 0;
+    for (int i = 0; i < n; i++)
+        if (!fields[i] || !grad_on[i]) return set_err(EDUS_ERR_INVALID, "null fields of a batch member");
+    int rc = batch_table_capacity(b, (size_t)nsteps);
+    if (rc) return rc;
+    std::vector<double> rows((size_t)nsteps * 4 * b->R_tot), one;
+    for (int i = 0; i < n; i++) {
+        const edus_ctx *c = b->inst[i];
+        const int rs = 7 + c->n_nbr;
+        rk4_rows(c, nsteps, fields[i], grad_on[i], one);
+        for (size_t r = 0; r < (size_t)nsteps * 4; r++)
+            std::copy(one.begin() + r * rs, one.begin() + (r + 1) * rs, rows.begin() + r * b->R_tot + b->roff[i]);
+    }
+    HIP_TRY(hipMemcpyAsync(b->tab, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    int launches = 0;
+    for (int s = 0; s < nsteps; s++) {
+        const size_t ts = (size_t)s * 4 * b->R_tot;
+        for (const auto &k : b->cls) if ((rc = batch_launch<RK1>(b, k, ts))) return rc;
+        for (const auto &k : b->cls) if ((rc = batch_launch<RK2>(b, k, ts))) return rc;
+        for (const auto &k : b->cls) if ((rc = batch_launch<RK3>(b, k, ts))) return rc;
+        for (const auto &k : b->cls) if ((rc = batch_launch<RK4S>(b, k, ts))) return rc;
+        launches += 4 * (int)b->cls.size();
+    }
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_launches = launches;
+    for (edus_ctx *c : b->inst) c->nstep += nsteps;
+    return 0;
+}
+
+int edus_batch_last_step_time(const edus_batch *b, double *ms, int32_t *nlaunch) {
+    if (!b) return set_err(EDUS_ERR_INVALID, "null batch");
+    if (ms) *ms = b->last_ms;
+    if (nlaunch) *nlaunch = b->last_launches;
+    return 0;
+}
+
+edus_ctx *edus_batch_member(edus_batch *b, int32_t i) {
+    if (!b || i < 0 || i >= (int)b->inst.size()) {
+        set_err(EDUS_ERR_INVALID, "batch member index out of range");
+        return nullptr;
+    }
+    return b->inst[i];
 }
 
 int edus_coulomb_coefficients(const edus_coulomb_gen *g, double *Ao, double *Bo, double *Co, double *Do, double *screen, double *vh) {

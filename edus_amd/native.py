@@ -102,6 +102,13 @@ hip.edus_set_state.argtypes = [C.c_void_p, _dp]
 hip.edus_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
 hip.edus_wannier_hk.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
 hip.edus_last_error.restype = C.c_char_p
+hip.edus_batch_create.argtypes = [C.POINTER(C.POINTER(Problem)), C.c_int32, C.POINTER(C.c_void_p)]
+hip.edus_batch_destroy.argtypes = [C.c_void_p]
+hip.edus_batch_size.argtypes = [C.c_void_p]
+hip.edus_batch_step_rk4.argtypes = [C.c_void_p, C.c_int32, C.POINTER(_dp), C.POINTER(C.POINTER(C.c_uint8))]
+hip.edus_batch_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
+hip.edus_batch_member.argtypes = [C.c_void_p, C.c_int32]
+hip.edus_batch_member.restype = C.c_void_p
 
 # ---- host
 host.edus_setup_from_input.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]
@@ -232,6 +239,53 @@ class Setup:
         return n
 
 
+class Batch:
+    """Independent RK4 problems advanced together (one launch per stage and kernel
+    class); members are Engine views for observables and state access."""
+
+    def __init__(self, problems, precision=None):
+        if precision is not None:
+            for p in problems:
+                p.precision = precision
+        arr = (C.POINTER(Problem) * len(problems))(*[C.pointer(p) for p in problems])
+        h = C.c_void_p()
+        _check_hip(hip.edus_batch_create(arr, len(problems), C.byref(h)), "edus_batch_create")
+        self._h = h
+        self.members = [Engine._member(hip.edus_batch_member(h, i), p) for i, p in enumerate(problems)]
+        self.nk = [p.nk for p in problems]
+
+    def step(self, fields, grad_on):
+        """fields[i]: [nsteps][3][6], grad_on[i]: [nsteps], as Engine.step for RK4."""
+        n = len(self.members)
+        fs = [np.ascontiguousarray(f, dtype=np.float64) for f in fields]
+        gs = [np.ascontiguousarray(g, dtype=np.uint8) for g in grad_on]
+        nsteps = len(gs[0])
+        if any(len(g) != nsteps for g in gs):
+            raise ValueError("every member steps the same number of steps")
+        fp = (_dp * n)(*[_dptr(f) for f in fs])
+        gp = (C.POINTER(C.c_uint8) * n)(*[g.ctypes.data_as(C.POINTER(C.c_uint8)) for g in gs])
+        _check_hip(hip.edus_batch_step_rk4(self._h, nsteps, fp, gp), "edus_batch_step_rk4")
+
+    def last_step_time(self):
+        ms = np.zeros(1)
+        n = C.c_int32()
+        _check_hip(hip.edus_batch_last_step_time(self._h, _dptr(ms), C.byref(n)), "edus_batch_last_step_time")
+        return float(ms[0]), n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            hip.edus_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Engine:
     """One GPU context; the arrays passed to create() are copied to HBM."""
 
@@ -243,10 +297,21 @@ class Engine:
         h = C.c_void_p()
         _check_hip(hip.edus_create(C.byref(problem), C.byref(h)), "edus_create")
         self._h = h
+        self._owned = True
+
+    @classmethod
+    def _member(cls, handle, problem):
+        """A batch member's context (owned by the batch)."""
+        e = cls.__new__(cls)
+        e.stepper, e.nk, e.nb = problem.stepper, problem.nk, problem.nb
+        e._h = C.c_void_p(handle)
+        e._owned = False
+        return e
 
     def close(self):
         if getattr(self, "_h", None):
-            hip.edus_destroy(self._h)
+            if self._owned:
+                hip.edus_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -83,7 +83,7 @@ def hbn_tb(directory, nk=60, nsteps=8000, dt=0.05, stepper="RK4", order=4, e_v_a
 
 def wannier_synthetic(directory, nw=16, rmax=10, nk=200, nsteps=4000, dt=0.05, stepper="RK4", order=4,
                       e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0, t2_fs=0.0, it_resolution=10,
-                      photon_ev=0.6, fwhm_fs=15.0):
+                      photon_ev=0.6, fwhm_fs=15.0, pol=(0, 0, 1), phase=0.0):
     """Config 1 / 4 family: seeded random Hermitian H(R), r(R) on a hexagonal
     lattice (a = 2.5 A), `nvalence` filled bands, pump along z."""
     from . import native
@@ -111,6 +111,72 @@ def wannier_synthetic(directory, nw=16, rmax=10, nk=200, nsteps=4000, dt=0.05, s
             f"bands{{\nvalence {nvalence}\nconduction {nw - nvalence}\n}}\n\n"
             f"nkPT{{\n1 {nk} {nk}\n}}\n\n"
             f"{dec}\n"
+            + _pump(photon_ev * EV_AU, e_v_ang, fwhm_fs, pol, t_peak, phase)
+            + "\n" + PROBE_OFF + "\n"
+            f"observables{{\nit_resolution {it_resolution}\nCurrent\nTAbs\n}}\n")
+    return write(os.path.join(directory, "input.txt"), text)
+
+
+def boron_nitride(directory, nk=48, nsteps=3000, dt=0.05, e_v_ang=0.3, photon_ev=0.6, fwhm_fs=15.0, pol=(0, 0, 1),
+                  phase=0.0, it_resolution=10):
+    """The reference's 2-band BoronNitride tight-binding model (a = 2.5 A), RK4,
+    gaussian pump."""
+    t_peak = t_fin_for(nsteps, dt) / 2.0 - 0.105 / TIME_AU_FS
+    text = ("NEWINPUT\n\n"
+            "tdse{\ntightbinding BoronNitride\n"
+            f"dt {dt!r} au\nt_fin {t_fin_for(nsteps, dt)!r} au\nnum_pump_pulses 1\n}}\n\n"
+            f"nkPT{{\n1 {nk} {nk}\n}}\n\n"
+            + _pump(photon_ev * EV_AU, e_v_ang, fwhm_fs, pol, t_peak, phase)
+            + "\n" + PROBE_OFF + "\n"
+            f"observables{{\nit_resolution {it_resolution}\nCurrent\nTAbs\n}}\n")
+    return write(os.path.join(directory, "input.txt"), text)
+
+
+def batch_scan(directory, n=256, seed=3003, nsteps=3000, dt=0.05):
+    """Config 3: n independent instances.  Grids from {48, 96, 144, 192}^2, band
+    counts from {2 (BoronNitride), 4, 8 (seeded Wannier, seed + i)}, and per
+    instance a gaussian pump with peak field U[0.05, 0.6] V/A, CEP U[0, 2 pi),
+    polarisation angle U[0, pi/3) in the yz plane, photon energy U[0.4, 1.0] eV
+    and intensity FWHM U[10, 20] fs; RK4.  Returns the input paths."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    paths = []
+    for i in range(n):
+        nk = int(rng.choice([48, 96, 144, 192]))
+        nb = int(rng.choice([2, 4, 8]))
+        e0 = float(rng.uniform(0.05, 0.6))
+        cep = float(rng.uniform(0.0, 2.0 * math.pi))
+        ang = float(rng.uniform(0.0, math.pi / 3.0))
+        w = float(rng.uniform(0.4, 1.0))
+        fwhm = float(rng.uniform(10.0, 20.0))
+        pol = (0.0, math.cos(ang), math.sin(ang))
+        d = os.path.join(directory, f"inst{i:03d}")
+        if nb == 2:
+            paths.append(boron_nitride(d, nk=nk, nsteps=nsteps, dt=dt, e_v_ang=e0, photon_ev=w, fwhm_fs=fwhm, pol=pol,
+                                       phase=cep))
+        else:
+            paths.append(wannier_synthetic(d, nw=nb, rmax=10, nk=nk, nsteps=nsteps, dt=dt, e_v_ang=e0, nvalence=nb // 2,
+                                           seed=seed + i, photon_ev=w, fwhm_fs=fwhm, pol=pol, phase=cep))
+    return paths
+
+
+def hbn_coulomb(directory, coef_dir, label, nk=300, nsteps=6000, dt=0.03, order=4, e_v_ang=0.3, it_resolution=10,
+                photon_ev=0.6, fwhm_fs=15.0):
+    """Config 2 family: the 3-band Haldane_CoreBN_Nedge hBN model of the
+    reference's Coulomb example with the Hartree-Fock mean field (Ncut 20,
+    Rytova-Keldysh), coefficients read from `coef_dir` (files named with
+    `label`), Adams-Bashforth `order`, gaussian pump along z."""
+    os.makedirs(os.path.join(directory, "Output"), exist_ok=True)
+    link = os.path.join(directory, "Output", "Coulomb")
+    if not os.path.exists(link):
+        os.symlink(os.path.abspath(coef_dir), link)
+    t_peak = t_fin_for(nsteps, dt) / 2.0 - 0.105 / TIME_AU_FS
+    text = ("NEWINPUT\n\n"
+            "tdse{\ntightbinding Haldane_CoreBN_Nedge gap=  7.25 t1=  -2.3 t2= 0 phi0= 0.0\n"
+            f"dt {dt!r} au\nt_fin {t_fin_for(nsteps, dt)!r} au\nAdams_Bashforth {order}\nnum_pump_pulses 1\n}}\n\n"
+            "Coulomb{\nqTF 0.01\nNcut 20\nepsilon_static 1.0\nRead_Coulomb_from_files\n"
+            f"labelInput {label}\n\nRytova_Keldysh\n}}\n\n"
+            f"nkPT{{\n1 {nk} {nk}\n}}\n\n"
             + _pump(photon_ev * EV_AU, e_v_ang, fwhm_fs, (0, 0, 1), t_peak)
             + "\n" + PROBE_OFF + "\n"
             f"observables{{\nit_resolution {it_resolution}\nCurrent\nTAbs\n}}\n")

@@ -175,6 +175,26 @@ int edus_set_state(edus_ctx *ctx, const double *rho);
  * with HIP events on the context's stream; *nlaunch receives the kernel count. */
 int edus_last_step_time(edus_ctx *ctx, double *ms, int32_t *nlaunch);
 
+/*
+ * Batches: n independent RK4 problems (different grids, band counts, models and
+ * pulses; no Coulomb term) advanced together.  No reference counterpart: the
+ * reference runs one problem per process.  This is for scans over laser
+ * parameters and materials, where one small problem cannot fill the GPU.  Each
+ * RK4 stage is one launch per kernel class present (small kernels by band
+ * count, wave kernels by tile and precision).  A member steps bit-identically
+ * to the same problem in its own context.
+ */
+typedef struct edus_batch edus_batch;
+int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch **out);
+int edus_batch_destroy(edus_batch *b);
+int edus_batch_size(const edus_batch *b);
+/* fields[i] and grad_on[i]: member i's arrays, laid out as for edus_step_rk4. */
+int edus_batch_step_rk4(edus_batch *b, int32_t nsteps, const double *const *fields, const uint8_t *const *grad_on);
+int edus_batch_last_step_time(const edus_batch *b, double *ms, int32_t *nlaunch);
+/* Member i's context, for edus_observables, edus_get_state, edus_set_state and
+ * edus_rhs; it is owned by the batch (do not destroy it).  NULL if out of range. */
+edus_ctx *edus_batch_member(edus_batch *b, int32_t i);
+
 /* Parameters of the Coulomb coefficient generator (Source_Main/Fourier_Coulomb.cpp:95-500
  * with headers/Coulomb.h:60-267 and integrationWeight.h:5-76), "011" orientation:
  * the sample lies in the plane of crystal directions 1 and 2, q = b1 q1 + b2 q2. */

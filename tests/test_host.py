@@ -82,3 +82,12 @@ def test_print_population_options(tmp_path):
         1, 0.0, 100.0 * 0.02418884326505, 20.0 * 0.02418884326505)
     i = N.Setup(_write(tmp_path, BASE.replace(line, ""))).info
     assert i["print_flags"] == 0
+
+
+def test_batch_rejects_multistep_problems(tmp_path):
+    """Batches run RK4 problems; the check happens before any device call."""
+    import helpers as Hh
+    p = Hh.coulomb_input(str(tmp_path), nk=6, stepper="Adams_Bashforth 3\n")
+    s = N.Setup(p)
+    with pytest.raises(RuntimeError, match="RK4|Coulomb"):
+        N.Batch([s.problem()])
