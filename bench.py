@@ -138,9 +138,14 @@ def main():
     rk4 = info["stepper"] == N.STEPPER_RK4
     transfers = 4 if rk4 else info["order"] + 2 + (1 if info["ncut"] else 0)
     bytes_launch = (transfers + n_op) * S * nk
-    # (mixed form: the four operator matrices and U at half width)
+    # The engine reads a dipole component only when some field of the call has
+    # that component (a zero field multiplies it by an exact zero): H plus the
+    # nd active components.  (Mixed form: operators and U at half width.)
+    ft = f[args.warmup:]
+    nd = sum(1 for x in range(3) if np.any(ft[..., x] != 0) or np.any(ft[..., 3 + x] != 0))
     opw = 1 if args.precision == "fp64" else 2
-    stored_launch = transfers * (S // 2) * nk + 4 * (S // 2 // opw) * nk + (S // opw * nk if info["n_diss"] > 0 else 0)
+    stored_launch = (transfers * (S // 2) * nk + (1 + nd) * (S // 2 // opw) * nk
+                     + (S // opw * nk if info["n_diss"] > 0 else 0))
     stepper = "RK4" if rk4 else f"AB{info['order']}"
     avg_launch_s = dev_ms * 1e-3 / nlaunch
     achieved = bytes_launch / avg_launch_s / 1e9
@@ -156,7 +161,8 @@ def main():
                       + (" (RK4 stages)" if rk4 else f" ({stepper} steps)")
                       + ("" if args.precision == "fp64" else ", FP32 operators"),
             "bytes_per_launch": bytes_launch,
-            "stored_bytes_per_launch": stored_launch, "achieved_stored": round(stored, 2),
+            "stored_bytes_per_launch": stored_launch, "dipole_components_read": nd,
+            "achieved_stored": round(stored, 2),
             "frac_stored": round(stored / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_launch_s * 1e6, 3)}
 
     cpu = None

@@ -86,7 +86,9 @@ struct StageArgs {
     double *OUT;         // stage output
     double *HIST;        // AB/Taylor history slots
     const double *tab;
-    int slot_out, order, pad0, pad1;
+    int slot_out, order;
+    int dmask;           // bit x: some field of the call has a nonzero x component (else D_x is not read)
+    int pad1;
     int slots[EDUS_MAX_ORDER];
     double coef[EDUS_MAX_ORDER];
     double tinv[kMaxTinv];
@@ -104,6 +106,21 @@ __device__ __forceinline__ cd cadd(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}
 __device__ __forceinline__ cd csub(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
 __device__ __forceinline__ cd cconj(cd a) { return cd{a.x, -a.y}; }
 __device__ __forceinline__ cd cscale(cd a, double s) { return cd{a.x * s, a.y * s}; }
+
+// Butterfly reductions over the 64 lanes of a wave (every lane gets the result;
+// the order of the additions is fixed).
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
 
 // Single-precision complex for the mixed-precision stage (operators and LDS products).
 struct cf {
@@ -177,7 +194,9 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_
 }
 
 // ================================================================ nb <= 4: thread per k
-template <int NB, int MODE>
+// DISS: the context has dephasing terms (c.diss); without them the U sandwich
+// and its registers are compiled out.
+template <int NB, int MODE, bool DISS>
 __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs &a, const int k) {
     constexpr int NP = NB * NB;
     if (k >= c.nk) return;
@@ -256,13 +275,14 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
             if (i == j) {
                 h = mk(ldnt(c.H + i * nkp + k), 0.0);
 #pragma unroll
-                for (int x = 0; x < 3; x++) dx[x] = mk(ldnt(c.D + (x * NP + i) * nkp + k), 0.0);
+                for (int x = 0; x < 3; x++) dx[x] = (a.dmask >> x & 1) ? mk(ldnt(c.D + (x * NP + i) * nkp + k), 0.0) : mk(0.0, 0.0);
             } else {
                 const int p = pair_index(i, j, NB);
                 h = mk(ldnt(c.H + (NB + 2 * p) * nkp + k), ldnt(c.H + (NB + 2 * p + 1) * nkp + k));
 #pragma unroll
                 for (int x = 0; x < 3; x++)
-                    dx[x] = mk(ldnt(c.D + (x * NP + NB + 2 * p) * nkp + k), ldnt(c.D + (x * NP + NB + 2 * p + 1) * nkp + k));
+                    dx[x] = (a.dmask >> x & 1) ? mk(ldnt(c.D + (x * NP + NB + 2 * p) * nkp + k), ldnt(c.D + (x * NP + NB + 2 * p + 1) * nkp + k))
+                                               : mk(0.0, 0.0);
             }
             if (c.coul) h = Hren[i][j];
             cd mij = h, mji = cconj(h);
@@ -299,7 +319,7 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
         }
     // Dephasing in the eigenbasis (RungeKutta.h:219-254) as
     // Pv -= conj(U) (T o (conj(U) P U^T)) conj(U) plus the hole source.
-    if (c.diss) {
+    if constexpr (DISS) {
         cd U[NB][NB];
         const double *u = c.U + (size_t)k * NB * NB * 2;
 #pragma unroll
@@ -389,9 +409,9 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
     for (int e = 0; e < NP; e++) stage_update<MODE>(a, e * nkp + k, (size_t)NP * nkp, pv[e], y[e]);
 }
 
-template <int NB, int MODE>
+template <int NB, int MODE, bool DISS>
 __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
-    stage_small<NB, MODE>(c, a, xcd_block() * 256 + threadIdx.x);
+    stage_small<NB, MODE, DISS>(c, a, xcd_block() * 256 + threadIdx.x);
 }
 
 // ================================================================ Coulomb mean field
@@ -419,42 +439,26 @@ struct CoulArgs {
 
 constexpr int kCoulThreads = 256;
 
-// S_y[i1][n][e] = sum_i2 y_n(i2) P_e(i1, i2): one workgroup of 1024 threads per
-// row i1.  Thread t owns output (n, e) = t % 256 and the quarter t / 256 of the
-// columns; the row and the trig tables are read through L1 (every output of the
-// row shares them), and the four quarter sums are added in order.
-constexpr int kCoulS1Threads = 1024;
-__global__ __launch_bounds__(kCoulS1Threads) void k_coul_s1(CoulArgs a) {
-    __shared__ double red[2][kCoulS1Threads];
-    const int i1 = blockIdx.x, t = threadIdx.x, NP = a.NP, ncut = a.ncut;
-    const int quarter = t >> 8, nout = ncut * NP;
-    const int q0 = (a.N2 * quarter) / 4, q1 = (a.N2 * (quarter + 1)) / 4;
-    for (int o0 = 0; o0 < nout; o0 += 256) {  // at most two passes (ncut * NP <= 512)
-        const int o = o0 + (t & 255);
-        double sc = 0.0, ss = 0.0;
-        if (o < nout) {
-            const int n = o / NP, e = o % NP;
-            const double *row = a.Y + (size_t)e * a.nkp + (size_t)i1 * a.N2;
-            const double *cy = a.cyn + (size_t)n * a.N2, *sy = a.syn + (size_t)n * a.N2;
+// S_y[i1][n][e] = sum_i2 y_n(i2) P_e(i1, i2): one thread per output, the column
+// loop unrolled so that its loads overlap.  (Variants that split the sum over
+// lanes or staged the row in LDS measured slower at 300 x 300: the pass is
+// bound by the latency and address throughput of its ~100k short sums.)
+__global__ __launch_bounds__(256) void k_coul_s1(CoulArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = a.N1 * a.ncut * a.NP;
+    if (t >= total) return;
+    const int e = t % a.NP, n = (t / a.NP) % a.ncut, i1 = t / (a.NP * a.ncut);
+    const double *row = a.Y + (size_t)e * a.nkp + (size_t)i1 * a.N2;
+    const double *cy = a.cyn + (size_t)n * a.N2, *sy = a.syn + (size_t)n * a.N2;
+    double sc = 0.0, ss = 0.0;
 #pragma unroll 8
-            for (int i2 = q0; i2 < q1; i2++) {
-                const double p = row[i2];
-                sc += cy[i2] * p;
-                ss += sy[i2] * p;
-            }
-        }
-        __syncthreads();  // the previous pass has read red
-        red[0][t] = sc;
-        red[1][t] = ss;
-        __syncthreads();
-        if (t < 256 && o0 + t < nout) {
-            const int n = (o0 + t) / NP, e = (o0 + t) % NP;
-            const double c0 = ((red[0][t] + red[0][t + 256]) + red[0][t + 512]) + red[0][t + 768];
-            const double c1 = ((red[1][t] + red[1][t + 256]) + red[1][t + 512]) + red[1][t + 768];
-            a.S[(((size_t)0 * a.N1 + i1) * ncut + n) * NP + e] = c0;
-            a.S[(((size_t)1 * a.N1 + i1) * ncut + n) * NP + e] = c1;
-        }
+    for (int i2 = 0; i2 < a.N2; i2++) {
+        const double p = row[i2];
+        sc += cy[i2] * p;
+        ss += sy[i2] * p;
     }
+    a.S[(((size_t)0 * a.N1 + i1) * a.ncut + n) * a.NP + e] = sc;
+    a.S[(((size_t)1 * a.N1 + i1) * a.ncut + n) * a.NP + e] = ss;
 }
 
 // F_xy[mm][e] = w (1 + corr_mn) sum_i1 x_m(i1) S_y[i1][n][e]: one workgroup per
@@ -722,6 +726,9 @@ struct WaveGeom {
     static constexpr int RB = NBT / GRID;
     static constexpr int PER = (NBT * NBT + TPK - 1) / TPK;  // packed doubles per thread
     static constexpr int LD = NBT + 1;  // LDS row length in complex elements (one pad)
+    // k per workgroup: at nb <= 8 one k is little work for a wave, so four waves
+    // (four k) share a workgroup and the dispatcher handles a quarter as many
+    static constexpr int KPB = NBT == 8 ? 4 : 1;
     static constexpr int GATHER = PER <= 1 ? 12 : NBT <= 16 ? 6 : 4;  // stencil neighbours per load batch
 };
 
@@ -781,15 +788,24 @@ __host__ __device__ constexpr int wave_mats(int diss) { return diss ? 3 : 2; }
 // R is the precision of the operators and of the LDS products: double for the
 // drop-in, float for the mixed-precision mode (state, stencil and increments stay
 // FP64 in both).
+// LDS of one k of the wave kernel, in doubles.
+template <int NBT, typename R>
+__host__ __device__ constexpr int wave_lds_doubles(int diss) {
+    return wave_mats(diss) * NBT * WaveGeom<NBT>::LD * (int)sizeof(typename CplxOf<R>::T) / 8;
+}
+
+// k: the k this group of TPK threads serves (a valid index); lane: thread within
+// the group; smem: the group's LDS; store: false for a group past the end of the
+// grid, which computes its clamped k again but writes nothing (some stage modes
+// update their accumulator in place).
 template <int NBT, int MODE, typename R>
-__device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &a, const int k) {
+__device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &a, const int k, const int lane, double *smem,
+                                           const bool store) {
     using CT = typename CplxOf<R>::T;
     constexpr int TPK = WaveGeom<NBT>::TPK, RB = WaveGeom<NBT>::RB, PER = WaveGeom<NBT>::PER;
     constexpr int LD = WaveGeom<NBT>::LD, MAT = NBT * LD;
-    extern __shared__ double smem_d[];
-    const int lane = threadIdx.x;  // thread of the k's workgroup
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
-    CT *A = reinterpret_cast<CT *>(smem_d), *B = A + MAT, *C = c.diss ? B + MAT : B;
+    CT *A = reinterpret_cast<CT *>(smem), *B = A + MAT, *C = c.diss ? B + MAT : B;
     const R *Hop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.H : (const void *)c.Hf);
     const R *Dop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.D : (const void *)c.Df);
     const R *Uop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.U : (const void *)c.Uf);
@@ -817,7 +833,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         y[m] = a.Y[base + e];
         hv[m] = ldnt(Hop + base + e);
 #pragma unroll
-        for (int x = 0; x < 3; x++) dv[m][x] = ldnt(Dop + (size_t)x * c.nk * NP + base + e);
+        for (int x = 0; x < 3; x++) dv[m][x] = (a.dmask >> x & 1) ? ldnt(Dop + (size_t)x * c.nk * NP + base + e) : (R)0;
         if constexpr (MODE == RK2 || MODE == RK3) {
             pre0[m] = ldnt(a.ACC + base + e);
             pre1[m] = ldnt(a.P0 + base + e);
@@ -947,6 +963,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
             pv[m] = ec[m] ? pv[m] - (double)w.y + (double)hole.y : pv[m] - (double)w.x + (double)hole.x;
         }
     }
+    if (!store) return;  // after the last barrier
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const double kv = pv[m] + g[m];
@@ -966,8 +983,17 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
 }
 
 template <int NBT, int MODE, typename R>
-__global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave(ConstArgs c, StageArgs a) {
-    stage_wave<NBT, MODE, R>(c, a, xcd_block());  // xcd_block is a bijection of [0, nk): every workgroup owns a k
+__global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_stage_wave(ConstArgs c, StageArgs a) {
+    extern __shared__ double smem_d[];
+    constexpr int TPK = WaveGeom<NBT>::TPK;
+    if constexpr (WaveGeom<NBT>::KPB == 1) {
+        stage_wave<NBT, MODE, R>(c, a, xcd_block(), threadIdx.x, smem_d, true);  // xcd_block: a bijection of [0, nk)
+    } else {
+        const int sub = threadIdx.x / TPK;
+        const int kk = xcd_block() * WaveGeom<NBT>::KPB + sub;
+        stage_wave<NBT, MODE, R>(c, a, min(kk, c.nk - 1), threadIdx.x % TPK, smem_d + sub * wave_lds_doubles<NBT, R>(c.diss),
+                                 kk < c.nk);
+    }
 }
 
 // ---------------------------------------------------------------- batched instances
@@ -984,22 +1010,31 @@ struct BatchEntry {
 template <int MODE>
 __host__ __device__ constexpr int rk4_stage_of() { return MODE == RK1 ? 0 : MODE == RK2 ? 1 : MODE == RK3 ? 2 : 3; }
 
-template <int NB, int MODE>
+template <int NB, int MODE, bool DISS>
 __global__ __launch_bounds__(256) void k_stage_small_batch(const BatchEntry *E, const int2 *blk, size_t tab_step) {
     const int2 w = blk[xcd_block()];
     const BatchEntry &be = E[w.x];
     StageArgs a = be.a[rk4_stage_of<MODE>()];
     a.tab += tab_step;
-    stage_small<NB, MODE>(be.c, a, w.y * 256 + threadIdx.x);
+    stage_small<NB, MODE, DISS>(be.c, a, w.y * 256 + threadIdx.x);
 }
 
 template <int NBT, int MODE, typename R>
-__global__ __launch_bounds__(WaveGeom<NBT>::TPK) void k_stage_wave_batch(const BatchEntry *E, const int2 *blk, size_t tab_step) {
+__global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_stage_wave_batch(const BatchEntry *E, const int2 *blk,
+                                                                                         size_t tab_step) {
+    extern __shared__ double smem_d[];
+    constexpr int TPK = WaveGeom<NBT>::TPK;
     const int2 w = blk[xcd_block()];
     const BatchEntry &be = E[w.x];
     StageArgs a = be.a[rk4_stage_of<MODE>()];
     a.tab += tab_step;
-    stage_wave<NBT, MODE, R>(be.c, a, w.y);
+    if constexpr (WaveGeom<NBT>::KPB == 1) {
+        stage_wave<NBT, MODE, R>(be.c, a, w.y, threadIdx.x, smem_d, true);
+    } else {
+        const int sub = threadIdx.x / TPK, kk = w.y * WaveGeom<NBT>::KPB + sub;
+        stage_wave<NBT, MODE, R>(be.c, a, min(kk, be.c.nk - 1), threadIdx.x % TPK,
+                                 smem_d + sub * wave_lds_doubles<NBT, R>(be.c.diss), kk < be.c.nk);
+    }
 }
 
 // ================================================================ observables
@@ -1015,19 +1050,6 @@ struct ObsArgs {
     int nk, nb, nch, ncc;
     size_t sk, se;       // state strides: element e of k at k*sk + e*se
 };
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
 
 struct Kahan {
     double s = 0.0, c = 0.0;
@@ -1494,6 +1516,7 @@ struct edus_ctx {
     int precision = EDUS_PRECISION_FP64;
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
     int *eidx = nullptr;  // packed element -> (i, j, comp) of the wave layout
+    int dmask = 7;        // D components some field of the current call needs (see upload_table)
     bool soa = true;
     bool own_stream = true;  // false when the stream belongs to a batch
     double dt = 0, wx2 = 0, j_norm = 0, tabs_norm = 0, nktot = 0;
@@ -1597,7 +1620,8 @@ CoulArgs coul_args(edus_ctx *c, const double *Y) {
 int launch_coulomb(edus_ctx *c, const double *Y) {
     CoulArgs a = coul_args(c, Y);
     const int tx = std::max(c->ncut * c->ncut * c->nu, c->nb), t3 = c->N2 * c->ncut * c->nu;
-    hipLaunchKernelGGL(k_coul_s1, dim3(c->N1), dim3(kCoulS1Threads), 0, c->stream, a);
+    const int t1 = c->N1 * c->ncut * c->np;
+    hipLaunchKernelGGL(k_coul_s1, dim3((t1 + 255) / 256), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_s2, dim3(c->ncut * c->ncut), dim3(kCoulThreads), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_x, dim3((tx + 255) / 256), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_s3, dim3((t3 + 255) / 256), dim3(256), 0, c->stream, a);
@@ -1621,7 +1645,8 @@ ConstArgs const_args(const edus_ctx *c) {
 }
 
 template <int MODE>
-int launch_stage(edus_ctx *c, const StageArgs &a) {
+int launch_stage(edus_ctx *c, StageArgs a) {
+    a.dmask = c->dmask;
     const ConstArgs ca = const_args(c);
     if (c->coul) {
         int rc = launch_coulomb(c, a.Y);
@@ -1629,21 +1654,28 @@ int launch_stage(edus_ctx *c, const StageArgs &a) {
     }
     if (c->soa) {
         const int blocks = (c->nk + 255) / 256;
-        switch (c->nb) {
-            case 1: hipLaunchKernelGGL((k_stage_small<1, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
-            case 2: hipLaunchKernelGGL((k_stage_small<2, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
-            case 3: hipLaunchKernelGGL((k_stage_small<3, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
-            case 4: hipLaunchKernelGGL((k_stage_small<4, MODE>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
-            default: return set_err(EDUS_ERR_UNSUPPORTED, "no small kernel for this band count");
-        }
+        auto go = [&](auto dtag) {
+            constexpr bool DISS = decltype(dtag)::value;
+            switch (c->nb) {
+                case 1: hipLaunchKernelGGL((k_stage_small<1, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+                case 2: hipLaunchKernelGGL((k_stage_small<2, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+                case 3: hipLaunchKernelGGL((k_stage_small<3, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+                default: hipLaunchKernelGGL((k_stage_small<4, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+            }
+        };
+        if (c->nb < 1 || c->nb > 4) return set_err(EDUS_ERR_UNSUPPORTED, "no small kernel for this band count");
+        if (c->diss) go(std::true_type{});
+        else go(std::false_type{});
     } else {
         auto go = [&](auto tag) {
-            constexpr int NBT = decltype(tag)::value;
-            const size_t lds = (size_t)wave_mats(c->diss) * NBT * WaveGeom<NBT>::LD * sizeof(cd);
+            constexpr int NBT = decltype(tag)::value, KPB = WaveGeom<NBT>::KPB;
+            const dim3 grid((c->nk + KPB - 1) / KPB), block(WaveGeom<NBT>::TPK * KPB);
+            const size_t lds_f = (size_t)KPB * wave_lds_doubles<NBT, float>(c->diss) * 8;
+            const size_t lds_d = (size_t)KPB * wave_lds_doubles<NBT, double>(c->diss) * 8;
             if (c->precision == EDUS_PRECISION_MIXED)
-                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float>), dim3(c->nk), dim3(WaveGeom<NBT>::TPK), lds / 2, c->stream, ca, a);
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float>), grid, block, lds_f, c->stream, ca, a);
             else
-                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double>), dim3(c->nk), dim3(WaveGeom<NBT>::TPK), lds, c->stream, ca, a);
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double>), grid, block, lds_d, c->stream, ca, a);
         };
         if (c->nb <= 8) go(std::integral_constant<int, 8>{});
         else if (c->nb <= 16) go(std::integral_constant<int, 16>{});
@@ -1654,7 +1686,16 @@ int launch_stage(edus_ctx *c, const StageArgs &a) {
 }
 
 // Field table rows for n stages: E (6), gradient coefficients (n_nbr), flag.
+// A field component that is zero in every row of the call multiplies its
+// dipole component by an exact zero, so the stage kernels skip loading it (the
+// sums are bit-identical).  Rows are [E_pump(3), E_probe/2(3), ...] of length rs.
 int upload_table(edus_ctx *c, const std::vector<double> &rows) {
+    const size_t rs = 7 + (size_t)c->n_nbr;
+    int mask = 0;
+    for (size_t r = 0; r + rs <= rows.size(); r += rs)
+        for (int x = 0; x < 3; x++)
+            if (rows[r + x] != 0.0 || rows[r + 3 + x] != 0.0) mask |= 1 << x;
+    c->dmask = mask;
     if (rows.size() > c->tab_cap) {
         if (c->tab) {
             HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1879,7 +1920,7 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
             return fail(set_err(EDUS_ERR_UNSUPPORTED, "the Coulomb term needs a 1 x n1 x n2 grid (sample in yz)"));
         if (q->ncut < 1 || !q->A || !q->B || !q->C || !q->D || !q->screen || !q->v_hartree)
             return fail(set_err(EDUS_ERR_INVALID, "incomplete Coulomb tables"));
-        if (q->ncut * nb * nb > 512) return fail(set_err(EDUS_ERR_UNSUPPORTED, "Ncut * nb^2 above 512"));
+        if (q->ncut > 32) return fail(set_err(EDUS_ERR_UNSUPPORTED, "Ncut above 32"));
         c->coul = 1; c->N1 = p->nk_dims[1]; c->N2 = p->nk_dims[2]; c->ncut = q->ncut; c->nu = nb * (nb + 1) / 2;
         const int nmm = c->ncut * c->ncut, nb2 = nb * nb;
         // trig tables cos/sin(2 pi m k) with k = -0.5 + i/N (integrationWeight.h:139-175)
@@ -1965,6 +2006,7 @@ static StageArgs rk4_stage_args(const edus_ctx *c, int st, const double *row, do
     std::memset(&a, 0, sizeof a);
     a.dt = dt;
     a.tab = row;
+    a.dmask = c->dmask;
     a.P0 = c->P0; a.ACC = c->ACC;
     const double dt6 = dt / 6.0, dt3 = dt / 3.0, dt2 = dt / 2.0;
     switch (st) {
@@ -2271,7 +2313,10 @@ int batch_write_entries(edus_batch *b) {
     for (size_t i = 0; i < b->inst.size(); i++) {
         const edus_ctx *c = b->inst[i];
         e[i].c = const_args(c);
-        for (int st = 0; st < 4; st++) e[i].a[st] = rk4_stage_args(c, st, b->tab + (size_t)st * b->R_tot + b->roff[i], c->dt);
+        for (int st = 0; st < 4; st++) {
+            e[i].a[st] = rk4_stage_args(c, st, b->tab + (size_t)st * b->R_tot + b->roff[i], c->dt);
+            e[i].a[st].dmask = 7;  // the entries outlive one call: every dipole component is read
+        }
     }
     HIP_TRY(hipMemcpyAsync(b->entries, e.data(), e.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -2293,20 +2338,27 @@ template <int MODE>
 int batch_launch(edus_batch *b, const edus_batch::Cls &k, size_t tab_step) {
     const dim3 grid(k.nblocks);
     if (k.soa) {
-        switch (k.nb_small) {
-            case 1: hipLaunchKernelGGL((k_stage_small_batch<1, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
-            case 2: hipLaunchKernelGGL((k_stage_small_batch<2, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
-            case 3: hipLaunchKernelGGL((k_stage_small_batch<3, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
-            default: hipLaunchKernelGGL((k_stage_small_batch<4, MODE>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
-        }
+        auto go = [&](auto dtag) {
+            constexpr bool DISS = decltype(dtag)::value;
+            switch (k.nb_small) {
+                case 1: hipLaunchKernelGGL((k_stage_small_batch<1, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+                case 2: hipLaunchKernelGGL((k_stage_small_batch<2, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+                case 3: hipLaunchKernelGGL((k_stage_small_batch<3, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+                default: hipLaunchKernelGGL((k_stage_small_batch<4, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+            }
+        };
+        if (k.diss) go(std::true_type{});
+        else go(std::false_type{});
     } else {
         auto go = [&](auto tag) {
-            constexpr int NBT = decltype(tag)::value;
-            const size_t lds = (size_t)wave_mats(k.diss) * NBT * WaveGeom<NBT>::LD * sizeof(cd);
+            constexpr int NBT = decltype(tag)::value, KPB = WaveGeom<NBT>::KPB;
+            const dim3 block(WaveGeom<NBT>::TPK * KPB);
+            const size_t lds_f = (size_t)KPB * wave_lds_doubles<NBT, float>(k.diss) * 8;
+            const size_t lds_d = (size_t)KPB * wave_lds_doubles<NBT, double>(k.diss) * 8;
             if (k.precision == EDUS_PRECISION_MIXED)
-                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, float>), grid, dim3(WaveGeom<NBT>::TPK), lds / 2, b->stream, b->entries, k.blk, tab_step);
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, float>), grid, block, lds_f, b->stream, b->entries, k.blk, tab_step);
             else
-                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, double>), grid, dim3(WaveGeom<NBT>::TPK), lds, b->stream, b->entries, k.blk, tab_step);
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, double>), grid, block, lds_d, b->stream, b->entries, k.blk, tab_step);
         };
         if (k.nbt == 8) go(std::integral_constant<int, 8>{});
         else if (k.nbt == 16) go(std::integral_constant<int, 16>{});
@@ -2361,26 +2413,30 @@ int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch
         b->roff.push_back(b->R_tot);
         b->R_tot += 7 + c->n_nbr;
     }
-    // kernel classes: small kernels by band count, wave kernels by tile and precision
+    // kernel classes: small kernels by band count, wave kernels by tile and
+    // precision, each split by the presence of dephasing
     for (int i = 0; i < n; i++) {
         const edus_ctx *c = b->inst[i];
         const int nbt = c->soa ? 0 : (c->nb <= 8 ? 8 : c->nb <= 16 ? 16 : 32);
         edus_batch::Cls *hit = nullptr;
         for (auto &k : b->cls)
-            if (k.soa == c->soa && k.nb_small == (c->soa ? c->nb : 0) && k.nbt == nbt && k.precision == c->precision) hit = &k;
+            if (k.soa == c->soa && k.nb_small == (c->soa ? c->nb : 0) && k.nbt == nbt && k.precision == c->precision &&
+                k.diss == c->diss)
+                hit = &k;
         if (!hit) {
             b->cls.emplace_back();
             hit = &b->cls.back();
             hit->soa = c->soa; hit->nb_small = c->soa ? c->nb : 0; hit->nbt = nbt; hit->precision = c->precision;
+            hit->diss = c->diss;
         }
         hit->members.push_back(i);
-        hit->diss |= c->diss;
     }
     for (auto &k : b->cls) {
         std::vector<int2> map;
         for (int i : k.members) {
             const edus_ctx *c = b->inst[i];
-            const int nblk = c->soa ? (c->nk + 255) / 256 : c->nk;
+            const int kpb = c->nb <= 8 ? WaveGeom<8>::KPB : c->nb <= 16 ? WaveGeom<16>::KPB : WaveGeom<32>::KPB;
+            const int nblk = c->soa ? (c->nk + 255) / 256 : (c->nk + kpb - 1) / kpb;
             for (int q = 0; q < nblk; q++) map.push_back(make_int2(i, q));
         }
         k.nblocks = (int)map.size();

@@ -43,8 +43,9 @@ def test_batch_members_match_their_own_contexts(tmp_path, precision):
             e.step(f, g)
         it += nsteps
     ms, launches = batch.last_step_time()
-    classes = len({(s.info["nb"] if s.info["nb"] <= 4 else (8 if s.info["nb"] <= 8 else 16 if s.info["nb"] <= 16 else 32))
-                   for s in setups})
+    # kernel classes: band count (small kernels) or tile (wave kernels), split by dephasing
+    classes = len({(s.info["nb"] if s.info["nb"] <= 4 else (8 if s.info["nb"] <= 8 else 16 if s.info["nb"] <= 16 else 32),
+                    s.info["n_diss"] > 0) for s in setups})
     assert launches == 4 * nsteps * classes and ms > 0
     for m, e, s in zip(batch.members, singles, setups):
         np.testing.assert_array_equal(m.get_state(), e.get_state())
