@@ -739,7 +739,11 @@ struct WaveGeom {
 template <int NBT, int OPA, int OPB, typename T>
 __device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
     constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = WaveGeom<NBT>::LD;
-    const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
+    // Lane (r0, c0) owns rows r0 + i GRID and columns c0 + j GRID (interleaved,
+    // not blocked): the GRID rows one LDS read instruction touches are then
+    // consecutive, and rows NBT+1 complex apart start 4 banks apart, so the
+    // reads are free of bank conflicts.
+    const int r0 = t / GRID, c0 = t % GRID;
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
@@ -749,12 +753,12 @@ __device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)
         T av[RB], bv[RB];
 #pragma unroll
         for (int i = 0; i < RB; i++) {
-            T v = A[(r0 + i) * LD + l];
+            T v = A[(r0 + i * GRID) * LD + l];
             av[i] = OPA ? T{v.x, -v.y} : v;
         }
 #pragma unroll
         for (int j = 0; j < RB; j++) {
-            T v = (OPB == 0) ? B[(c0 + j) * LD + l] : B[l * LD + c0 + j];
+            T v = (OPB == 0) ? B[(c0 + j * GRID) * LD + l] : B[l * LD + c0 + j * GRID];
             bv[j] = (OPB == 2) ? T{v.x, -v.y} : v;
         }
 #pragma unroll
@@ -770,11 +774,11 @@ __device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)
 template <int NBT, typename T>
 __device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
     constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = WaveGeom<NBT>::LD;
-    const int r0 = (t / GRID) * RB, c0 = (t % GRID) * RB;
+    const int r0 = t / GRID, c0 = t % GRID;  // rows r0 + i GRID, columns c0 + j GRID
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
-        for (int j = 0; j < RB; j++) C[(r0 + i) * LD + c0 + j] = acc[i][j];
+        for (int j = 0; j < RB; j++) C[(r0 + i * GRID) * LD + c0 + j * GRID] = acc[i][j];
 }
 
 // Number of LDS matrices of one wave: P, M (the product X = P M^T overwrites M)
@@ -927,12 +931,12 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // G = conj(U) T1
         {
             constexpr int GRID = WaveGeom<NBT>::GRID;
-            const int r0 = (lane / GRID) * RB, c0 = (lane % GRID) * RB;
+            const int r0 = lane / GRID, c0 = lane % GRID;
 #pragma unroll
             for (int i = 0; i < RB; i++)
 #pragma unroll
                 for (int j = 0; j < RB; j++) {
-                    const int ia = r0 + i, ib = c0 + j;
+                    const int ia = r0 + i * GRID, ib = c0 + j * GRID;
                     const R t = (ia < nb && ib < nb) ? (R)c.T[ia * nb + ib] : (R)0;
                     acc[i][j] = CT{acc[i][j].x * t, acc[i][j].y * t};
                 }
