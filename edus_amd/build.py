@@ -38,7 +38,7 @@ def build_engine():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libedus_hip.so")
     src = os.path.join(CSRC, "hip", "engine.hip")
-    _run([HIPCC, *HIP_FLAGS, src, "-o", out])
+    _run([HIPCC, *HIP_FLAGS, src, "-o", out, "-lrocsolver", "-lrocblas"])
     return out
 
 

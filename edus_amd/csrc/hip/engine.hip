@@ -19,6 +19,8 @@
 // Kahan-compensated partials (Observables_MPI.h:217-775) and the Wannier phase
 // contraction (Wannier.h:278-471).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <complex>
@@ -1015,23 +1017,25 @@ template <int MODE>
 __host__ __device__ constexpr int rk4_stage_of() { return MODE == RK1 ? 0 : MODE == RK2 ? 1 : MODE == RK3 ? 2 : 3; }
 
 template <int NB, int MODE, bool DISS>
-__global__ __launch_bounds__(256) void k_stage_small_batch(const BatchEntry *E, const int2 *blk, size_t tab_step) {
+__global__ __launch_bounds__(256) void k_stage_small_batch(const BatchEntry *E, const int2 *blk, const int *dmask, size_t tab_step) {
     const int2 w = blk[xcd_block()];
     const BatchEntry &be = E[w.x];
     StageArgs a = be.a[rk4_stage_of<MODE>()];
     a.tab += tab_step;
+    a.dmask = dmask[w.x];
     stage_small<NB, MODE, DISS>(be.c, a, w.y * 256 + threadIdx.x);
 }
 
 template <int NBT, int MODE, typename R>
 __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_stage_wave_batch(const BatchEntry *E, const int2 *blk,
-                                                                                         size_t tab_step) {
+                                                                                         const int *dmask, size_t tab_step) {
     extern __shared__ double smem_d[];
     constexpr int TPK = WaveGeom<NBT>::TPK;
     const int2 w = blk[xcd_block()];
     const BatchEntry &be = E[w.x];
     StageArgs a = be.a[rk4_stage_of<MODE>()];
     a.tab += tab_step;
+    a.dmask = dmask[w.x];
     if constexpr (WaveGeom<NBT>::KPB == 1) {
         stage_wave<NBT, MODE, R>(be.c, a, w.y, threadIdx.x, smem_d, true);
     } else {
@@ -2300,6 +2304,7 @@ struct edus_batch {
     };
     std::vector<Cls> cls;
     BatchEntry *entries = nullptr;
+    int *masks = nullptr;   // [n] dipole components the call's fields need, per member
     double *tab = nullptr;  // [step][stage][R_tot]; instance i's row at roff[i]
     size_t tab_steps = 0;
     int R_tot = 0;
@@ -2319,7 +2324,7 @@ int batch_write_entries(edus_batch *b) {
         e[i].c = const_args(c);
         for (int st = 0; st < 4; st++) {
             e[i].a[st] = rk4_stage_args(c, st, b->tab + (size_t)st * b->R_tot + b->roff[i], c->dt);
-            e[i].a[st].dmask = 7;  // the entries outlive one call: every dipole component is read
+            e[i].a[st].dmask = 7;  // replaced per call by the member's mask (b->masks)
         }
     }
     HIP_TRY(hipMemcpyAsync(b->entries, e.data(), e.size() * sizeof(BatchEntry), hipMemcpyHostToDevice, b->stream));
@@ -2345,10 +2350,10 @@ int batch_launch(edus_batch *b, const edus_batch::Cls &k, size_t tab_step) {
         auto go = [&](auto dtag) {
             constexpr bool DISS = decltype(dtag)::value;
             switch (k.nb_small) {
-                case 1: hipLaunchKernelGGL((k_stage_small_batch<1, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
-                case 2: hipLaunchKernelGGL((k_stage_small_batch<2, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
-                case 3: hipLaunchKernelGGL((k_stage_small_batch<3, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
-                default: hipLaunchKernelGGL((k_stage_small_batch<4, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, tab_step); break;
+                case 1: hipLaunchKernelGGL((k_stage_small_batch<1, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
+                case 2: hipLaunchKernelGGL((k_stage_small_batch<2, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
+                case 3: hipLaunchKernelGGL((k_stage_small_batch<3, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
+                default: hipLaunchKernelGGL((k_stage_small_batch<4, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
             }
         };
         if (k.diss) go(std::true_type{});
@@ -2360,9 +2365,9 @@ int batch_launch(edus_batch *b, const edus_batch::Cls &k, size_t tab_step) {
             const size_t lds_f = (size_t)KPB * wave_lds_doubles<NBT, float>(k.diss) * 8;
             const size_t lds_d = (size_t)KPB * wave_lds_doubles<NBT, double>(k.diss) * 8;
             if (k.precision == EDUS_PRECISION_MIXED)
-                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, float>), grid, block, lds_f, b->stream, b->entries, k.blk, tab_step);
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, float>), grid, block, lds_f, b->stream, b->entries, k.blk, b->masks, tab_step);
             else
-                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, double>), grid, block, lds_d, b->stream, b->entries, k.blk, tab_step);
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, double>), grid, block, lds_d, b->stream, b->entries, k.blk, b->masks, tab_step);
         };
         if (k.nbt == 8) go(std::integral_constant<int, 8>{});
         else if (k.nbt == 16) go(std::integral_constant<int, 16>{});
@@ -2452,11 +2457,87 @@ int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch
     if (hipMalloc(&ent, (size_t)n * sizeof(BatchEntry)) != hipSuccess) return fail(set_err(EDUS_ERR_NOMEM, "batch entries"));
     b->allocs.push_back(ent);
     b->entries = static_cast<BatchEntry *>(ent);
+    void *msk = nullptr;
+    if (hipMalloc(&msk, (size_t)n * sizeof(int)) != hipSuccess) return fail(set_err(EDUS_ERR_NOMEM, "batch masks"));
+    b->allocs.push_back(msk);
+    b->masks = static_cast<int *>(msk);
     int rc = batch_table_capacity(b, 16);
     if (rc) return fail(rc);
     *out = b;
     g_err.clear();
     return 0;
+}
+
+// ---------------------------------------------------------------- setup: batched eigensolve
+// Rows a of U are conj(v_a), the eigenvectors of H in ascending eigenvalue order
+// (Wannier.h:496-542).  rocSOLVER's batched Jacobi solver reads our row-major
+// H as its column-major transpose conj(H) (its lower triangle is our upper
+// one, the triangle the engine packs), whose eigenvectors conj(v_a) it
+// returns as columns, i.e. as our rows: the memory is U up to the phases,
+// which k_eig_phase then fixes (largest-modulus component real positive, the
+// first of equal moduli within 1e-12 -- the host Jacobi's convention).
+__global__ void k_eig_phase(cd *U, int nmat, int nb) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (long long)nmat * nb) return;
+    cd *row = U + r * nb;
+    int best = 0;
+    double bm = -1.0;
+    for (int i = 0; i < nb; i++) {
+        const double m = sqrt(row[i].x * row[i].x + row[i].y * row[i].y);
+        if (m > bm + 1e-12) { bm = m; best = i; }
+    }
+    const cd ub = row[best];
+    const double mag = sqrt(ub.x * ub.x + ub.y * ub.y);
+    if (mag == 0.0) return;
+    const cd ph = mk(ub.x / mag, -ub.y / mag);  // conj(u_best) / |u_best|
+    for (int i = 0; i < nb; i++) row[i] = cmul(row[i], ph);
+}
+
+int edus_eigh_batched(int32_t nmat, int32_t nb, const double *H, double *w, double *U) {
+    if (nmat < 0 || nb < 1 || nb > EDUS_MAX_NB || (nmat > 0 && (!H || !w || !U))) return set_err(EDUS_ERR_INVALID, "bad eigensolve arguments");
+    if (nmat == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(EDUS_ERR_NODEVICE, "no HIP device visible");
+    const size_t nn = (size_t)nb * nb, chunk = std::min<size_t>((size_t)nmat, 32768);
+    rocblas_handle hd = nullptr;
+    if (rocblas_create_handle(&hd) != rocblas_status_success) return set_err(EDUS_ERR_HIP, "rocblas_create_handle failed");
+    hipStream_t st = nullptr;
+    void *dA = nullptr, *dW = nullptr, *dres = nullptr, *dinfo = nullptr, *dsw = nullptr;
+    int rc = 0;
+    auto done = [&](int r) {
+        if (st) (void)hipStreamSynchronize(st);
+        (void)hipFree(dA); (void)hipFree(dW); (void)hipFree(dres); (void)hipFree(dinfo); (void)hipFree(dsw);
+        if (st) (void)hipStreamDestroy(st);
+        rocblas_destroy_handle(hd);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || rocblas_set_stream(hd, st) != rocblas_status_success)
+        return done(set_err(EDUS_ERR_HIP, "eigensolve stream"));
+    if (hipMalloc(&dA, chunk * nn * 16) != hipSuccess || hipMalloc(&dW, chunk * nb * 8) != hipSuccess ||
+        hipMalloc(&dres, chunk * 8) != hipSuccess || hipMalloc(&dinfo, chunk * 4) != hipSuccess || hipMalloc(&dsw, chunk * 4) != hipSuccess)
+        return done(set_err(EDUS_ERR_NOMEM, "eigensolve buffers"));
+    std::vector<int> info(chunk);
+    for (size_t m0 = 0; m0 < (size_t)nmat; m0 += chunk) {
+        const size_t m = std::min(chunk, (size_t)nmat - m0);
+        if (hipMemcpyAsync(dA, H + m0 * nn * 2, m * nn * 16, hipMemcpyHostToDevice, st) != hipSuccess)
+            return done(set_err(EDUS_ERR_HIP, "eigensolve upload"));
+        const rocblas_status s1 = rocsolver_zheevj_strided_batched(
+            hd, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_lower, nb, static_cast<rocblas_double_complex *>(dA), nb,
+            (rocblas_stride)nn, 0.0, static_cast<double *>(dres), 100, static_cast<rocblas_int *>(dsw), static_cast<double *>(dW), nb,
+            static_cast<rocblas_int *>(dinfo), (rocblas_int)m);
+        if (s1 != rocblas_status_success) return done(set_err(EDUS_ERR_HIP, "rocsolver_zheevj_strided_batched failed"));
+        const long long rows = (long long)m * nb;
+        hipLaunchKernelGGL(k_eig_phase, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, static_cast<cd *>(dA), (int)m, nb);
+        if (hipGetLastError() != hipSuccess) return done(set_err(EDUS_ERR_HIP, "k_eig_phase launch"));
+        if (hipMemcpyAsync(U + m0 * nn * 2, dA, m * nn * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(w + m0 * nb, dW, m * nb * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(info.data(), dinfo, m * 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            return done(set_err(EDUS_ERR_HIP, "eigensolve download"));
+        for (size_t i = 0; i < m; i++)
+            if (info[i] != 0) return done(set_err(EDUS_ERR_HIP, "Jacobi eigensolver did not converge for matrix " + std::to_string(m0 + i)));
+    }
+    g_err.clear();
+    return done(rc);
 }
 
 int edus_batch_size(const edus_batch *b) { return b ? (int)b->inst.size() : 0; }
@@ -2472,14 +2553,19 @@ int edus_batch_step_rk4(edus_batch *b, int32_t nsteps, const double *const *fiel
     int rc = batch_table_capacity(b, (size_t)nsteps);
     if (rc) return rc;
     std::vector<double> rows((size_t)nsteps * 4 * b->R_tot), one;
+    std::vector<int> masks(n, 0);
     for (int i = 0; i < n; i++) {
         const edus_ctx *c = b->inst[i];
         const int rs = 7 + c->n_nbr;
         rk4_rows(c, nsteps, fields[i], grad_on[i], one);
-        for (size_t r = 0; r < (size_t)nsteps * 4; r++)
+        for (size_t r = 0; r < (size_t)nsteps * 4; r++) {
             std::copy(one.begin() + r * rs, one.begin() + (r + 1) * rs, rows.begin() + r * b->R_tot + b->roff[i]);
+            for (int x = 0; x < 3; x++)
+                if (one[r * rs + x] != 0.0 || one[r * rs + 3 + x] != 0.0) masks[i] |= 1 << x;
+        }
     }
     HIP_TRY(hipMemcpyAsync(b->tab, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->masks, masks.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
     int launches = 0;
     for (int s = 0; s < nsteps; s++) {

@@ -127,15 +127,11 @@ void Setup::build(const std::string &input_path, int flags) {
         int rc = edus_wannier_hk(in.Nk, w.nR, nb, w.R.data(), HR.data(), rR.data(), reinterpret_cast<double *>(H.data()),
                                  reinterpret_cast<double *>(D.data()));
         if (rc != 0) throw Error(std::string("Wannier contraction on the GPU failed: ") + edus_last_error());
-        // U from the eigenvectors of H(k) (Wannier.h:496-542): rows are conj(v_a).
-#pragma omp parallel for schedule(dynamic, 64)
-        for (int ik = 0; ik < nk; ik++) {
-            std::vector<double> ev(nb);
-            std::vector<cplx> V((size_t)nb * nb);
-            eigh_jacobi(nb, &H[mi(nb, ik, 0, 0)], ev.data(), V.data());
-            for (int a = 0; a < nb; a++)
-                for (int j = 0; j < nb; j++) U[mi(nb, ik, a, j)] = std::conj(V[(size_t)j * nb + a]);
-        }
+        // U from the eigenvectors of H(k) (Wannier.h:496-542): rows are conj(v_a),
+        // batched on the GPU like the contraction.
+        std::vector<double> ev((size_t)nk * nb);
+        rc = edus_eigh_batched(nk, nb, reinterpret_cast<const double *>(H.data()), ev.data(), reinterpret_cast<double *>(U.data()));
+        if (rc != 0) throw Error(std::string("eigensolve on the GPU failed: ") + edus_last_error());
     }
     // Scissor: conduction_shift on bands ic >= Nb[1] in the eigenbasis (save_HUD.cpp:147-181).
     if (in.change_gap_constant > 1e-16) {

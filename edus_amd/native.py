@@ -102,6 +102,7 @@ hip.edus_set_state.argtypes = [C.c_void_p, _dp]
 hip.edus_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
 hip.edus_wannier_hk.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
 hip.edus_last_error.restype = C.c_char_p
+hip.edus_eigh_batched.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp]
 hip.edus_batch_create.argtypes = [C.POINTER(C.POINTER(Problem)), C.c_int32, C.POINTER(C.c_void_p)]
 hip.edus_batch_destroy.argtypes = [C.c_void_p]
 hip.edus_batch_size.argtypes = [C.c_void_p]
@@ -237,6 +238,17 @@ class Setup:
         if n < 0:
             raise EdusError(host.edus_host_last_error().decode())
         return n
+
+
+def eigh_batched(H):
+    """Eigenvalues (ascending) and U (rows conj(v_a), largest component of each
+    eigenvector real positive) of a stack of Hermitian matrices [n][nb][nb]."""
+    H = np.ascontiguousarray(H, dtype=np.complex128)
+    n, nb = H.shape[0], H.shape[1]
+    w = np.zeros((n, nb))
+    U = np.zeros_like(H)
+    _check_hip(hip.edus_eigh_batched(n, nb, _dptr(H.view(np.float64)), _dptr(w), _dptr(U.view(np.float64))), "edus_eigh_batched")
+    return w, U
 
 
 class Batch:

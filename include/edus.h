@@ -195,6 +195,13 @@ int edus_batch_last_step_time(const edus_batch *b, double *ms, int32_t *nlaunch)
  * edus_rhs; it is owned by the batch (do not destroy it).  NULL if out of range. */
 edus_ctx *edus_batch_member(edus_batch *b, int32_t i);
 
+/* Eigen-decomposition of nmat Hermitian nb x nb matrices H [nmat][nb][nb] complex
+ * (row-major, upper triangle read) for the setup's U tables: w [nmat][nb]
+ * ascending eigenvalues, U [nmat][nb][nb] complex with row a = conj(v_a), each
+ * eigenvector phased so that its largest-modulus component is real positive.
+ * Replaces the per-k eig_sym of Wannier.h:496-542 (save_HUD.cpp:142-210). */
+int edus_eigh_batched(int32_t nmat, int32_t nb, const double *H, double *w, double *U);
+
 /* Parameters of the Coulomb coefficient generator (Source_Main/Fourier_Coulomb.cpp:95-500
  * with headers/Coulomb.h:60-267 and integrationWeight.h:5-76), "011" orientation:
  * the sample lies in the plane of crystal directions 1 and 2, q = b1 q1 + b2 q2. */
