@@ -203,27 +203,42 @@ def currents(ob):
     return ob["j1"][0::2] - 2.0 * ob["j2"][1::2]
 
 
-def mixed_j_error(s, eng, it0, nsteps, every=10):
-    """max_t |J_mixed - J_fp64| / max_t |J_fp64| over nsteps from rho0 at the pulse
-    peak window, observables every `every` steps; the FP64 side is the drop-in path."""
+def mixed_j_error(s, eng, it0, nsteps, every=2):
+    """The mixed form's error against the FP64 drop-in path over nsteps from rho0
+    in the pulse-peak window, both engines given the same fields: J(t) (every
+    `every` steps), the final conduction population, and the harmonic spectrum
+    |FFT(J w)|^2 (Hann window w) of the J(t) series."""
     from edus_amd import native as N
 
+    info = s.info
     ref = N.Engine(s.problem(), precision=N.PRECISION_FP64)
-    P0 = s.array("rho0", True).reshape(-1, s.info["nb"], s.info["nb"])
+    P0 = s.array("rho0", True).reshape(-1, info["nb"], info["nb"])
     f, g = s.fields(it0, nsteps)
     jm, jr = [], []
     for e in (eng, ref):
         e.set_state(P0)
+    obs = {}
     for a in range(0, nsteps, every):
         b = min(a + every, nsteps)
         for e, acc in ((eng, jm), (ref, jr)):
             e.step(f[a:b], g[a:b])
-            acc.append(currents(e.observables()))
+            o = e.observables()
+            acc.append(currents(o))
+            obs[id(e)] = o
     jm, jr = np.array(jm), np.array(jr)
     scale = max(float(np.max(np.abs(jr))), 1e-300)
+    # conduction population: the bands above the filled ones, summed
+    ncc = info["ncc"]
+    cm, cr = float(np.sum(obs[id(eng)]["pop"][ncc:])), float(np.sum(obs[id(ref)]["pop"][ncc:]))
+    # harmonic intensities of each Cartesian component, relative to the largest
+    win = np.hanning(len(jr))[:, None]
+    im = np.abs(np.fft.rfft(jm * win, axis=0)) ** 2
+    ir = np.abs(np.fft.rfft(jr * win, axis=0)) ** 2
     pm, pr = eng.get_state(), ref.get_state()
     ref.close()
     return {"J_rel_err": float(np.max(np.abs(jm - jr)) / scale), "J_max": scale,
+            "conduction_population_rel_err": abs(cm - cr) / max(abs(cr), 1e-300), "conduction_population": cr,
+            "harmonic_intensity_rel_err": float(np.max(np.abs(im - ir)) / max(float(np.max(ir)), 1e-300)),
             "rho_rel_err": float(np.max(np.abs(pm - pr)) / np.max(np.abs(pr))),
             "steps": nsteps, "observed_every": every, "reference": "FP64 path, same steps and fields"}
 

@@ -1300,8 +1300,38 @@ __global__ void k_eigen_element(const double *Y, const double *U, int nk, int nb
 // Tiled complex GEMM: 64 k x 64 m per block, 16x16 threads with 4x4 outputs.
 constexpr int WT = 64, WK = 8;
 
-__global__ __launch_bounds__(256) void k_wannier(const int *__restrict__ R, const cd *__restrict__ Val, const cd *__restrict__ tw,
-                                                 int N0, int N1, int N2, int off1, int off2, int nR, int nm, int nk, int kbase, cd *__restrict__ Out) {
+// Per-direction phase tables of the contraction, built once per call:
+// PT[Off_l + r N_l + i] = tw[Off'_l + (i R_l mod N_l)] (-1)^{R_l}, the factor of
+// direction l in e^{2 pi i k.R} for grid index i (exact integer reduction; the
+// sign flip for odd R_l is the shift of the grid to k = -0.5 + i/N); 1 for a
+// direction with a single k point.
+__global__ void k_wannier_ptab(const int *R, const cd *tw, int nR, int N0, int N1, int N2, int toff1, int toff2, cd *PT) {
+    const int Nn[3] = {N0, N1, N2}, Toff[3] = {0, toff1, toff2};
+    const long long o1 = (long long)nR * N0, o2 = o1 + (long long)nR * N1, tot = o2 + (long long)nR * N2;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+        const int l = t < o1 ? 0 : t < o2 ? 1 : 2;
+        const long long u = t - (l == 0 ? 0 : l == 1 ? o1 : o2);
+        const int r = (int)(u / Nn[l]), i = (int)(u % Nn[l]);
+        const long long Rl = R[r * 3 + l];
+        long long q = ((long long)i * Rl) % Nn[l];
+        if (q < 0) q += Nn[l];
+        cd w = tw[Toff[l] + q];
+        if (Rl & 1) w = mk(-w.x, -w.y);
+        PT[t] = Nn[l] == 1 ? mk(1.0, 0.0) : w;  // a direction without k points contributes no phase
+    }
+}
+
+// Phase of grid point ka and lattice vector r from the tables above.
+__device__ __forceinline__ cd wannier_phase(const cd *PT, int nR, int ka, int r, int N0, int N1, int N2) {
+    const int i0 = ka / (N1 * N2), i1 = (ka / N2) % N1, i2 = ka % N2;
+    const size_t o1 = (size_t)nR * N0, o2 = o1 + (size_t)nR * N1;
+    cd ph = PT[(size_t)r * N0 + i0];
+    ph = cmul(ph, PT[o1 + (size_t)r * N1 + i1]);
+    return cmul(ph, PT[o2 + (size_t)r * N2 + i2]);
+}
+
+__global__ __launch_bounds__(256) void k_wannier(const cd *__restrict__ PT, const cd *__restrict__ Val, int N0, int N1, int N2, int nR,
+                                                 int nm, int nk, int kbase, cd *__restrict__ Out) {
     __shared__ cd Ph[WT][WK + 1];
     __shared__ cd Vs[WK][WT + 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -1314,23 +1344,7 @@ __global__ __launch_bounds__(256) void k_wannier(const int *__restrict__ R, cons
     for (int r0 = 0; r0 < nR; r0 += WK) {
         for (int t = threadIdx.x; t < WT * WK; t += 256) {
             const int kk = t / WK, rr = t % WK, k = k0 + kk, r = r0 + rr;
-            cd ph = mk(0.0, 0.0);
-            if (k < nk && r < nR) {
-                const int ka = kbase + k;
-                const int i0 = ka / (N1 * N2), i1 = (ka / N2) % N1, i2 = ka % N2;
-                const int Ii[3] = {i0, i1, i2}, Nn[3] = {N0, N1, N2}, Off[3] = {0, off1, off2};
-                ph = mk(1.0, 0.0);
-                for (int l = 0; l < 3; l++) {
-                    if (Nn[l] == 1) continue;
-                    const long long Rl = R[r * 3 + l];
-                    long long q = ((long long)Ii[l] * Rl) % Nn[l];
-                    if (q < 0) q += Nn[l];
-                    cd w = tw[Off[l] + q];
-                    if (Rl & 1) w = mk(-w.x, -w.y);
-                    ph = cmul(ph, w);
-                }
-            }
-            Ph[kk][rr] = ph;
+            Ph[kk][rr] = (k < nk && r < nR) ? wannier_phase(PT, nR, kbase + k, r, N0, N1, N2) : mk(0.0, 0.0);
         }
         for (int t = threadIdx.x; t < WK * WT; t += 256) {
             const int rr = t / WT, mm = t % WT, r = r0 + rr, m = m0 + mm;
@@ -1360,6 +1374,58 @@ __global__ __launch_bounds__(256) void k_wannier(const int *__restrict__ R, cons
         for (int j = 0; j < 4; j++) {
             const int k = k0 + ty * 4 + i, m = m0 + tx * 4 + j;
             if (k < nk && m < nm) Out[(size_t)k * nm + m] = acc[i][j];
+        }
+}
+
+// The same contraction on the FP64 matrix cores: Out = Ph Val as four real
+// products (Re = PhR ValR - PhI ValI, Im = PhR ValI + PhI ValR) of 16x16x4
+// v_mfma_f64 tiles.  A workgroup of 4 waves owns 64 k x 32 columns (each wave
+// 16 k x 32 columns: two 16x16 tiles with real and imaginary accumulators); R
+// is consumed 16 at a time through LDS.  Lane l feeds A[row l%16][kk l/16] and
+// B[kk l/16][col l%16] and holds D[l/16 + 4 i][l%16], i = 0..3 (the layout of
+// the gfx950 instruction, checked against a host product).
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int WMK = 64, WMN = 32, WMR = 16;
+
+__global__ __launch_bounds__(256) void k_wannier_mfma(const cd *__restrict__ PT, const cd *__restrict__ Val, int N0, int N1, int N2,
+                                                      int nR, int nm, int nk, int kbase, cd *__restrict__ Out) {
+    __shared__ cd Ph[WMK][WMR + 1];
+    __shared__ cd Vs[WMR][WMN + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k0 = blockIdx.y * WMK, m0 = blockIdx.x * WMN;
+    v4d are[2], aim[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) are[t] = aim[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int r0 = 0; r0 < nR; r0 += WMR) {
+        for (int t = threadIdx.x; t < WMK * WMR; t += 256) {
+            const int kk = t / WMR, rr = t % WMR, k = k0 + kk, r = r0 + rr;
+            Ph[kk][rr] = (k < nk && r < nR) ? wannier_phase(PT, nR, kbase + k, r, N0, N1, N2) : mk(0.0, 0.0);
+        }
+        for (int t = threadIdx.x; t < WMR * WMN; t += 256) {
+            const int rr = t / WMN, mm = t % WMN, r = r0 + rr, m = m0 + mm;
+            Vs[rr][mm] = (r < nR && m < nm) ? Val[(size_t)r * nm + m] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < WMR / 4; ks++) {
+            const cd a = Ph[wave * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const cd b = Vs[ks * 4 + (lane >> 4)][t * 16 + (lane & 15)];
+                are[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, are[t], 0, 0, 0);
+                are[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, are[t], 0, 0, 0);
+                aim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, aim[t], 0, 0, 0);
+                aim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, aim[t], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = k0 + wave * 16 + (lane >> 4) + 4 * i, m = m0 + t * 16 + (lane & 15);
+            if (k < nk && m < nm) Out[(size_t)k * nm + m] = mk(are[t][i], aim[t][i]);
         }
 }
 
@@ -2838,11 +2904,13 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
         }
     }
     int *dR = nullptr;
-    double *dVal = nullptr, *dTw = nullptr, *dOut = nullptr;
-    auto cleanup = [&]() { (void)hipFree(dR); (void)hipFree(dVal); (void)hipFree(dTw); (void)hipFree(dOut); };
+    double *dVal = nullptr, *dTw = nullptr, *dOut = nullptr, *dPT = nullptr;
+    auto cleanup = [&]() { (void)hipFree(dR); (void)hipFree(dVal); (void)hipFree(dTw); (void)hipFree(dOut); (void)hipFree(dPT); };
+    const size_t npt = (size_t)nR * (nk3[0] + nk3[1] + nk3[2]);
     const size_t chunk = std::min<long long>(nk, std::max<long long>(64, (1LL << 28) / ((long long)nm * 16)));
     if (hipMalloc(&dR, (size_t)nR * 3 * sizeof(int)) != hipSuccess || hipMalloc(&dVal, val.size() * sizeof(double)) != hipSuccess ||
-        hipMalloc(&dTw, tw.size() * sizeof(double)) != hipSuccess || hipMalloc(&dOut, chunk * nm * 2 * sizeof(double)) != hipSuccess) {
+        hipMalloc(&dTw, tw.size() * sizeof(double)) != hipSuccess || hipMalloc(&dOut, chunk * nm * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dPT, npt * 2 * sizeof(double)) != hipSuccess) {
         cleanup();
         return set_err(EDUS_ERR_NOMEM, "Wannier buffers");
     }
@@ -2852,12 +2920,14 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
         cleanup();
         return set_err(EDUS_ERR_HIP, "Wannier upload");
     }
+    hipLaunchKernelGGL(k_wannier_ptab, dim3((unsigned)std::min<size_t>((npt + 255) / 256, 65536)), dim3(256), 0, 0, (const int *)dR,
+                       (const cd *)dTw, nR, nk3[0], nk3[1], nk3[2], offs[1], offs[2], (cd *)dPT);
     std::vector<double> all(chunk * nm * 2);
     for (long long k0 = 0; k0 < nk; k0 += chunk) {
         const int kc = (int)std::min<long long>(chunk, nk - k0);
-        dim3 grid((nm + WT - 1) / WT, (unsigned)((kc + WT - 1) / WT));
-        hipLaunchKernelGGL(k_wannier, grid, dim3(256), 0, 0, (const int *)dR, (const cd *)dVal, (const cd *)dTw, nk3[0], nk3[1], nk3[2],
-                           offs[1], offs[2], nR, nm, kc, (int)k0, (cd *)dOut);
+        const dim3 grid((nm + WMN - 1) / WMN, (unsigned)((kc + WMK - 1) / WMK));
+        hipLaunchKernelGGL(k_wannier_mfma, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR, nm, kc,
+                           (int)k0, (cd *)dOut);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpy(all.data(), dOut, (size_t)kc * nm * 16, hipMemcpyDeviceToHost);
         if (e != hipSuccess) {

@@ -66,3 +66,21 @@ def test_adaptive_taylor_with_mean_field_matches_oracle(tmp_path):
     steps = s.run(out, max_steps=n)
     assert steps == len(dts)
     _compare_files(os.path.join(out, "Output"), rows, 3, "coulomb taylor", rel=1e-8)
+
+
+def test_adaptive_wannier_run_matches_oracle(tmp_path):
+    """Adaptive dt on the wave kernel (8 Wannier bands, RK4)."""
+    from edus_amd import workloads
+    p = workloads.wannier_synthetic(str(tmp_path), nw=8, rmax=2, nk=10, nsteps=400, dt=0.2, stepper="RK4", nvalence=4,
+                                    seed=11, e_v_ang=2.0, it_resolution=2)
+    text = open(p).read().replace("num_pump_pulses 1\n", "num_pump_pulses 1\ndynamical_dt_evolution\nepsStepAbs 2e-3\n")
+    open(p, "w").write(text)
+    s = N.Setup(p)
+    assert s.info["dyn_dt"] == 1
+    osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
+    n = 80
+    rows, _, dts = Hh.oracle_run_dyn(s, osys, n)
+    out = str(tmp_path / "run")
+    steps = s.run(out, max_steps=n)
+    assert steps == len(dts)
+    _compare_files(os.path.join(out, "Output"), rows, 8, "wannier8 adaptive")

@@ -184,7 +184,7 @@ def test_trajectory_small_tb_matches_oracle(tmp_path, stepper, order):
     assert np.max(np.abs(Pg - Pora)) <= 1e-11 * np.max(np.abs(Pora))
 
 
-@pytest.mark.parametrize("nw,stepper", [(8, "RK4"), (16, "RK4"), (16, "Taylor"), (32, "RK4")])
+@pytest.mark.parametrize("nw,stepper", [(8, "RK4"), (8, "AB"), (16, "RK4"), (16, "Taylor"), (32, "RK4")])
 def test_trajectory_wannier_matches_oracle(tmp_path, nw, stepper):
     from edus_amd import workloads
     p = workloads.wannier_synthetic(str(tmp_path), nw=nw, rmax=2, nk=12, nsteps=400, dt=0.05, stepper=stepper,
