@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--rmax", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--stepper", choices=["rk4", "taylor4"], default="rk4",
+                    help="wannier workloads: classic RK4 (four stage launches a step) or the Taylor-4 multistep "
+                         "integrator (one launch a step)")
     ap.add_argument("--acc-steps", type=int, default=400, help="steps of the mixed-vs-FP64 J(t) comparison")
     ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
     ap.add_argument("--end-to-end", action="store_true",
@@ -72,18 +75,24 @@ def main():
     from edus_amd import workloads
 
     wd = args.workdir or tempfile.mkdtemp(prefix="edus_bench_")
+    if args.stepper != "rk4" and args.workload not in ("wannier", "wannier32"):
+        ap.error("--stepper taylor4 applies to the wannier workloads")
+    wstep = "RK4" if args.stepper == "rk4" else "Taylor"
     nsteps_total = args.steps + args.warmup
     t_setup = time.perf_counter()
     if args.workload == "wannier":
         nkd = args.nk or 200
         inp = workloads.wannier_synthetic(wd, nw=args.nw, rmax=args.rmax, nk=nkd, nsteps=max(4000, nsteps_total),
-                                          dt=0.05, stepper="RK4", e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0)
-        wname = f"wannier_nw{args.nw}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{nkd}x{nkd}_rk4_dt0.05"
+                                          dt=0.05, stepper=wstep, e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0)
+        wname = f"wannier_nw{args.nw}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{nkd}x{nkd}_{args.stepper}_dt0.05"
     elif args.workload == "wannier32":
+        # 32 bands, |R| <= 15, 512 x 512, two pumps of 20 fs intensity FWHM:
+        # 0.4 eV at 0.4 V/A and 0.8 eV at 0.1 V/A, dt 0.04 au
         nkd = args.nk or 512
         inp = workloads.wannier_synthetic(wd, nw=32, rmax=15, nk=nkd, nsteps=max(5000, nsteps_total), dt=0.04,
-                                          stepper="RK4", e_v_ang=0.4, nvalence=8, seed=4004, decay_ang=3.0)
-        wname = f"wannier_nw32_R31x31_{nkd}x{nkd}_rk4_dt0.04"
+                                          stepper=wstep, e_v_ang=0.4, nvalence=8, seed=4004, decay_ang=3.0,
+                                          photon_ev=0.4, fwhm_fs=20.0, extra_pumps=((0.8, 0.1),))
+        wname = f"wannier_nw32_R31x31_{nkd}x{nkd}_2pumps_{args.stepper}_dt0.04"
     elif args.workload == "hbn_coulomb":
         nkd = args.nk or 300
         inp = workloads.hbn_coulomb(wd, os.path.join(ROOT, "tests", "golden", "coulomb"), "G_num502657_Ncut20_", nk=nkd,
@@ -131,8 +140,9 @@ def main():
     # n_op operator matrices per RHS (H, Dx, Dy, Dz; +U with dephasing).  The engine
     # stores each Hermitian matrix packed (8 nb^2 bytes), so the bytes it has to move
     # are half of that (U stays full complex).
-    # Adams-Bashforth s: S * (s + 2 + n_op) per step (read y_n, write f_n, read s-1
-    # old f, write y_n+1), one launch per step, plus S for the Coulomb pass.
+    # Adams-Bashforth s and Taylor s: S * (s + 2 + n_op) per step (read y_n, write
+    # f_n, read s-1 old f, write y_n+1), one launch per step, plus S for the
+    # Coulomb pass.
     S = 16 * nb * nb
     n_op = 4 + (1 if info["n_diss"] > 0 else 0)
     rk4 = info["stepper"] == N.STEPPER_RK4
@@ -146,7 +156,7 @@ def main():
     opw = 1 if args.precision == "fp64" else 2
     stored_launch = (transfers * (S // 2) * nk + (1 + nd) * (S // 2 // opw) * nk
                      + (S // opw * nk if info["n_diss"] > 0 else 0))
-    stepper = "RK4" if rk4 else f"AB{info['order']}"
+    stepper = "RK4" if rk4 else (f"Taylor{info['order']}" if info["stepper"] == N.STEPPER_TAYLOR else f"AB{info['order']}")
     avg_launch_s = dev_ms * 1e-3 / nlaunch
     achieved = bytes_launch / avg_launch_s / 1e9
     stored = stored_launch / avg_launch_s / 1e9

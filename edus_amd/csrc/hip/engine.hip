@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/edus.h"
@@ -783,6 +784,73 @@ __device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom
         for (int j = 0; j < RB; j++) C[(r0 + i * GRID) * LD + c0 + j * GRID] = acc[i][j];
 }
 
+// The same products on the matrix cores (NBT = 16: the k's one wave computes the
+// 16x16 result; NBT = 32: wave w of the k's four computes tile (w / 2, w % 2)).
+// A complex product is four real 16x16x4 MFMA chains per 4-wide slice of l:
+// Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br.  Lane l feeds op(A)[row l%16][l/16]
+// and op(B)[l/16][col l%16]; accumulator element i of lane l is C[row][l%16] with
+// row = l/16 + 4 i for the FP64 instruction and 4 (l/16) + i for the FP32 one.
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <typename R> struct MfmaAcc;
+template <> struct MfmaAcc<double> { using T = v4d; };
+template <> struct MfmaAcc<float> { using T = v4f; };
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ v4f mfma16(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+template <typename R>
+__device__ __forceinline__ int mfma_row(int lane, int i) {
+    return std::is_same<R, double>::value ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
+}
+
+template <int NBT>
+__device__ __forceinline__ void mfma_tile(int t, int &r0, int &c0) {
+    const int w = t >> 6;
+    r0 = NBT == 32 ? (w >> 1) * 16 : 0;
+    c0 = NBT == 32 ? (w & 1) * 16 : 0;
+}
+
+template <int NBT, int OPA, int OPB, typename R, typename T = typename CplxOf<R>::T, typename V = typename MfmaAcc<R>::T>
+__device__ __forceinline__ void gemm_mfma(const T *A, const T *B, int t, V &cre, V &cim) {
+    constexpr int LD = WaveGeom<NBT>::LD;
+    int r0, c0;
+    mfma_tile<NBT>(t, r0, c0);
+    const int lane = t & 63, ra = r0 + (lane & 15), cb = c0 + (lane & 15), kq = lane >> 4;
+    cre = V{0, 0, 0, 0};
+    cim = V{0, 0, 0, 0};
+#pragma unroll
+    for (int k0 = 0; k0 < NBT; k0 += 4) {
+        const int l = k0 + kq;
+        T a = A[ra * LD + l];
+        if (OPA) a.y = -a.y;
+        T b = (OPB == 0) ? B[cb * LD + l] : B[l * LD + cb];
+        if (OPB == 2) b.y = -b.y;
+        cre = mfma16(a.x, b.x, cre);
+        cre = mfma16(-a.y, b.y, cre);
+        cim = mfma16(a.x, b.y, cim);
+        cim = mfma16(a.y, b.x, cim);
+    }
+}
+
+template <int NBT, typename R, typename T = typename CplxOf<R>::T, typename V = typename MfmaAcc<R>::T>
+__device__ __forceinline__ void store_mfma(T *C, int t, const V &cre, const V &cim) {
+    constexpr int LD = WaveGeom<NBT>::LD;
+    int r0, c0;
+    mfma_tile<NBT>(t, r0, c0);
+    const int lane = t & 63;
+#pragma unroll
+    for (int i = 0; i < 4; i++) C[(r0 + mfma_row<R>(lane, i)) * LD + c0 + (lane & 15)] = T{cre[i], cim[i]};
+}
+
+// Row and column of accumulator element i of thread t (for the dephasing rates).
+template <int NBT, typename R>
+__device__ __forceinline__ void mfma_elem(int t, int i, int &row, int &col) {
+    int r0, c0;
+    mfma_tile<NBT>(t, r0, c0);
+    row = r0 + mfma_row<R>(t & 63, i);
+    col = c0 + (t & 15);
+}
+
 // Number of LDS matrices of one wave: P, M (the product X = P M^T overwrites M)
 // and, with dephasing, U.
 __host__ __device__ constexpr int wave_mats(int diss) { return diss ? 3 : 2; }
@@ -811,6 +879,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     constexpr int TPK = WaveGeom<NBT>::TPK, RB = WaveGeom<NBT>::RB, PER = WaveGeom<NBT>::PER;
     constexpr int LD = WaveGeom<NBT>::LD, MAT = NBT * LD;
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
+    constexpr bool kMfma = NBT >= 16;  // 16x16 matrix-core tiles; NBT = 8 stays on the VALU
     CT *A = reinterpret_cast<CT *>(smem), *B = A + MAT, *C = c.diss ? B + MAT : B;
     const R *Hop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.H : (const void *)c.Hf);
     const R *Dop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.D : (const void *)c.Df);
@@ -901,7 +970,12 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     }
     __syncthreads();
     // X = P M^T; -i[M,rho] (transposed storage) = -i (X - X^+)
-    {
+    if constexpr (kMfma) {
+        typename MfmaAcc<R>::T cre, cim;
+        gemm_mfma<NBT, 0, 0, R>(A, B, lane, cre, cim);
+        if (C == B) __syncthreads();  // every lane has finished reading M
+        store_mfma<NBT, R>(C, lane, cre, cim);
+    } else {
         CT acc[RB][RB];
         gemm_lds<NBT, 0, 0>(A, B, lane, acc);
         if (C == B) __syncthreads();  // every lane has finished reading M
@@ -925,35 +999,63 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         const R *u = Uop + (size_t)k * nb * nb * 2;
         for (int t = lane; t < nb * nb; t += TPK) C[(t / nb) * LD + (t % nb)] = CT{ldnt(u + 2 * t), ldnt(u + 2 * t + 1)};
         __syncthreads();
-        CT acc[RB][RB];
-        gemm_lds<NBT, 0, 0>(A, C, lane, acc);     // T1 = P U^T
-        __syncthreads();
-        store_block<NBT>(B, lane, acc);
-        __syncthreads();
-        gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // G = conj(U) T1
-        {
-            constexpr int GRID = WaveGeom<NBT>::GRID;
-            const int r0 = lane / GRID, c0 = lane % GRID;
+        if constexpr (kMfma) {
+            typename MfmaAcc<R>::T cre, cim;
+            gemm_mfma<NBT, 0, 0, R>(A, C, lane, cre, cim);     // T1 = P U^T
+            __syncthreads();
+            store_mfma<NBT, R>(B, lane, cre, cim);
+            __syncthreads();
+            gemm_mfma<NBT, 1, 1, R>(C, B, lane, cre, cim);     // G = conj(U) T1
 #pragma unroll
-            for (int i = 0; i < RB; i++)
-#pragma unroll
-                for (int j = 0; j < RB; j++) {
-                    const int ia = r0 + i * GRID, ib = c0 + j * GRID;
-                    const R t = (ia < nb && ib < nb) ? (R)c.T[ia * nb + ib] : (R)0;
-                    acc[i][j] = CT{acc[i][j].x * t, acc[i][j].y * t};
-                }
+            for (int i = 0; i < 4; i++) {
+                int ia, ib;
+                mfma_elem<NBT, R>(lane, i, ia, ib);
+                const R t = (ia < nb && ib < nb) ? (R)c.T[ia * nb + ib] : (R)0;
+                cre[i] *= t;
+                cim[i] *= t;
+            }
+            __syncthreads();
+            store_mfma<NBT, R>(A, lane, cre, cim);               // A <- K = T o G
+            __syncthreads();
+            gemm_mfma<NBT, 0, 2, R>(A, C, lane, cre, cim);     // T2 = K conj(U)
+            __syncthreads();
+            store_mfma<NBT, R>(B, lane, cre, cim);
+            __syncthreads();
+            gemm_mfma<NBT, 1, 1, R>(C, B, lane, cre, cim);     // W = conj(U) T2
+            __syncthreads();
+            store_mfma<NBT, R>(A, lane, cre, cim);
+            __syncthreads();
+        } else {
+            CT acc[RB][RB];
+            gemm_lds<NBT, 0, 0>(A, C, lane, acc);     // T1 = P U^T
+            __syncthreads();
+            store_block<NBT>(B, lane, acc);
+            __syncthreads();
+            gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // G = conj(U) T1
+            {
+                constexpr int GRID = WaveGeom<NBT>::GRID;
+                const int r0 = lane / GRID, c0 = lane % GRID;
+    #pragma unroll
+                for (int i = 0; i < RB; i++)
+    #pragma unroll
+                    for (int j = 0; j < RB; j++) {
+                        const int ia = r0 + i * GRID, ib = c0 + j * GRID;
+                        const R t = (ia < nb && ib < nb) ? (R)c.T[ia * nb + ib] : (R)0;
+                        acc[i][j] = CT{acc[i][j].x * t, acc[i][j].y * t};
+                    }
+            }
+            __syncthreads();
+            store_block<NBT>(A, lane, acc);               // A <- K = T o G
+            __syncthreads();
+            gemm_lds<NBT, 0, 2>(A, C, lane, acc);     // T2 = K conj(U)
+            __syncthreads();
+            store_block<NBT>(B, lane, acc);
+            __syncthreads();
+            gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // W = conj(U) T2
+            __syncthreads();
+            store_block<NBT>(A, lane, acc);
+            __syncthreads();
         }
-        __syncthreads();
-        store_block<NBT>(A, lane, acc);               // A <- K = T o G
-        __syncthreads();
-        gemm_lds<NBT, 0, 2>(A, C, lane, acc);     // T2 = K conj(U)
-        __syncthreads();
-        store_block<NBT>(B, lane, acc);
-        __syncthreads();
-        gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // W = conj(U) T2
-        __syncthreads();
-        store_block<NBT>(A, lane, acc);
-        __syncthreads();
 #pragma unroll
         for (int m = 0; m < PER; m++) {
             const int i = ei[m], j = ej[m];
@@ -1384,7 +1486,6 @@ __global__ __launch_bounds__(256) void k_wannier(const cd *__restrict__ PT, cons
 // is consumed 16 at a time through LDS.  Lane l feeds A[row l%16][kk l/16] and
 // B[kk l/16][col l%16] and holds D[l/16 + 4 i][l%16], i = 0..3 (the layout of
 // the gfx950 instruction, checked against a host product).
-typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int WMK = 64, WMN = 32, WMR = 16;
 
 __global__ __launch_bounds__(256) void k_wannier_mfma(const cd *__restrict__ PT, const cd *__restrict__ Val, int N0, int N1, int N2,

@@ -83,9 +83,11 @@ def hbn_tb(directory, nk=60, nsteps=8000, dt=0.05, stepper="RK4", order=4, e_v_a
 
 def wannier_synthetic(directory, nw=16, rmax=10, nk=200, nsteps=4000, dt=0.05, stepper="RK4", order=4,
                       e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0, t2_fs=0.0, it_resolution=10,
-                      photon_ev=0.6, fwhm_fs=15.0, pol=(0, 0, 1), phase=0.0):
+                      photon_ev=0.6, fwhm_fs=15.0, pol=(0, 0, 1), phase=0.0, extra_pumps=()):
     """Config 1 / 4 family: seeded random Hermitian H(R), r(R) on a hexagonal
-    lattice (a = 2.5 A), `nvalence` filled bands, pump along z."""
+    lattice (a = 2.5 A), `nvalence` filled bands, pump along z.  `extra_pumps`
+    lists further (photon eV, V/A) gaussian pumps of the same width, polarisation
+    and peak time (num_pump_pulses > 1, ReadInput.h:485-490, 965-967)."""
     from . import native
 
     seedname = f"syn_nw{nw}_r{rmax}_s{seed}"
@@ -106,12 +108,13 @@ def wannier_synthetic(directory, nw=16, rmax=10, nk=200, nsteps=4000, dt=0.05, s
             f"dt {dt!r} au\n"
             f"t_fin {t_fin_for(nsteps, dt)!r} au\n"
             f"{st}"
-            "num_pump_pulses 1\n"
+            f"num_pump_pulses {1 + len(extra_pumps)}\n"
             "}\n\n"
             f"bands{{\nvalence {nvalence}\nconduction {nw - nvalence}\n}}\n\n"
             f"nkPT{{\n1 {nk} {nk}\n}}\n\n"
             f"{dec}\n"
             + _pump(photon_ev * EV_AU, e_v_ang, fwhm_fs, pol, t_peak, phase)
+            + "".join(_pump(ev * EV_AU, e0, fwhm_fs, pol, t_peak, phase) for ev, e0 in extra_pumps)
             + "\n" + PROBE_OFF + "\n"
             f"observables{{\nit_resolution {it_resolution}\nCurrent\nTAbs\n}}\n")
     return write(os.path.join(directory, "input.txt"), text)
