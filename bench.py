@@ -48,6 +48,9 @@ def main():
                     help="wannier workloads: classic RK4 (four stage launches a step) or the Taylor-4 multistep "
                          "integrator (one launch a step)")
     ap.add_argument("--acc-steps", type=int, default=400, help="steps of the mixed-vs-FP64 J(t) comparison")
+    ap.add_argument("--contraction", choices=["fp64", "fp32"], default="fp64",
+                    help="wannier workloads: Wannier contraction on the FP64 or the FP32 matrix cores (with "
+                         "--precision mixed the J(t) error is then taken against an FP64-contracted setup)")
     ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
     ap.add_argument("--end-to-end", action="store_true",
                     help="time the drop-in's own loop (observables and output files included) for --steps steps")
@@ -102,7 +105,9 @@ def main():
         nkd = args.nk or 60
         inp = workloads.hbn_tb(wd, nk=nkd, nsteps=max(8000, nsteps_total), dt=0.05, stepper="RK4")
         wname = f"hbn_gen2d_3band_{nkd}x{nkd}_T2_10fs_rk4_dt0.05"
-    s = N.Setup(inp)
+    if args.contraction == "fp32" and args.workload not in ("wannier", "wannier32"):
+        ap.error("--contraction fp32 applies to the wannier workloads")
+    s = N.Setup(inp, fp32_contraction=args.contraction == "fp32")
     info = s.info
     if args.end_to_end:
         s.run(os.path.join(wd, "warm"), max_steps=args.warmup)
@@ -178,7 +183,8 @@ def main():
     cpu = None
     accuracy = None
     if args.precision == "mixed":
-        accuracy = mixed_j_error(s, eng, it0, args.acc_steps)
+        accuracy = mixed_j_error(s, eng, it0, args.acc_steps,
+                                 ref_setup=N.Setup(inp) if args.contraction == "fp32" else None)
     elif not args.no_cpu and rk4 and not info["ncut"]:  # the CPU sample runs the plain RK4 oracle
         cpu = cpu_baseline(s, eng, args.cpu_steps)
 
@@ -198,6 +204,7 @@ def main():
         "config": {"workload": wname,
                    "nk": nk, "bands": nb, "stepper": stepper,
                    "precision": "fp64" if args.precision == "fp64" else "mixed (FP32 operators and products, FP64 state)",
+                   "wannier_contraction": args.contraction if args.workload in ("wannier", "wannier32") else None,
                    "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
                    "device_ms_per_step": dev_ms / args.steps},
         "roofline": roof,
@@ -213,20 +220,24 @@ def currents(ob):
     return ob["j1"][0::2] - 2.0 * ob["j2"][1::2]
 
 
-def mixed_j_error(s, eng, it0, nsteps, every=2):
+def mixed_j_error(s, eng, it0, nsteps, every=2, ref_setup=None):
     """The mixed form's error against the FP64 drop-in path over nsteps from rho0
     in the pulse-peak window, both engines given the same fields: J(t) (every
     `every` steps), the final conduction population, and the harmonic spectrum
-    |FFT(J w)|^2 (Hann window w) of the J(t) series."""
+    |FFT(J w)|^2 (Hann window w) of the J(t) series.  ref_setup: the FP64 path's
+    own tables (an FP64 Wannier contraction) when `s` was built otherwise; both
+    start from their own rho0."""
     from edus_amd import native as N
 
     info = s.info
-    ref = N.Engine(s.problem(), precision=N.PRECISION_FP64)
+    rs = ref_setup or s
+    ref = N.Engine(rs.problem(), precision=N.PRECISION_FP64)
     P0 = s.array("rho0", True).reshape(-1, info["nb"], info["nb"])
+    P0r = rs.array("rho0", True).reshape(-1, info["nb"], info["nb"])
     f, g = s.fields(it0, nsteps)
     jm, jr = [], []
-    for e in (eng, ref):
-        e.set_state(P0)
+    eng.set_state(P0)
+    ref.set_state(P0r)
     obs = {}
     for a in range(0, nsteps, every):
         b = min(a + every, nsteps)

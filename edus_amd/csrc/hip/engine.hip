@@ -65,6 +65,7 @@ struct ConstArgs {
     const int *nbr;      // [n_nbr][nkp]
     const int *eidx;     // packed element e -> i | j << 8 | comp << 16 (wave layout)
     int nk, nkp, nb, nch, ncc, n_nbr, diss, dherm;  // dherm: D_ji = conj(D_ij) (else D_ji = D_ij)
+    int sw, scol;        // wave kernels: strip width and row length of the workgroup order (sw = 0: k order)
     double wx2;
     // Coulomb mean field (small-nb kernel only): the mean-field matrix per k,
     // peak correction against PW0, renormalised H out.
@@ -140,6 +141,25 @@ __device__ __forceinline__ int xcd_block() {
     const int G = gridDim.x, b = blockIdx.x;
     const int x = b & 7, idx = b >> 3, per = G >> 3, rem = G & 7;
     return x * per + min(x, rem) + idx;
+}
+
+// k of the wave kernels' workgroup L when the grid is walked in strips: the
+// nk / scol rows of scol points (k = row * scol + col) are cut into strips of
+// sw columns (the last one narrower if sw does not divide scol), and each strip
+// is walked row by row.  A strip's rows then meet their stencil neighbours on
+// the rows above and below while those are still in L2, where whole rows of
+// large matrices would not fit.  A bijection of [0, nk); sw = 0 keeps k = L.
+constexpr long long kStripL2Bytes = 1 << 20;
+
+__device__ __forceinline__ int strip_k(int L, int nk, int sw, int scol) {
+    if (sw == 0) return L;
+    const int nrow = nk / scol, nfull = scol / sw, per = nrow * sw, s = L / per;
+    if (s < nfull) {
+        const int u = L - s * per;
+        return (u / sw) * scol + s * sw + u % sw;
+    }
+    const int wr = scol - nfull * sw, u = L - nfull * per;
+    return (u / wr) * scol + nfull * sw + u % wr;
 }
 
 // Pair index of (i<j) in row-major upper-triangle order.
@@ -1095,12 +1115,13 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_sta
     extern __shared__ double smem_d[];
     constexpr int TPK = WaveGeom<NBT>::TPK;
     if constexpr (WaveGeom<NBT>::KPB == 1) {
-        stage_wave<NBT, MODE, R>(c, a, xcd_block(), threadIdx.x, smem_d, true);  // xcd_block: a bijection of [0, nk)
+        // xcd_block and strip_k: bijections of [0, nk)
+        stage_wave<NBT, MODE, R>(c, a, strip_k(xcd_block(), c.nk, c.sw, c.scol), threadIdx.x, smem_d, true);
     } else {
         const int sub = threadIdx.x / TPK;
         const int kk = xcd_block() * WaveGeom<NBT>::KPB + sub;
-        stage_wave<NBT, MODE, R>(c, a, min(kk, c.nk - 1), threadIdx.x % TPK, smem_d + sub * wave_lds_doubles<NBT, R>(c.diss),
-                                 kk < c.nk);
+        stage_wave<NBT, MODE, R>(c, a, strip_k(min(kk, c.nk - 1), c.nk, c.sw, c.scol), threadIdx.x % TPK,
+                                 smem_d + sub * wave_lds_doubles<NBT, R>(c.diss), kk < c.nk);
     }
 }
 
@@ -1530,6 +1551,78 @@ __global__ __launch_bounds__(256) void k_wannier_mfma(const cd *__restrict__ PT,
         }
 }
 
+// The contraction in FP32 on the matrix cores (v_mfma_f32_16x16x4f32, FP32
+// accumulation): the phases are formed in FP64 from the exact per-direction
+// tables and rounded once into LDS, H(R) and r(R) likewise; the sums are widened
+// to FP64 on the way out.  Same tiling as k_wannier_mfma; the FP32 instruction
+// holds D[4 (l/16) + i][l%16] in lane l.
+__global__ __launch_bounds__(256) void k_wannier_mfma32(const cd *__restrict__ PT, const cd *__restrict__ Val, int N0, int N1, int N2,
+                                                        int nR, int nm, int nk, int kbase, cd *__restrict__ Out) {
+    __shared__ cf Ph[WMK][WMR + 1];
+    __shared__ cf Vs[WMR][WMN + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k0 = blockIdx.y * WMK, m0 = blockIdx.x * WMN;
+    v4f are[2], aim[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) are[t] = aim[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int r0 = 0; r0 < nR; r0 += WMR) {
+        for (int t = threadIdx.x; t < WMK * WMR; t += 256) {
+            const int kk = t / WMR, rr = t % WMR, k = k0 + kk, r = r0 + rr;
+            const cd ph = (k < nk && r < nR) ? wannier_phase(PT, nR, kbase + k, r, N0, N1, N2) : mk(0.0, 0.0);
+            Ph[kk][rr] = cf{(float)ph.x, (float)ph.y};
+        }
+        for (int t = threadIdx.x; t < WMR * WMN; t += 256) {
+            const int rr = t / WMN, mm = t % WMN, r = r0 + rr, m = m0 + mm;
+            const cd v = (r < nR && m < nm) ? Val[(size_t)r * nm + m] : mk(0.0, 0.0);
+            Vs[rr][mm] = cf{(float)v.x, (float)v.y};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < WMR / 4; ks++) {
+            const cf a = Ph[wave * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const cf b = Vs[ks * 4 + (lane >> 4)][t * 16 + (lane & 15)];
+                are[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, are[t], 0, 0, 0);
+                are[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, b.y, are[t], 0, 0, 0);
+                aim[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, aim[t], 0, 0, 0);
+                aim[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, aim[t], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = k0 + wave * 16 + 4 * (lane >> 4) + i, m = m0 + t * 16 + (lane & 15);
+            if (k < nk && m < nm) Out[(size_t)k * nm + m] = mk((double)are[t][i], (double)aim[t][i]);
+        }
+}
+
+// Contraction output [k][c][i][j] (c = 0 H, 1..3 r) to the caller's layouts,
+// one thread per (k, i, j): H with zero diagonal imaginary parts
+// (Wannier.h:489-492) and D = (r + r^T)/2 with zero diagonal imaginary parts
+// (Wannier.h:577-578, save_HUD.cpp:187-191), [k][i][j][x].
+__global__ void k_wannier_finish(const cd *__restrict__ Out, int kc, int nw, double *__restrict__ Hd, double *__restrict__ Dd) {
+    const int nw2 = nw * nw;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)kc * nw2) return;
+    const int kk = (int)(t / nw2), ij = (int)(t % nw2), i = ij / nw, j = ij % nw;
+    const cd *src = Out + (size_t)kk * 4 * nw2;
+    const cd h = src[ij];
+    Hd[2 * t] = h.x;
+    Hd[2 * t + 1] = (i == j) ? 0.0 : h.y;
+    if (!Dd) return;
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        const cd a = src[(x + 1) * nw2 + i * nw + j], b = src[(x + 1) * nw2 + j * nw + i];
+        const double ai = (i == j) ? 0.0 : a.y, bi = (i == j) ? 0.0 : b.y;
+        Dd[(3 * t + x) * 2] = 0.5 * (a.x + b.x);
+        Dd[(3 * t + x) * 2 + 1] = 0.5 * (ai + bi);
+    }
+}
+
 // ================================================================ Coulomb coefficient generator
 // W(q) = sum_G V(|q+G|) e^{i (q+G).(d_i - d_j)} on a grid of q (Create_Wq_with_G_sum,
 // Coulomb.h:87-175), one thread per q point, the G list streamed through LDS in
@@ -1688,6 +1781,7 @@ struct edus_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
+    int sw = 0, scol = 0;  // strip order of the wave kernels (strip_k)
     int precision = EDUS_PRECISION_FP64;
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
     int *eidx = nullptr;  // packed element -> (i, j, comp) of the wave layout
@@ -1812,7 +1906,7 @@ ConstArgs const_args(const edus_ctx *c) {
     ca.H = c->H; ca.D = c->D; ca.U = c->U; ca.T = c->T; ca.nbr = c->nbr;
     ca.Hf = c->Hf; ca.Df = c->Df; ca.Uf = c->Uf; ca.eidx = c->eidx;
     ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
-    ca.diss = c->diss; ca.dherm = c->dherm; ca.wx2 = c->wx2;
+    ca.diss = c->diss; ca.dherm = c->dherm; ca.wx2 = c->wx2; ca.sw = c->sw; ca.scol = c->scol;
     ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
     ca.cxm = c->cxm; ca.sxm = c->sxm; ca.Tt = c->cT; ca.PW0 = c->cPW0; ca.EH = c->cEH; ca.screen = c->cscreen; ca.Hr = c->cHr;
     ca.Xw = c->cXw;
@@ -1957,6 +2051,21 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
     if (c->precision == EDUS_PRECISION_MIXED && c->soa)
         return fail(set_err(EDUS_ERR_UNSUPPORTED, "mixed precision is implemented for nb >= 5"));
     c->nkp = c->soa ? ((p->nk + 255) / 256) * 256 : p->nk;
+    // Wave kernels: walk the grid in strips (strip_k) when a strip's two previous
+    // rows of packed state, the stencil's reach, fit in kStripL2Bytes while whole
+    // rows would not (512 x 512 k at 32 bands: 64-column strips).
+    if (!c->soa) {
+        const long long n0 = p->nk_dims[0], n1 = p->nk_dims[1], n2 = p->nk_dims[2];
+        if (n0 > 0 && n1 > 0 && n2 > 0 && n0 * n1 * n2 == p->nk) {
+            const long long row_k = 2LL * 8 * nb * nb;  // two rows' bytes per column
+            long long sw = 16;
+            while (2 * sw * row_k <= kStripL2Bytes) sw *= 2;
+            if (sw < n2) {
+                c->sw = (int)sw;
+                c->scol = (int)n2;
+            }
+        }
+    }
     c->dt = p->dt; c->wx2 = p->wx2; c->j_norm = p->j_norm; c->tabs_norm = p->tabs_norm; c->nktot = p->nktot;
     c->wb.assign(p->nbr_wb, p->nbr_wb + (size_t)p->n_nbr * 3);
     for (int i = 0; i < nb * nb; i++)
@@ -2973,8 +3082,8 @@ int edus_coulomb_coefficients(const edus_coulomb_gen *g, double *Ao, double *Bo,
     return 0;
 }
 
-int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR, double *H_out,
-                    double *D_out) {
+static int wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR, double *H_out,
+                     double *D_out, bool fp32) {
     if (!nk3 || !R || !HR || !rR || !H_out || nR < 1 || nw < 1) return set_err(EDUS_ERR_INVALID, "bad Wannier arguments");
     for (int l = 0; l < 3; l++)
         if (nk3[l] < 1) return set_err(EDUS_ERR_INVALID, "bad grid");
@@ -3005,13 +3114,17 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
         }
     }
     int *dR = nullptr;
-    double *dVal = nullptr, *dTw = nullptr, *dOut = nullptr, *dPT = nullptr;
-    auto cleanup = [&]() { (void)hipFree(dR); (void)hipFree(dVal); (void)hipFree(dTw); (void)hipFree(dOut); (void)hipFree(dPT); };
+    double *dVal = nullptr, *dTw = nullptr, *dOut = nullptr, *dPT = nullptr, *dH = nullptr, *dD = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(dR); (void)hipFree(dVal); (void)hipFree(dTw); (void)hipFree(dOut); (void)hipFree(dPT);
+        (void)hipFree(dH); (void)hipFree(dD);
+    };
     const size_t npt = (size_t)nR * (nk3[0] + nk3[1] + nk3[2]);
     const size_t chunk = std::min<long long>(nk, std::max<long long>(64, (1LL << 28) / ((long long)nm * 16)));
     if (hipMalloc(&dR, (size_t)nR * 3 * sizeof(int)) != hipSuccess || hipMalloc(&dVal, val.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&dTw, tw.size() * sizeof(double)) != hipSuccess || hipMalloc(&dOut, chunk * nm * 2 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&dPT, npt * 2 * sizeof(double)) != hipSuccess) {
+        hipMalloc(&dPT, npt * 2 * sizeof(double)) != hipSuccess || hipMalloc(&dH, chunk * nw2 * 2 * sizeof(double)) != hipSuccess ||
+        (D_out && hipMalloc(&dD, chunk * nw2 * 6 * sizeof(double)) != hipSuccess)) {
         cleanup();
         return set_err(EDUS_ERR_NOMEM, "Wannier buffers");
     }
@@ -3023,42 +3136,38 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
     }
     hipLaunchKernelGGL(k_wannier_ptab, dim3((unsigned)std::min<size_t>((npt + 255) / 256, 65536)), dim3(256), 0, 0, (const int *)dR,
                        (const cd *)dTw, nR, nk3[0], nk3[1], nk3[2], offs[1], offs[2], (cd *)dPT);
-    std::vector<double> all(chunk * nm * 2);
     for (long long k0 = 0; k0 < nk; k0 += chunk) {
         const int kc = (int)std::min<long long>(chunk, nk - k0);
         const dim3 grid((nm + WMN - 1) / WMN, (unsigned)((kc + WMK - 1) / WMK));
-        hipLaunchKernelGGL(k_wannier_mfma, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR, nm, kc,
-                           (int)k0, (cd *)dOut);
+        if (fp32)
+            hipLaunchKernelGGL(k_wannier_mfma32, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR, nm,
+                               kc, (int)k0, (cd *)dOut);
+        else
+            hipLaunchKernelGGL(k_wannier_mfma, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR, nm,
+                               kc, (int)k0, (cd *)dOut);
+        const long long nt = (long long)kc * nw2;
+        hipLaunchKernelGGL(k_wannier_finish, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, 0, (const cd *)dOut, kc, nw, dH, dD);
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpy(all.data(), dOut, (size_t)kc * nm * 16, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(H_out + (size_t)k0 * nw2 * 2, dH, (size_t)nt * 16, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && D_out) e = hipMemcpy(D_out + (size_t)k0 * nw2 * 6, dD, (size_t)nt * 48, hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
             cleanup();
             return set_err(EDUS_ERR_HIP, std::string("k_wannier: ") + hipGetErrorString(e));
-        }
-        for (int kk = 0; kk < kc; kk++) {
-            const long long k = k0 + kk;
-            const double *src = &all[(size_t)kk * nm * 2];
-            double *h = H_out + (size_t)k * nw2 * 2;
-            for (int t = 0; t < nw2; t++) { h[2 * t] = src[2 * t]; h[2 * t + 1] = src[2 * t + 1]; }
-            for (int i = 0; i < nw; i++) h[(i * nw + i) * 2 + 1] = 0.0;   // Wannier.h:489-492
-            if (D_out) {
-                double *d = D_out + (size_t)k * nw2 * 6;
-                for (int i = 0; i < nw; i++)
-                    for (int j = 0; j < nw; j++)
-                        for (int x = 0; x < 3; x++) {
-                            const double *a = &src[((x + 1) * nw2 + i * nw + j) * 2];
-                            const double *b = &src[((x + 1) * nw2 + j * nw + i) * 2];
-                            const double ai = (i == j) ? 0.0 : a[1], bi = (i == j) ? 0.0 : b[1];
-                            // (D + D^T)/2 with zero diagonal imaginary parts (Wannier.h:577-578, save_HUD.cpp:187-191)
-                            d[((i * nw + j) * 3 + x) * 2] = 0.5 * (a[0] + b[0]);
-                            d[((i * nw + j) * 3 + x) * 2 + 1] = 0.5 * (ai + bi);
-                        }
-            }
         }
     }
     cleanup();
     g_err.clear();
     return 0;
+}
+
+int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR, double *H_out,
+                    double *D_out) {
+    return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, false);
+}
+
+int edus_wannier_hk_fp32(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR,
+                         double *H_out, double *D_out) {
+    return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, true);
 }
 
 }  // extern "C"

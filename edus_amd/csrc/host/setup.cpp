@@ -124,8 +124,9 @@ void Setup::build(const std::string &input_path, int flags) {
         std::vector<double> HR((size_t)w.nR * nb * nb * 2), rR((size_t)w.nR * nb * nb * 6);
         std::memcpy(HR.data(), w.HR.data(), HR.size() * sizeof(double));
         std::memcpy(rR.data(), w.rR.data(), rR.size() * sizeof(double));
-        int rc = edus_wannier_hk(in.Nk, w.nR, nb, w.R.data(), HR.data(), rR.data(), reinterpret_cast<double *>(H.data()),
-                                 reinterpret_cast<double *>(D.data()));
+        auto *contract = (flags & 2) ? edus_wannier_hk_fp32 : edus_wannier_hk;
+        int rc = contract(in.Nk, w.nR, nb, w.R.data(), HR.data(), rR.data(), reinterpret_cast<double *>(H.data()),
+                          reinterpret_cast<double *>(D.data()));
         if (rc != 0) throw Error(std::string("Wannier contraction on the GPU failed: ") + edus_last_error());
         // U from the eigenvectors of H(k) (Wannier.h:496-542): rows are conj(v_a),
         // batched on the GPU like the contraction.

@@ -101,6 +101,7 @@ hip.edus_get_state.argtypes = [C.c_void_p, _dp]
 hip.edus_set_state.argtypes = [C.c_void_p, _dp]
 hip.edus_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
 hip.edus_wannier_hk.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
+hip.edus_wannier_hk_fp32.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
 hip.edus_last_error.restype = C.c_char_p
 hip.edus_eigh_batched.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp]
 hip.edus_batch_create.argtypes = [C.POINTER(C.POINTER(Problem)), C.c_int32, C.POINTER(C.c_void_p)]
@@ -149,9 +150,11 @@ def _check_hip(rc, what):
 class Setup:
     """Setup tables built by the C++ host from a reference input.txt."""
 
-    def __init__(self, input_path, verify_neighbours=False):
+    def __init__(self, input_path, verify_neighbours=False, fp32_contraction=False):
+        """fp32_contraction: Wannier H(k), r(k) from the FP32 matrix-core contraction."""
         h = C.c_void_p()
-        rc = host.edus_setup_from_input(os.fsencode(input_path), 1 if verify_neighbours else 0, C.byref(h))
+        flags = (1 if verify_neighbours else 0) | (2 if fp32_contraction else 0)
+        rc = host.edus_setup_from_input(os.fsencode(input_path), flags, C.byref(h))
         if rc != 0:
             raise EdusError(host.edus_host_last_error().decode())
         self._h = h
@@ -387,8 +390,9 @@ class Engine:
         return ms.value, n.value
 
 
-def wannier_hk(nk3, R, HR, rR, nw):
-    """GPU Wannier contraction: returns H [nk][nw][nw], D [nk][nw][nw][3] (complex)."""
+def wannier_hk(nk3, R, HR, rR, nw, fp32=False):
+    """GPU Wannier contraction: returns H [nk][nw][nw], D [nk][nw][nw][3] (complex).
+    fp32: the FP32 matrix-core form (edus_wannier_hk_fp32)."""
     nk3 = np.ascontiguousarray(nk3, dtype=np.int32)
     R = np.ascontiguousarray(R, dtype=np.int32)
     HR = np.ascontiguousarray(HR, dtype=np.complex128)
@@ -396,9 +400,10 @@ def wannier_hk(nk3, R, HR, rR, nw):
     nk = int(np.prod(nk3))
     H = np.zeros((nk, nw, nw), dtype=np.complex128)
     D = np.zeros((nk, nw, nw, 3), dtype=np.complex128)
-    rc = hip.edus_wannier_hk(nk3.ctypes.data_as(_ip), len(R), nw, R.ctypes.data_as(_ip), _dptr(HR.view(np.float64)),
-                             _dptr(rR.view(np.float64)), _dptr(H.view(np.float64)), _dptr(D.view(np.float64)))
-    _check_hip(rc, "edus_wannier_hk")
+    fn = hip.edus_wannier_hk_fp32 if fp32 else hip.edus_wannier_hk
+    rc = fn(nk3.ctypes.data_as(_ip), len(R), nw, R.ctypes.data_as(_ip), _dptr(HR.view(np.float64)),
+            _dptr(rR.view(np.float64)), _dptr(H.view(np.float64)), _dptr(D.view(np.float64)))
+    _check_hip(rc, "edus_wannier_hk_fp32" if fp32 else "edus_wannier_hk")
     return H, D
 
 

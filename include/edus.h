@@ -243,6 +243,14 @@ int edus_coulomb_coefficients(const edus_coulomb_gen *g, double *A, double *B, d
 int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R,
                     const double *HR, const double *rR, double *H_out, double *D_out);
 
+/* The same contraction with FP32 products on the matrix cores and FP32 sums
+ * (the phases are formed in FP64 and rounded once, as are H(R) and r(R)); the
+ * outputs are FP64 arrays holding the FP32 results.  The mixed-precision form of
+ * Wannier::energy / dipolex/y/z (headers/Wannier.h:278-304, 336-471); each
+ * element carries at most (2 nR + 4) 2^-24 sum_R |X(R)_mn| of error. */
+int edus_wannier_hk_fp32(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R,
+                         const double *HR, const double *rR, double *H_out, double *D_out);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char *edus_last_error(void);
 

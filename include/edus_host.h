@@ -52,7 +52,8 @@ typedef struct edus_setup_info {
 /* Parses `input_path` (reference grammar, headers/ReadInput.h:332-1234) and builds
  * every table.  Wannier seeds are resolved relative to the input file's directory.
  * flags: bit 0 = also verify the closed-form neighbour table against the
- * reference's brute-force search (O(Nk^2), small grids only). */
+ * reference's brute-force search (O(Nk^2), small grids only); bit 1 = run the
+ * Wannier contraction in FP32 (edus_wannier_hk_fp32), for mixed-precision runs. */
 int edus_setup_from_input(const char *input_path, int32_t flags, edus_setup **out);
 void edus_setup_free(edus_setup *s);
 int edus_setup_get_info(const edus_setup *s, edus_setup_info *info);

@@ -158,6 +158,32 @@ def test_wannier_contraction_matches_direct_sum(tmp_path, nw):
     assert np.all(np.diff(ev, axis=1) >= -1e-12)
 
 
+@pytest.mark.parametrize("nw", [5, 16])
+def test_wannier_contraction_fp32_within_rounding_bound(tmp_path, nw):
+    """FP32 matrix-core contraction (phases formed in FP64, rounded once) against
+    the FP64 direct sum.  Bound per element: (2 nR + 4) 2^-24 sum_R |X(R)_mn|
+    (inputs rounded once, 2 nR real products summed per part; |phase| = 1), times
+    sqrt(2) for the modulus of a complex error; a wrong phase, sign or index errs
+    by the size of X itself."""
+    s = _wannier_setup(str(tmp_path), nw, rmax=3)
+    seed = [f for f in os.listdir(tmp_path) if f.endswith("_tb.dat")][0]
+    R, HR, rR = _read_tb(os.path.join(tmp_path, seed), nw)
+    kpt = O.kgrid(1, 12, 12)
+    oH, oD = O.wannier_tables(kpt, R, HR, rR, nw)
+    H, D = N.wannier_hk(np.array([1, 12, 12]), R, HR, rR, nw, fp32=True)
+    g = np.sqrt(2.0) * (2 * len(R) + 4) * 2.0 ** -24
+    bH = g * np.sum(np.abs(HR), axis=0)
+    bR = g * np.sum(np.abs(rR), axis=0)
+    bD = 0.5 * (bR + np.swapaxes(bR, 0, 1))
+    eH, eD = np.abs(H - oH), np.abs(D - oD)
+    print(f"nw={nw} max err H {eH.max():.3e} (bound {bH.max():.3e}), D {eD.max():.3e} (bound {bD.max():.3e})", flush=True)
+    assert np.all(eH <= bH[None]) and np.all(eD <= bD[None])
+    assert eH.max() > 1e-12 * np.abs(oH).max()  # the FP32 form is the one that ran
+    # the setup flag routes the model build through the same contraction
+    s32 = N.Setup(os.path.join(tmp_path, "input.txt"), fp32_contraction=True)
+    np.testing.assert_array_equal(s32.array("H", True).reshape(H.shape), H)
+
+
 @pytest.mark.parametrize("nw,t2", [(5, 0.0), (8, 10.0), (12, 0.0), (16, 0.0), (16, 10.0), (20, 10.0), (32, 0.0), (32, 10.0)])
 def test_rhs_wave_kernel_matches_oracle(tmp_path, nw, t2):
     s = _wannier_setup(str(tmp_path), nw, t2=t2)
@@ -169,6 +195,22 @@ def test_rhs_wave_kernel_matches_oracle(tmp_path, nw, t2):
         pv = eng.rhs(np.array(field6), on)
         ref = osys["sys"].rhs(P, np.array(field6), on)
         assert np.max(np.abs(pv - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("nw,nk", [(32, 80), (20, 150)])
+def test_rhs_wave_kernel_strip_order_matches_oracle(tmp_path, nw, nk):
+    """Grids whose rows are longer than the engine's strip width (64 columns at
+    32 bands, 128 at 20): the workgroups walk 64- or 128-column strips row by row
+    plus a narrower last strip, and every k must still be computed exactly once."""
+    s = _wannier_setup(str(tmp_path), nw, nk=nk, rmax=1)
+    osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
+    eng = N.Engine(s.problem())
+    P = random_hermitian_state(s.array("rho0", True).reshape(-1, nw, nw), 0.05, nw)
+    eng.set_state(P)
+    field6 = np.array([1e-3, 2e-3, -3e-3, 4e-4, 0, 0])
+    pv = eng.rhs(field6, True)
+    ref = osys["sys"].rhs(P, field6, True)
+    assert np.max(np.abs(pv - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
 @pytest.mark.parametrize("stepper,order", [("RK4", 4), ("AB", 3), ("AB", 5), ("Taylor", 4), ("Taylor", 6)])
