@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1349,6 +1350,41 @@ __global__ __launch_bounds__(256) void k_to_float(const double *src, float *dst,
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = (float)src[i];
 }
 
+// Engine creation packs the host's reference-layout tables on the device: a
+// chunk of kc k is copied as is, then one thread per (k, i, j) writes the upper
+// triangle of component x (complex stride cs per element) to dst[k sk + e se]
+// (e = i on the diagonal, real part only; nb + 2 pair(i, j) and + 1 above it).
+__global__ void k_pack(const double *__restrict__ src, long long n, int nb, int cs, int x, size_t sk, size_t se,
+                       double *__restrict__ dst) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int nb2 = nb * nb, ij = (int)(t % nb2), i = ij / nb, j = ij % nb;
+    if (j < i) return;
+    const size_t k = (size_t)(t / nb2);
+    const double re = src[((size_t)t * cs + x) * 2], im = src[((size_t)t * cs + x) * 2 + 1];
+    if (i == j) {
+        dst[k * sk + (size_t)i * se] = re;
+    } else {
+        const size_t e = nb + 2 * (size_t)pair_index(i, j, nb);
+        dst[k * sk + e * se] = re;
+        dst[k * sk + (e + 1) * se] = im;
+    }
+}
+
+// D on the core <-> non-core block (rows i < nch, columns j >= nch) for the
+// transient absorption, [k][3][nch (nb - nch)] complex, from a chunk of the
+// reference [k][nb][nb][3] table.
+__global__ void k_pack_dt(const double *__restrict__ src, int kc, int nb, int nch, double *__restrict__ dst) {
+    const int nblk = nch * (nb - nch);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)kc * 3 * nblk) return;
+    const int r = (int)(t % (3 * nblk)), x = r / nblk, q = r % nblk, i = q / (nb - nch), j = nch + q % (nb - nch);
+    const size_t k = (size_t)(t / (3 * nblk));
+    const double *d = src + (((k * nb + i) * nb + j) * 3 + x) * 2;
+    dst[2 * t] = d[0];
+    dst[2 * t + 1] = d[1];
+}
+
 __global__ void k_build_q(const double *Hp, const double *Dp, int nk, int nb, size_t sk, size_t se, size_t dxs, int dherm, double *Q) {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)nk * 3 * nb * nb) return;
@@ -1867,6 +1903,30 @@ void unpack_state(const edus_ctx *c, const std::vector<double> &in, double *full
     }
 }
 
+// A host table of nk rows (each nb x nb elements of cs complex values, the
+// reference layout) copied in chunks of stage_k rows through the device buffer
+// `stage`; pack(k0, kc) launches the packing of each chunk, in stream order.
+template <class F>
+int upload_packed(edus_ctx *c, const double *src, int cs, double *stage, int stage_k, F pack) {
+    const size_t per = (size_t)c->nb * c->nb * cs * 2;
+    for (int k0 = 0; k0 < c->nk; k0 += stage_k) {
+        const int kc = std::min(stage_k, c->nk - k0);
+        HIP_TRY(hipMemcpyAsync(stage, src + (size_t)k0 * per, (size_t)kc * per * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        pack(k0, kc);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+void launch_pack(const edus_ctx *c, const double *stage, int kc, int cs, int x, size_t sk, size_t se, double *dst) {
+    const long long n = (long long)kc * c->nb * c->nb;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, stage, n, c->nb, cs, x, sk, se, dst);
+}
+
+struct DeviceFree {
+    void operator()(double *p) const { (void)hipFree(p); }
+};
+
 int upload(edus_ctx *c, double *dst, const std::vector<double> &v) {
     HIP_TRY(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     return 0;
@@ -2105,48 +2165,52 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
             hipStreamSynchronize(c->stream) != hipSuccess)
             return fail(rc ? rc : set_err(EDUS_ERR_HIP, "element table upload"));
     }
-    // Operators in the state layout.
-    std::vector<double> hp;
-    pack_state(c, p->H, hp);
-    if ((rc = dalloc(c, &c->H, hp.size())) || (rc = upload(c, c->H, hp))) return fail(rc);
+    // Operators in the state layout, packed on the device from chunks of the
+    // host tables (sk, sek: k and element strides of the state layout).
+    const size_t sk = c->soa ? 1 : (size_t)c->np, sek = c->soa ? (size_t)c->nkp : 1;
+    const int stage_k = (int)std::max<long long>(1, std::min<long long>(p->nk, (1LL << 25) / ((long long)nb * nb * 6)));
+    std::unique_ptr<double, DeviceFree> stage;
     {
-        std::vector<double> dall, one, tmp((size_t)p->nk * nb * nb * 2);
-        for (int x = 0; x < 3; x++) {
-            for (size_t t = 0; t < (size_t)p->nk * nb * nb; t++) {
-                tmp[2 * t] = p->D[t * 6 + 2 * x];
-                tmp[2 * t + 1] = p->D[t * 6 + 2 * x + 1];
-            }
-            pack_state(c, tmp.data(), one);
-            dall.insert(dall.end(), one.begin(), one.end());
-        }
-        if ((rc = dalloc(c, &c->D, dall.size())) || (rc = upload(c, c->D, dall))) return fail(rc);
-        // D on the core <-> non-core block for the transient absorption.
+        double *q = nullptr;
+        if (hipMalloc(&q, (size_t)stage_k * nb * nb * 6 * sizeof(double)) != hipSuccess)
+            return fail(set_err(EDUS_ERR_NOMEM, "staging buffer"));
+        stage.reset(q);
+    }
+    double *st = stage.get();
+    if ((rc = dalloc(c, &c->H, se)) || (rc = dalloc(c, &c->D, 3 * se))) return fail(rc);
+    if (c->soa && (hipMemsetAsync(c->H, 0, se * sizeof(double), c->stream) != hipSuccess ||
+                   hipMemsetAsync(c->D, 0, 3 * se * sizeof(double), c->stream) != hipSuccess))
+        return fail(set_err(EDUS_ERR_HIP, "memset operators"));  // padding columns of the SoA layout
+    if ((rc = upload_packed(c, p->H, 1, st, stage_k, [&](int k0, int kc) { launch_pack(c, st, kc, 1, 0, sk, sek, c->H + k0 * sk); })))
+        return fail(rc);
+    {
+        // D: three packed components [x][state], and its core <-> non-core block.
         const int nblk = c->nch * (nb - c->nch);
-        std::vector<double> dt((size_t)p->nk * 3 * std::max(nblk, 1) * 2, 0.0);
-        for (int k = 0; k < p->nk; k++)
-            for (int x = 0; x < 3; x++)
-                for (int t = 0; t < nblk; t++) {
-                    const int i = t / (nb - c->nch), j = c->nch + t % (nb - c->nch);
-                    const double *d = p->D + (((size_t)k * nb + i) * nb + j) * 6 + 2 * x;
-                    dt[(((size_t)k * 3 + x) * nblk + t) * 2] = d[0];
-                    dt[(((size_t)k * 3 + x) * nblk + t) * 2 + 1] = d[1];
-                }
-        if ((rc = dalloc(c, &c->DT, dt.size())) || (rc = upload(c, c->DT, dt))) return fail(rc);
+        const size_t ndt = (size_t)p->nk * 3 * std::max(nblk, 1) * 2;
+        if ((rc = dalloc(c, &c->DT, ndt))) return fail(rc);
+        if (hipMemsetAsync(c->DT, 0, ndt * sizeof(double), c->stream) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "memset DT"));
+        rc = upload_packed(c, p->D, 3, st, stage_k, [&](int k0, int kc) {
+            for (int x = 0; x < 3; x++) launch_pack(c, st, kc, 3, x, sk, sek, c->D + x * se + k0 * sk);
+            if (nblk > 0) {
+                const long long n = (long long)kc * 3 * nblk;
+                hipLaunchKernelGGL(k_pack_dt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (const double *)st, kc, nb, c->nch,
+                                   c->DT + (size_t)k0 * 3 * nblk * 2);
+            }
+        });
+        if (rc) return fail(rc);
         // Q_x = H D_x^+ for Current2, built on the device from the packed operators.
         const size_t nq = (size_t)p->nk * 3 * nb * nb;
         if ((rc = dalloc(c, &c->Q, nq * 2))) return fail(rc);
-        {
-            const size_t sk = c->soa ? 1 : (size_t)c->np, sek = c->soa ? (size_t)c->nkp : 1;
-            const size_t dxs = c->soa ? (size_t)c->np * c->nkp : (size_t)c->nk * c->np;
-            hipLaunchKernelGGL(k_build_q, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, (const double *)c->H,
-                               (const double *)c->D, p->nk, nb, sk, sek, dxs, c->dherm, c->Q);
-            if (hipGetLastError() != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "k_build_q launch"));
-        }
+        hipLaunchKernelGGL(k_build_q, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, (const double *)c->H,
+                           (const double *)c->D, p->nk, nb, sk, sek, se, c->dherm, c->Q);
+        if (hipGetLastError() != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "k_build_q launch"));
     }
     {
-        std::vector<double> u(p->U, p->U + (size_t)p->nk * nb * nb * 2), t(p->T, p->T + (size_t)nb * nb);
-        if ((rc = dalloc(c, &c->U, u.size())) || (rc = upload(c, c->U, u))) return fail(rc);
-        if ((rc = dalloc(c, &c->T, t.size())) || (rc = upload(c, c->T, t))) return fail(rc);
+        const size_t nu = (size_t)p->nk * nb * nb * 2;
+        if ((rc = dalloc(c, &c->U, nu)) || (rc = dalloc(c, &c->T, (size_t)nb * nb))) return fail(rc);
+        if (hipMemcpyAsync(c->U, p->U, nu * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->T, p->T, (size_t)nb * nb * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return fail(set_err(EDUS_ERR_HIP, "upload U, T"));
     }
     if (c->precision == EDUS_PRECISION_MIXED) {
         const size_t nh = se, nd = 3 * se, nu = (size_t)p->nk * nb * nb * 2;
@@ -2160,21 +2224,12 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
     }
     if (p->gradH) {
         // packed Hermitian gradient of H, [k][3][NP] (always k-major)
-        std::vector<double> ge((size_t)p->nk * 3 * c->np);
-        for (int k = 0; k < p->nk; k++)
-            for (int x = 0; x < 3; x++) {
-                double *g = &ge[((size_t)k * 3 + x) * c->np];
-                for (int i = 0; i < nb; i++) {
-                    g[i] = p->gradH[(((size_t)k * nb + i) * nb + i) * 6 + 2 * x];
-                    for (int j = i + 1; j < nb; j++) {
-                        const int pp = pair_index(i, j, nb);
-                        const double *v = p->gradH + (((size_t)k * nb + i) * nb + j) * 6 + 2 * x;
-                        g[nb + 2 * pp] = v[0];
-                        g[nb + 2 * pp + 1] = v[1];
-                    }
-                }
-            }
-        if ((rc = dalloc(c, &c->GE, ge.size())) || (rc = upload(c, c->GE, ge))) return fail(rc);
+        const size_t np3 = (size_t)3 * c->np;
+        if ((rc = dalloc(c, &c->GE, (size_t)p->nk * np3))) return fail(rc);
+        rc = upload_packed(c, p->gradH, 3, st, stage_k, [&](int k0, int kc) {
+            for (int x = 0; x < 3; x++) launch_pack(c, st, kc, 3, x, np3, 1, c->GE + (size_t)k0 * np3 + (size_t)x * c->np);
+        });
+        if (rc) return fail(rc);
     }
     {
         std::vector<int> nbr((size_t)std::max(p->n_nbr, 1) * c->nkp, 0);
@@ -2187,9 +2242,12 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
     c->obs_blocks = std::max(1, std::min(kObsMaxBlocks, p->nk));
     if ((rc = dalloc(c, &c->part, (size_t)c->obs_blocks * (nb + 17))) || (rc = dalloc(c, &c->obs, nb + 17))) return fail(rc);
     {
-        std::vector<double> st;
-        pack_state(c, p->rho0, st);
-        if ((rc = upload(c, c->P0, st))) return fail(rc);
+        if (c->soa && hipMemsetAsync(c->P0, 0, se * sizeof(double), c->stream) != hipSuccess)
+            return fail(set_err(EDUS_ERR_HIP, "memset state"));
+        if ((rc = upload_packed(c, p->rho0, 1, st, stage_k, [&](int k0, int kc) { launch_pack(c, st, kc, 1, 0, sk, sek, c->P0 + k0 * sk); })))
+            return fail(rc);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "table packing"));
+        stage.reset();
         if (c->soa) {  // padding columns stay zero in every state buffer
             if (hipMemsetAsync(c->ACC, 0, se * sizeof(double), c->stream) != hipSuccess ||
                 hipMemsetAsync(c->YA, 0, se * sizeof(double), c->stream) != hipSuccess ||
