@@ -127,7 +127,7 @@ const double *edus_setup_array(const edus_setup *h, const char *name, int64_t *c
     if (!h || !name) return nullptr;
     const edus::Setup &s = h->s;
     auto ret = [&](const double *p, size_t n) { if (count) *count = (int64_t)n; return p; };
-    auto retc = [&](const std::vector<edus::cplx> &v) { return ret(reinterpret_cast<const double *>(v.data()), v.size() * 2); };
+    auto retc = [&](const auto &v) { return ret(reinterpret_cast<const double *>(v.data()), v.size() * 2); };
     std::string n = name;
     if (n == "kpt") return ret(s.kpt.data(), s.kpt.size());
     if (n == "a") return ret(&s.in.a[0][0], 9);

@@ -10,13 +10,39 @@
 #include <array>
 #include <complex>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace edus {
 
 using cplx = std::complex<double>;
+
+// Allocator whose value-initialisation does nothing: the per-k tables (up to
+// 10^10 bytes) are sized without a serial zero fill and then filled by parallel
+// loops (first touch) or copies.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *) noexcept {}
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+using ctable = std::vector<cplx, UninitAlloc<cplx>>;
+
+// Sizes a table to n elements and zeroes it with all threads.
+void zero_table(ctable &v, size_t n);
 
 // Unit conversions with the digits of headers/Constants.h:14-37 (they fix
 // every time, energy and field value the reference writes).
@@ -174,8 +200,8 @@ struct Setup {
     std::vector<int32_t> nbr;      // [nk][n_nbr]
     std::vector<double> nbr_wb;    // [n_nbr][3]
     std::vector<double> bvec_flat; // [n_nbr][3]
-    std::vector<cplx> H, U, rho0;  // [nk][nb][nb]
-    std::vector<cplx> D, gradH;    // [nk][nb][nb][3]
+    ctable H, U, rho0;             // [nk][nb][nb]
+    ctable D, gradH;               // [nk][nb][nb][3]
     std::vector<double> T;         // [nb][nb]
     std::vector<double> tau_inv;   // Taylor
     int n_diss = 0;

@@ -10,10 +10,13 @@
 #include "host.hpp"
 
 #include <cctype>
+#include <charconv>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 
 namespace edus {
@@ -156,44 +159,80 @@ cplx TBModel::dipole(int ic, int jc, int x, const double k[3]) const {
 }
 
 // ------------------------------------------------------------------ Wannier
+namespace {
+// The numbers of a file read whole and taken one at a time (std::from_chars:
+// correctly rounded like stream extraction, locale-independent, and an order of
+// magnitude faster on the 10^7 numbers of a large seedname_tb.dat).
+struct NumberStream {
+    std::string buf;
+    const char *p = nullptr, *end = nullptr;
+    bool ok = true;
+    void skip_lines(int n) {
+        for (; n > 0 && p < end; n--) {
+            const char *q = static_cast<const char *>(std::memchr(p, '\n', (size_t)(end - p)));
+            p = q ? q + 1 : end;
+        }
+    }
+    double next() {
+        while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) p++;
+        if (p < end && *p == '+') p++;
+        double v = 0.0;
+        const auto r = std::from_chars(p, end, v);
+        if (r.ec != std::errc() || r.ptr == p) {
+            ok = false;
+            return 0.0;
+        }
+        p = r.ptr;
+        return v;
+    }
+};
+}  // namespace
+
 void WannierModel::read_tb(const std::string &path) {
-    std::ifstream f(path);
-    if (!f) throw Error("No path " + path + " file found.");
-    std::string line;
-    for (int i = 0; i < 4; i++) std::getline(f, line);
-    f >> nw >> nR;
-    if (!f || nw <= 0 || nR <= 0) throw Error("malformed wannier90 file " + path);
+    NumberStream f;
+    {
+        std::ifstream in(path, std::ios::binary);
+        if (!in) throw Error("No path " + path + " file found.");
+        f.buf.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+    }
+    f.p = f.buf.data();
+    f.end = f.p + f.buf.size();
+    f.skip_lines(4);
+    const double nw_d = f.next(), nR_d = f.next();
+    nw = (int)nw_d;
+    nR = (int)nR_d;
+    if (!f.ok || nw <= 0 || nR <= 0) throw Error("malformed wannier90 file " + path);
     ndegen.resize(nR);
-    for (int i = 0; i < nR; i++) f >> ndegen[i];
+    for (int i = 0; i < nR; i++) ndegen[i] = f.next();
     R.assign((size_t)nR * 3, 0);
     HR.assign((size_t)nR * nw * nw, 0.);
     rR.assign((size_t)nR * nw * nw * 3, 0.);
     const double eV = Units::energy_eV_au(), A = Units::space_A_au();
     // H block: per R the vector, then lines "m n Re Im" with m fastest.
     for (int iR = 0; iR < nR; iR++) {
-        double Rv[3];
-        f >> Rv[0] >> Rv[1] >> Rv[2];
-        for (int c = 0; c < 3; c++) R[(size_t)iR * 3 + c] = (int32_t)std::lround(Rv[c]);
+        for (int c = 0; c < 3; c++) R[(size_t)iR * 3 + c] = (int32_t)std::lround(f.next());
         for (int in = 0; in < nw; in++)
             for (int im = 0; im < nw; im++) {
-                double m_, n_, hr, hi;
-                f >> m_ >> n_ >> hr >> hi;
+                f.next();
+                f.next();
+                const double hr = f.next(), hi = f.next();
                 HR[((size_t)iR * nw + im) * nw + in] = cplx(hr / ndegen[iR] * eV, hi / ndegen[iR] * eV);
             }
     }
     // Position block: per R the vector, then "m n xr xi yr yi zr zi".
     for (int iR = 0; iR < nR; iR++) {
-        double Rv[3];
-        f >> Rv[0] >> Rv[1] >> Rv[2];
+        for (int c = 0; c < 3; c++) f.next();
         for (int in = 0; in < nw; in++)
             for (int im = 0; im < nw; im++) {
-                double m_, n_, v[6];
-                f >> m_ >> n_ >> v[0] >> v[1] >> v[2] >> v[3] >> v[4] >> v[5];
+                f.next();
+                f.next();
+                double v[6];
+                for (int q = 0; q < 6; q++) v[q] = f.next();
                 for (int c = 0; c < 3; c++)
                     rR[(((size_t)iR * nw + im) * nw + in) * 3 + c] = cplx(v[2 * c] / ndegen[iR] * A, v[2 * c + 1] / ndegen[iR] * A);
             }
     }
-    if (!f) throw Error("wannier90 file ended early: " + path);
+    if (!f.ok) throw Error("wannier90 file ended early: " + path);
 }
 
 namespace {

@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iomanip>
 #include <sys/stat.h>
@@ -22,7 +23,23 @@ namespace edus {
 
 static inline size_t mi(int nb, int k, int i, int j) { return ((size_t)k * nb + i) * nb + j; }
 
+void zero_table(ctable &v, size_t n) {
+    v.clear();
+    v.resize(n);
+    cplx *p = v.data();
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) p[i] = cplx(0., 0.);
+}
+
 void Setup::build(const std::string &input_path, int flags) {
+    // EDUS_SETUP_TIMING=1 prints the wall time of each setup phase to stderr.
+    const bool timing = std::getenv("EDUS_SETUP_TIMING") != nullptr;
+    double t_last = omp_get_wtime();
+    auto mark = [&](const char *phase) {
+        const double t = omp_get_wtime();
+        if (timing) std::fprintf(stderr, "[edus setup] %-26s %8.3f s\n", phase, t - t_last);
+        t_last = t;
+    };
     in = parse_input(input_path);
     if (in.coulomb && in.coulomb_diag_basis)
         throw Error("Coulomb_diag_basis is not supported by this build (Wannier basis only)");
@@ -82,10 +99,12 @@ void Setup::build(const std::string &input_path, int flags) {
             }
     }
 
+    mark("input, k-grid, neighbours");
     // ---- model tables (save_HUD.cpp:142-210)
-    H.assign((size_t)nk * nb * nb, 0.);
-    U.assign((size_t)nk * nb * nb, 0.);
-    D.assign((size_t)nk * nb * nb * 3, 0.);
+    zero_table(H, (size_t)nk * nb * nb);
+    zero_table(U, (size_t)nk * nb * nb);
+    zero_table(D, (size_t)nk * nb * nb * 3);
+    mark("table allocation");
     if (in.iMode == "TB") {
         TBModel tb;
         tb.init(in.TBtype, nch, nb, in.hopping);
@@ -120,6 +139,7 @@ void Setup::build(const std::string &input_path, int flags) {
         std::string seed = in.ref_coord;
         std::string path = (seed.size() && seed[0] == '/') ? seed + "_tb.dat" : in.base_dir + "/" + seed + "_tb.dat";
         w.read_tb(path);
+        mark("_tb.dat read");
         if (w.nw != nb) throw Error("bands in input (" + std::to_string(nb) + ") differ from wannier90 nwannier (" + std::to_string(w.nw) + ")");
         std::vector<double> HR((size_t)w.nR * nb * nb * 2), rR((size_t)w.nR * nb * nb * 6);
         std::memcpy(HR.data(), w.HR.data(), HR.size() * sizeof(double));
@@ -128,12 +148,14 @@ void Setup::build(const std::string &input_path, int flags) {
         int rc = contract(in.Nk, w.nR, nb, w.R.data(), HR.data(), rR.data(), reinterpret_cast<double *>(H.data()),
                           reinterpret_cast<double *>(D.data()));
         if (rc != 0) throw Error(std::string("Wannier contraction on the GPU failed: ") + edus_last_error());
+        mark("Wannier contraction");
         // U from the eigenvectors of H(k) (Wannier.h:496-542): rows are conj(v_a),
         // batched on the GPU like the contraction.
         std::vector<double> ev((size_t)nk * nb);
         rc = edus_eigh_batched(nk, nb, reinterpret_cast<const double *>(H.data()), ev.data(), reinterpret_cast<double *>(U.data()));
         if (rc != 0) throw Error(std::string("eigensolve on the GPU failed: ") + edus_last_error());
     }
+    mark("model tables");
     // Scissor: conduction_shift on bands ic >= Nb[1] in the eigenbasis (save_HUD.cpp:147-181).
     if (in.change_gap_constant > 1e-16) {
 #pragma omp parallel for schedule(static)
@@ -154,7 +176,7 @@ void Setup::build(const std::string &input_path, int flags) {
     if (in.coulomb && !in.coulomb_read) generate_coulomb(disp_at_reference_point());
     // GradientEnergy for Current1 (save_HUD.cpp:236-254): sum_shell,b w*b_x*(H(k+b)-H(k)).
     if (in.iCurrent) {
-        gradH.assign((size_t)nk * nb * nb * 3, 0.);
+        zero_table(gradH, (size_t)nk * nb * nb * 3);
 #pragma omp parallel for schedule(static)
         for (int ik = 0; ik < nk; ik++)
             for (int x = 0; x < 3; x++)
@@ -172,6 +194,7 @@ void Setup::build(const std::string &input_path, int flags) {
                     }
     }
 
+    mark("scissor, Coulomb, grad H");
     // ---- decoherence matrix (InitParameters.h:203-224) and its nonzero count
     T.assign((size_t)nb * nb, 0.);
     for (int i = nch; i < nb; i++)
@@ -187,7 +210,7 @@ void Setup::build(const std::string &input_path, int flags) {
         if (std::fabs(t) > 1e-24) n_diss++;
 
     // ---- initial density matrix, stored transposed: P0[a][b] = sum_{i<ncc} U[i][a] conj(U[i][b])
-    rho0.assign((size_t)nk * nb * nb, 0.);
+    zero_table(rho0, (size_t)nk * nb * nb);
 #pragma omp parallel for schedule(static)
     for (int ik = 0; ik < nk; ik++)
         for (int a = 0; a < nb; a++)
@@ -197,6 +220,7 @@ void Setup::build(const std::string &input_path, int flags) {
                 rho0[mi(nb, ik, a, b)] = acc;
             }
 
+    mark("T, rho0");
     // ---- stepper and time axis (const_dt_evolution.cpp:164-172, runge_kutta_init.cpp:356-370)
     // The adaptive loop runs Taylor or else RK4 (dyn_dt_evolution.cpp:51-58).
     if (in.taylor) { stepper = EDUS_STEPPER_TAYLOR; order = in.solver_order; }
