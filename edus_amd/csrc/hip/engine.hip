@@ -435,7 +435,7 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
 
 template <int NB, int MODE, bool DISS>
 __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
-    stage_small<NB, MODE, DISS>(c, a, xcd_block() * 256 + threadIdx.x);
+    stage_small<NB, MODE, DISS>(c, a, xcd_block() * blockDim.x + threadIdx.x);
 }
 
 // ================================================================ Coulomb mean field
@@ -1982,14 +1982,17 @@ int launch_stage(edus_ctx *c, StageArgs a) {
         if (rc) return rc;
     }
     if (c->soa) {
-        const int blocks = (c->nk + 255) / 256;
+        // A grid of fewer k than 256 per CU runs 64-thread workgroups, so that
+        // more CUs share it (the k past nk, up to the padded nkp, are computed
+        // and never read).
+        const int tpb = c->nk < 256 * 256 ? 64 : 256, blocks = (c->nk + tpb - 1) / tpb;
         auto go = [&](auto dtag) {
             constexpr bool DISS = decltype(dtag)::value;
             switch (c->nb) {
-                case 1: hipLaunchKernelGGL((k_stage_small<1, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
-                case 2: hipLaunchKernelGGL((k_stage_small<2, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
-                case 3: hipLaunchKernelGGL((k_stage_small<3, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
-                default: hipLaunchKernelGGL((k_stage_small<4, MODE, DISS>), dim3(blocks), dim3(256), 0, c->stream, ca, a); break;
+                case 1: hipLaunchKernelGGL((k_stage_small<1, MODE, DISS>), dim3(blocks), dim3(tpb), 0, c->stream, ca, a); break;
+                case 2: hipLaunchKernelGGL((k_stage_small<2, MODE, DISS>), dim3(blocks), dim3(tpb), 0, c->stream, ca, a); break;
+                case 3: hipLaunchKernelGGL((k_stage_small<3, MODE, DISS>), dim3(blocks), dim3(tpb), 0, c->stream, ca, a); break;
+                default: hipLaunchKernelGGL((k_stage_small<4, MODE, DISS>), dim3(blocks), dim3(tpb), 0, c->stream, ca, a); break;
             }
         };
         if (c->nb < 1 || c->nb > 4) return set_err(EDUS_ERR_UNSUPPORTED, "no small kernel for this band count");
