@@ -463,26 +463,49 @@ struct CoulArgs {
 
 constexpr int kCoulThreads = 256;
 
-// S_y[i1][n][e] = sum_i2 y_n(i2) P_e(i1, i2): one thread per output, the column
-// loop unrolled so that its loads overlap.  (Variants that split the sum over
-// lanes or staged the row in LDS measured slower at 300 x 300: the pass is
-// bound by the latency and address throughput of its ~100k short sums.)
-__global__ __launch_bounds__(256) void k_coul_s1(CoulArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int total = a.N1 * a.ncut * a.NP;
-    if (t >= total) return;
-    const int e = t % a.NP, n = (t / a.NP) % a.ncut, i1 = t / (a.NP * a.ncut);
-    const double *row = a.Y + (size_t)e * a.nkp + (size_t)i1 * a.N2;
-    const double *cy = a.cyn + (size_t)n * a.N2, *sy = a.syn + (size_t)n * a.N2;
-    double sc = 0.0, ss = 0.0;
-#pragma unroll 8
-    for (int i2 = 0; i2 < a.N2; i2++) {
-        const double p = row[i2];
-        sc += cy[i2] * p;
-        ss += sy[i2] * p;
+// S_y[i1][n][e] = sum_i2 y_n(i2) P_e(i1, i2) as one product on the FP64 matrix
+// cores (a thread per output with a sequential sum took 28 us at 300 x 300,
+// bound by the address throughput of its uncoalesced loads): rows r = e N1 + i1
+// (the state row Y[e][i1 N2 + i2], contiguous in i2), columns c = q ncut + n
+// (q = 0: cos, 1: sin), K = N2.  One wave per 16 x 16 tile walks K four at a
+// time with v_mfma_f64_16x16x4f64 (lane l feeds A[row l%16][k l/16] and
+// B[k l/16][col l%16] and ends with C[l/16 + 4 i][l%16]); out-of-range rows,
+// columns and k are fed zeros.
+__global__ __launch_bounds__(64) void k_coul_s1_mfma(CoulArgs a) {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x, r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
+    const int nrow = a.NP * a.N1, ncol = 2 * a.ncut;
+    const int ra = r0 + (lane & 15), cb = c0 + (lane & 15), kq = lane >> 4;
+    const bool rok = ra < nrow, cok = cb < ncol;
+    const double *yrow = a.Y + (rok ? (size_t)(ra / a.N1) * a.nkp + (size_t)(ra % a.N1) * a.N2 : 0);
+    const int q = cok ? cb / a.ncut : 0, n = cok ? cb % a.ncut : 0;
+    const double *trig = (q ? a.syn : a.cyn) + (size_t)n * a.N2;
+    v4 acc = {0.0, 0.0, 0.0, 0.0};
+    // K in blocks of 32: the block's 16 loads (at a clamped k, always in
+    // bounds) are all issued before its 8 products; values past the edges are
+    // zeroed after loading.
+    for (int k0 = 0; k0 < a.N2; k0 += 32) {
+        double yv[8], tv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int kc = min(k0 + 4 * u + kq, a.N2 - 1);
+            yv[u] = yrow[kc];
+            tv[u] = trig[kc];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const bool kok = k0 + 4 * u + kq < a.N2;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((rok && kok) ? yv[u] : 0.0, (cok && kok) ? tv[u] : 0.0, acc, 0, 0, 0);
+        }
     }
-    a.S[(((size_t)0 * a.N1 + i1) * a.ncut + n) * a.NP + e] = sc;
-    a.S[(((size_t)1 * a.N1 + i1) * a.ncut + n) * a.NP + e] = ss;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int r = r0 + (lane >> 4) + 4 * i, c = c0 + (lane & 15);
+        if (r < nrow && c < ncol) {
+            const int e = r / a.N1, i1 = r % a.N1, qq = c / a.ncut, nn = c % a.ncut;
+            a.S[(((size_t)qq * a.N1 + i1) * a.ncut + nn) * a.NP + e] = acc[i];
+        }
+    }
 }
 
 // F_xy[mm][e] = w (1 + corr_mn) sum_i1 x_m(i1) S_y[i1][n][e]: one workgroup per
@@ -1949,11 +1972,12 @@ CoulArgs coul_args(edus_ctx *c, const double *Y) {
 int launch_coulomb(edus_ctx *c, const double *Y) {
     CoulArgs a = coul_args(c, Y);
     const int tx = std::max(c->ncut * c->ncut * c->nu, c->nb), t3 = c->N2 * c->ncut * c->nu;
-    const int t1 = c->N1 * c->ncut * c->np;
-    hipLaunchKernelGGL(k_coul_s1, dim3((t1 + 255) / 256), dim3(256), 0, c->stream, a);
+    // 64-thread workgroups: the passes have a few 10^4 outputs each, and small
+    // workgroups spread them over all CUs.  Pass 1 runs a wave per 16 x 16 tile.
+    hipLaunchKernelGGL(k_coul_s1_mfma, dim3((c->np * c->N1 + 15) / 16, (2 * c->ncut + 15) / 16), dim3(64), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_s2, dim3(c->ncut * c->ncut), dim3(kCoulThreads), 0, c->stream, a);
-    hipLaunchKernelGGL(k_coul_x, dim3((tx + 255) / 256), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_coul_s3, dim3((t3 + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_x, dim3((tx + 63) / 64), dim3(64), 0, c->stream, a);
+    hipLaunchKernelGGL(k_coul_s3, dim3((t3 + 63) / 64), dim3(64), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_w, dim3((c->N1 + kCoulRows - 1) / kCoulRows, (c->nu * c->N2 + 255) / 256), dim3(256), 0, c->stream, a,
                        c->N1);
     HIP_TRY(hipGetLastError());
