@@ -607,11 +607,26 @@ __global__ void k_coul_s3(CoulArgs a) {
     }
     const cd *X = reinterpret_cast<const cd *>(a.X);
     cd tc = mk(0, 0), ts = mk(0, 0);
-    for (int n = 0; n < a.ncut; n++) {
-        const size_t d = (size_t)(m * a.ncut + n) * nb2 + j * nb + i;
-        const double yc = a.cyn[n * a.N2 + i2], ys = a.syn[n * a.N2 + i2];
-        tc = cadd(tc, cadd(cscale(X[0 * (size_t)nmm * nb2 + d], yc), cscale(X[1 * (size_t)nmm * nb2 + d], ys)));
-        ts = cadd(ts, cadd(cscale(X[2 * (size_t)nmm * nb2 + d], yc), cscale(X[3 * (size_t)nmm * nb2 + d], ys)));
+    // n in blocks of 4: the block's loads (at a clamped n) are issued together,
+    // the sums keep the order of n.
+    for (int n0 = 0; n0 < a.ncut; n0 += 4) {
+        cd xv[4][4];
+        double ycv[4], ysv[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int n = min(n0 + b, a.ncut - 1);
+            const size_t d = (size_t)(m * a.ncut + n) * nb2 + j * nb + i;
+#pragma unroll
+            for (int qd = 0; qd < 4; qd++) xv[b][qd] = X[qd * (size_t)nmm * nb2 + d];
+            ycv[b] = a.cyn[n * a.N2 + i2];
+            ysv[b] = a.syn[n * a.N2 + i2];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if (n0 + b < a.ncut) {
+                tc = cadd(tc, cadd(cscale(xv[b][0], ycv[b]), cscale(xv[b][1], ysv[b])));
+                ts = cadd(ts, cadd(cscale(xv[b][2], ycv[b]), cscale(xv[b][3], ysv[b])));
+            }
     }
     cd *T = reinterpret_cast<cd *>(a.Tt);
     T[(((size_t)0 * a.ncut + m) * a.nu + u) * a.N2 + i2] = tc;
