@@ -635,34 +635,39 @@ __global__ void k_coul_s3(CoulArgs a) {
 
 // Mean-field matrix of every k from the column sums (Coulomb.h:510-541):
 // X_W(i1, i2)_u = -sum_m [cos_m(i1) Tc(m, u, i2) + sin_m(i1) Ts(m, u, i2)], stored
-// SoA as Xw[2u + re/im][k].  A workgroup serves 8 rows i1, so each table value
-// read serves 8 k.
-constexpr int kCoulRows = 8;
-__global__ __launch_bounds__(256) void k_coul_w(CoulArgs a, int N1) {
-    const int v = blockIdx.y * 256 + threadIdx.x;
-    if (v >= a.nu * a.N2) return;
-    const int u = v / a.N2, i2 = v % a.N2, r0 = blockIdx.x * kCoulRows, ncut = a.ncut;
-    const cd *T = reinterpret_cast<const cd *>(a.Tt);
-    cd acc[kCoulRows];
+// SoA as Xw[2u + re/im][k], formed as products on the FP64 matrix cores: for each
+// (u, re/im), Xw(i1, i2) = -sum_kk A[i1][kk] B[kk][i2] with A = [cos_m(i1) |
+// sin_m(i1)] and B = [Tc(m, u, i2) ; Ts(m, u, i2)] (K = 2 Ncut).  Rows are i1,
+// columns c = (2u + re/im) N2 + i2; one wave per 16 x 16 tile, its K loads
+// issued before the products (lane l feeds A[row l%16][kk l/16] and
+// B[kk l/16][col l%16] and ends with C[l/16 + 4 i][l%16]).
+__global__ __launch_bounds__(64) void k_coul_w_mfma(CoulArgs a, int N1) {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    constexpr int KB = 16;  // K terms loaded per block (2 Ncut <= 64)
+    const int lane = threadIdx.x, r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
+    const int ncut = a.ncut, K = 2 * ncut, ncol = 2 * a.nu * a.N2;
+    const int ra = r0 + (lane & 15), cb = c0 + (lane & 15), kq = lane >> 4;
+    const bool rok = ra < N1, cok = cb < ncol;
+    const int i1 = rok ? ra : 0, uc = cok ? cb / a.N2 : 0, i2 = cok ? cb % a.N2 : 0, u = uc >> 1, ri = uc & 1;
+    v4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KB) {
+        double av[KB / 4], bv[KB / 4];
 #pragma unroll
-    for (int r = 0; r < kCoulRows; r++) acc[r] = mk(0.0, 0.0);
-    for (int m = 0; m < ncut; m++) {
-        const cd tc = T[(((size_t)0 * ncut + m) * a.nu + u) * a.N2 + i2];
-        const cd ts = T[(((size_t)1 * ncut + m) * a.nu + u) * a.N2 + i2];
+        for (int b = 0; b < KB / 4; b++) {
+            const int kk = min(k0 + 4 * b + kq, K - 1), q = kk >= ncut, m = kk - q * ncut;
+            av[b] = (q ? a.sxm : a.cxm)[m * N1 + i1];
+            bv[b] = a.Tt[((((size_t)q * ncut + m) * a.nu + u) * a.N2 + i2) * 2 + ri];
+        }
 #pragma unroll
-        for (int r = 0; r < kCoulRows; r++) {
-            const int i1 = min(r0 + r, N1 - 1);
-            const double cm = a.cxm[m * N1 + i1], sm = a.sxm[m * N1 + i1];
-            acc[r].x -= cm * tc.x + sm * ts.x;
-            acc[r].y -= cm * tc.y + sm * ts.y;
+        for (int b = 0; b < KB / 4; b++) {
+            const bool kok = k0 + 4 * b + kq < K;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64((rok && kok) ? av[b] : 0.0, (cok && kok) ? bv[b] : 0.0, acc, 0, 0, 0);
         }
     }
 #pragma unroll
-    for (int r = 0; r < kCoulRows; r++) {
-        if (r0 + r >= N1) break;
-        const size_t k = (size_t)(r0 + r) * a.N2 + i2;
-        a.Xw[(size_t)(2 * u) * a.nkp + k] = acc[r].x;
-        a.Xw[(size_t)(2 * u + 1) * a.nkp + k] = acc[r].y;
+    for (int i = 0; i < 4; i++) {
+        const int r = r0 + (lane >> 4) + 4 * i, c = c0 + (lane & 15);
+        if (r < N1 && c < ncol) a.Xw[(size_t)(c / a.N2) * a.nkp + (size_t)r * a.N2 + c % a.N2] = -acc[i];
     }
 }
 
@@ -1993,7 +1998,7 @@ int launch_coulomb(edus_ctx *c, const double *Y) {
     hipLaunchKernelGGL(k_coul_s2, dim3(c->ncut * c->ncut), dim3(kCoulThreads), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_x, dim3((tx + 63) / 64), dim3(64), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_s3, dim3((t3 + 63) / 64), dim3(64), 0, c->stream, a);
-    hipLaunchKernelGGL(k_coul_w, dim3((c->N1 + kCoulRows - 1) / kCoulRows, (c->nu * c->N2 + 255) / 256), dim3(256), 0, c->stream, a,
+    hipLaunchKernelGGL(k_coul_w_mfma, dim3((c->N1 + 15) / 16, (2 * c->nu * c->N2 + 15) / 16), dim3(64), 0, c->stream, a,
                        c->N1);
     HIP_TRY(hipGetLastError());
     return 0;
