@@ -1175,6 +1175,11 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_sta
 // table, and every instance's arguments sit in a device array.  The stage body
 // is the single-problem kernel's, so a batched step is bit-identical to
 // stepping each problem on its own.
+// Threads per workgroup of the batched small kernel: an instance of a few
+// thousand k then spreads over more CUs (the k past nk, up to the padded nkp,
+// are computed and never read).
+constexpr int kSmallBatchThreads = 64;
+
 struct BatchEntry {
     ConstArgs c;
     StageArgs a[4];  // RK4 stages; tab is the instance's row of the call's first step
@@ -1190,7 +1195,7 @@ __global__ __launch_bounds__(256) void k_stage_small_batch(const BatchEntry *E, 
     StageArgs a = be.a[rk4_stage_of<MODE>()];
     a.tab += tab_step;
     a.dmask = dmask[w.x];
-    stage_small<NB, MODE, DISS>(be.c, a, w.y * 256 + threadIdx.x);
+    stage_small<NB, MODE, DISS>(be.c, a, w.y * blockDim.x + threadIdx.x);
 }
 
 template <int NBT, int MODE, typename R>
@@ -2731,10 +2736,10 @@ int batch_launch(edus_batch *b, const edus_batch::Cls &k, size_t tab_step) {
         auto go = [&](auto dtag) {
             constexpr bool DISS = decltype(dtag)::value;
             switch (k.nb_small) {
-                case 1: hipLaunchKernelGGL((k_stage_small_batch<1, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
-                case 2: hipLaunchKernelGGL((k_stage_small_batch<2, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
-                case 3: hipLaunchKernelGGL((k_stage_small_batch<3, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
-                default: hipLaunchKernelGGL((k_stage_small_batch<4, MODE, DISS>), grid, dim3(256), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
+                case 1: hipLaunchKernelGGL((k_stage_small_batch<1, MODE, DISS>), grid, dim3(kSmallBatchThreads), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
+                case 2: hipLaunchKernelGGL((k_stage_small_batch<2, MODE, DISS>), grid, dim3(kSmallBatchThreads), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
+                case 3: hipLaunchKernelGGL((k_stage_small_batch<3, MODE, DISS>), grid, dim3(kSmallBatchThreads), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
+                default: hipLaunchKernelGGL((k_stage_small_batch<4, MODE, DISS>), grid, dim3(kSmallBatchThreads), 0, b->stream, b->entries, k.blk, b->masks, tab_step); break;
             }
         };
         if (k.diss) go(std::true_type{});
@@ -2826,7 +2831,7 @@ int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch
         for (int i : k.members) {
             const edus_ctx *c = b->inst[i];
             const int kpb = c->nb <= 8 ? WaveGeom<8>::KPB : c->nb <= 16 ? WaveGeom<16>::KPB : WaveGeom<32>::KPB;
-            const int nblk = c->soa ? (c->nk + 255) / 256 : (c->nk + kpb - 1) / kpb;
+            const int nblk = c->soa ? (c->nk + kSmallBatchThreads - 1) / kSmallBatchThreads : (c->nk + kpb - 1) / kpb;
             for (int q = 0; q < nblk; q++) map.push_back(make_int2(i, q));
         }
         k.nblocks = (int)map.size();
