@@ -157,9 +157,12 @@ def write_crystal_input(d, nkf=25, nk=24, norb=3, valence=1, stepper="", nsteps=
     return p, H, B, (a1, a2, b1, b2)
 
 
-def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, coulomb=False):
+def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, coulomb=False, coul_dir=COULOMB,
+                  coul_label=COULOMB_LABEL):
     """Oracle tables for a setup: k-grid, shells, neighbours and model from the oracle's
-    own restatement (tight binding), the setup's parameters and pulses."""
+    own restatement (tight binding), the setup's parameters and pulses.  With
+    coulomb, the mean-field coefficients are parsed from the files in coul_dir
+    (the reference's example files by default)."""
     info = setup.info
     N3 = [info["nk0"], info["nk1"], info["nk2"]]
     nb = info["nb"]
@@ -185,7 +188,7 @@ def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, cou
     sys_ = O.System(H, D, U, T, nbr, wshell, bvec, info["nch"], info["ncc"], info["wx2"])
     if coulomb:
         ncut = info["ncut"]
-        cf = read_coulomb_files(COULOMB, COULOMB_LABEL, nb, ncut)
+        cf = read_coulomb_files(coul_dir, coul_label, nb, ncut)
         sys_.set_coulomb(ncut, kpt, cf["A"], cf["B"], cf["C"], cf["D"], cf["screen"], cf["vh"])
     return {"sys": sys_, "kpt": kpt, "bvec": bvec, "w": w, "sizes": sizes, "res": res, "nbr": nbr, "H": H, "U": U, "D": D,
             "T": T}
