@@ -1,22 +1,26 @@
 """Benchmark of the k-space time-stepping path on one MI355X.
 
-Workload (default): the synthetic Wannier model -- 16 bands, 441 lattice vectors
-(|R1|,|R2| <= 10, hexagonal, a = 2.5 A, seed 1001), 200x200 k-grid, 4 filled bands,
-gaussian pump 0.6 eV / 0.5 V/A along z, RK4 with dt = 0.05 a.u. (4000 steps in the
-full run).  A step is one RK4 step (4 fused RHS+stage launches) over all k, with the
-state and every operator resident in HBM before the timed region.
+Workload (default): the large synthetic Wannier model -- 32 bands, 961 lattice
+vectors (|R1|,|R2| <= 15, hexagonal, a = 2.5 A, exp(-|R|/3 A) decay, seed 4004),
+512x512 k-grid, 8 filled bands, two gaussian pumps along z (0.4 eV at 0.4 V/A and
+0.8 eV at 0.1 V/A, 20 fs intensity FWHM), RK4 with dt = 0.04 a.u.  A step is one
+RK4 step (4 fused RHS+stage launches) over all k, with the state and every
+operator resident in HBM before the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--precision fp64|mixed]
+The line carries both forms of the stage kernel: the plain FP64 path (the
+drop-in, `value`) and the mixed form (FP32 operators and products, FP64 state;
+`mixed`), with the mixed form's J(t), final conduction population and
+harmonic-intensity errors against the FP64 path over a window of steps at the
+pulse peak.
 
-With --precision mixed the operators and commutator products run in FP32 (state
-and increments FP64); the line then also states the J(t) error of that form
-against the FP64 path over a window of steps around the pulse peak.
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload ...] [--forms fp64,mixed] [--no-cpu]
 
 Prints one JSON line.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -27,45 +31,61 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MATRIX_PEAK_TFLOPS = 78.6
+FP32_MATRIX_PEAK_TFLOPS = 157.3
+METRIC = "k-point·steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %"
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["wannier", "hbn", "hbn_coulomb", "wannier32", "ab5", "batch"], default="wannier",
-                    help="wannier: synthetic 16-band Wannier model, 200x200 (default); hbn: 3-band hBN tight binding, "
-                         "60x60, T2 = 10 fs (the CPU-runnable case); wannier32: 32 bands, |R| <= 15, 512x512; "
-                         "ab5: the reference's own AB5 example run end to end through the drop-in (all 16591 steps, "
-                         "observables and files), the workload of its one published time")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["wannier32", "wannier", "hbn", "hbn_coulomb", "ab5", "batch"], default="wannier32",
+                    help="wannier32: 32 bands, |R| <= 15, 512x512, two pumps (default); wannier: 16 bands, |R| <= 10, "
+                         "200x200; hbn: 3-band hBN tight binding, 60x60, T2 = 10 fs (the CPU-runnable case); "
+                         "hbn_coulomb: 3-band hBN with the mean field, 300x300, AB4; ab5: the reference's own AB5 "
+                         "example run end to end through the drop-in (all 16591 steps, observables and files), the "
+                         "workload of its one published time; batch: the batched scan of independent instances")
     ap.add_argument("--nk", type=int, default=0, help="grid points per direction (0: the workload's own)")
     ap.add_argument("--nw", type=int, default=16)
     ap.add_argument("--rmax", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--forms", default="fp64,mixed",
+                    help="stage-kernel forms to time, the first one is `value`: fp64 (the drop-in), mixed (FP32 "
+                         "operators and products, FP64 state; 5 bands or more)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="time one form only (same as --forms <form>)")
     ap.add_argument("--stepper", choices=["rk4", "taylor4"], default="rk4",
                     help="wannier workloads: classic RK4 (four stage launches a step) or the Taylor-4 multistep "
                          "integrator (one launch a step)")
-    ap.add_argument("--acc-steps", type=int, default=400, help="steps of the mixed-vs-FP64 J(t) comparison")
+    ap.add_argument("--acc-steps", type=int, default=200, help="steps of the mixed-vs-FP64 J(t) comparison")
     ap.add_argument("--contraction", choices=["fp64", "fp32"], default="fp64",
-                    help="wannier workloads: Wannier contraction on the FP64 or the FP32 matrix cores (with "
-                         "--precision mixed the J(t) error is then taken against an FP64-contracted setup)")
+                    help="wannier workloads: Wannier contraction on the FP64 or the FP32 matrix cores (the J(t) "
+                         "error of the mixed form is then taken against an FP64-contracted setup)")
     ap.add_argument("--instances", type=int, default=256, help="batch workload: number of instances")
     ap.add_argument("--end-to-end", action="store_true",
                     help="time the drop-in's own loop (observables and output files included) for --steps steps")
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps of the CPU sample (0: auto)")
+    ap.add_argument("--cpu-subgrid", type=int, default=128,
+                    help="grid points per direction of the CPU sample when the workload's grid is above 200x200")
+    ap.add_argument("--no-probe", action="store_true", help="skip the streaming-rate probe")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run 20 plain device copies of MIB MiB (HBM counter calibration) and exit")
+                    help="only run the library's streaming probe on arrays of MIB MiB (its copy kernel is the "
+                         "known-size kernel of the HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the k-grid of one problem lives in one HBM); "
               "multi-GPU k-slab sharding is a design in DESIGN.md, not built.", file=sys.stderr)
         sys.exit(2)
+    if args.precision:
+        args.forms = args.precision
 
     if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
+        from edus_amd import native as N
+        r = N.stream_probe(args.copy_calibration << 20, 5)
+        print(json.dumps({"probe_mib": args.copy_calibration, "gbs": r}))
         return
     if args.workload == "ab5":
         bench_ab5()
@@ -73,146 +93,224 @@ def main():
     if args.workload == "batch":
         bench_batch(args)
         return
+    bench_single(args)
 
-    from edus_amd import native as N
+
+def workload_input(args, wd, nk_override=None):
+    """Input file and name of a single-problem workload."""
     from edus_amd import workloads
 
-    wd = args.workdir or tempfile.mkdtemp(prefix="edus_bench_")
-    if args.stepper != "rk4" and args.workload not in ("wannier", "wannier32"):
-        ap.error("--stepper taylor4 applies to the wannier workloads")
     wstep = "RK4" if args.stepper == "rk4" else "Taylor"
     nsteps_total = args.steps + args.warmup
-    t_setup = time.perf_counter()
     if args.workload == "wannier":
-        nkd = args.nk or 200
+        nkd = nk_override or args.nk or 200
         inp = workloads.wannier_synthetic(wd, nw=args.nw, rmax=args.rmax, nk=nkd, nsteps=max(4000, nsteps_total),
                                           dt=0.05, stepper=wstep, e_v_ang=0.5, nvalence=4, seed=1001, decay_ang=2.0)
-        wname = f"wannier_nw{args.nw}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{nkd}x{nkd}_{args.stepper}_dt0.05"
-    elif args.workload == "wannier32":
-        # 32 bands, |R| <= 15, 512 x 512, two pumps of 20 fs intensity FWHM:
-        # 0.4 eV at 0.4 V/A and 0.8 eV at 0.1 V/A, dt 0.04 au
-        nkd = args.nk or 512
+        return inp, f"wannier_nw{args.nw}_R{2 * args.rmax + 1}x{2 * args.rmax + 1}_{nkd}x{nkd}_{args.stepper}_dt0.05"
+    if args.workload == "wannier32":
+        nkd = nk_override or args.nk or 512
         inp = workloads.wannier_synthetic(wd, nw=32, rmax=15, nk=nkd, nsteps=max(5000, nsteps_total), dt=0.04,
                                           stepper=wstep, e_v_ang=0.4, nvalence=8, seed=4004, decay_ang=3.0,
                                           photon_ev=0.4, fwhm_fs=20.0, extra_pumps=((0.8, 0.1),))
-        wname = f"wannier_nw32_R31x31_{nkd}x{nkd}_2pumps_{args.stepper}_dt0.04"
-    elif args.workload == "hbn_coulomb":
-        nkd = args.nk or 300
+        return inp, f"wannier_nw32_R31x31_{nkd}x{nkd}_2pumps_{args.stepper}_dt0.04"
+    if args.workload == "hbn_coulomb":
+        nkd = nk_override or args.nk or 300
         inp = workloads.hbn_coulomb(wd, os.path.join(ROOT, "tests", "golden", "coulomb"), "G_num502657_Ncut20_", nk=nkd,
                                     nsteps=max(6000, nsteps_total), dt=0.03, order=4)
-        wname = f"hbn_coulomb_haldane_corebn_{nkd}x{nkd}_ncut20_ab4_dt0.03"
-    else:
-        nkd = args.nk or 60
-        inp = workloads.hbn_tb(wd, nk=nkd, nsteps=max(8000, nsteps_total), dt=0.05, stepper="RK4")
-        wname = f"hbn_gen2d_3band_{nkd}x{nkd}_T2_10fs_rk4_dt0.05"
+        return inp, f"hbn_coulomb_haldane_corebn_{nkd}x{nkd}_ncut20_ab4_dt0.03"
+    nkd = nk_override or args.nk or 60
+    inp = workloads.hbn_tb(wd, nk=nkd, nsteps=max(8000, nsteps_total), dt=0.05, stepper="RK4")
+    return inp, f"hbn_gen2d_3band_{nkd}x{nkd}_T2_10fs_rk4_dt0.05"
+
+
+def stage_model(info, nd, form):
+    """Bytes and matrix-core flops one stage launch must move / do per k in the
+    engine's own layout: packed Hermitian state (8 nb^2 B in FP64), packed H and
+    the nd dipole components the field needs (8 nb^2 B each in FP64, 4 nb^2 B in
+    the mixed form), U full complex (16 / 8 nb^2 B) when dephasing is on.  State
+    transfers per launch: RK4 3, 5, 5, 3 (mean 4); Adams-Bashforth / Taylor of
+    order s: s + 2 (read y_n, write f_n, read s - 1 old f, write y_n+1), +1 for
+    the Coulomb pass.  Flops: one complex NBT^3 product for the commutator
+    (NBT the padded tile, 16 or 32 on the matrix cores), four more with
+    dephasing.  Also the survey's figure S (4 + n_op) per RK4 launch
+    (S = 16 nb^2, every operator full complex, all three dipole components)."""
+    from edus_amd import native as N
+
+    nb = info["nb"]
+    rk4 = info["stepper"] == N.STEPPER_RK4
+    transfers = 4 if rk4 else info["order"] + 2 + (1 if info["ncut"] else 0)
+    opw = 8 if form == "fp64" else 4
+    diss = info["n_diss"] > 0
+    per_k = transfers * 8 * nb * nb + (1 + nd) * opw * nb * nb + (2 * opw * nb * nb if diss else 0)
+    nbt = nb if nb <= 4 else 8 if nb <= 8 else 16 if nb <= 16 else 32
+    flop_k = (8 * nbt ** 3) * (5 if diss else 1) if nbt >= 16 else 0
+    S = 16 * nb * nb
+    n_op = 4 + (1 if diss else 0)
+    survey_k = (4 + n_op) * S if rk4 else (info["order"] + 2 + n_op + (1 if info["ncut"] else 0)) * S
+    kern = (f"k_stage_wave<{nbt}>" if nb > 4 else f"k_stage_small<{nb}>")
+    stepper = "RK4" if rk4 else (f"Taylor{info['order']}" if info["stepper"] == N.STEPPER_TAYLOR else f"AB{info['order']}")
+    return {"bytes_k": per_k, "flop_k": flop_k, "survey_bytes_k": survey_k, "kernel": kern, "stepper": stepper}
+
+
+def time_form(s, form, f, g, args):
+    """Engine of one precision form: warm-up, then args.steps timed steps (wall
+    clock around a synchronised call; kernel time from the engine's HIP events)."""
+    from edus_amd import native as N
+
+    prec = N.PRECISION_MIXED if form == "mixed" else N.PRECISION_FP64
+    t0 = time.perf_counter()
+    eng = N.Engine(s.problem(), precision=prec)
+    t_create = time.perf_counter() - t0
+    if args.warmup:
+        eng.step(f[:args.warmup], g[:args.warmup])
+    t0 = time.perf_counter()
+    eng.step(f[args.warmup:], g[args.warmup:])
+    wall = time.perf_counter() - t0
+    dev_ms, nlaunch = eng.last_step_time()
+    return eng, {"wall_s": wall, "dev_ms": dev_ms, "nlaunch": nlaunch, "create_s": t_create}
+
+
+def bench_single(args):
+    from edus_amd import native as N
+
+    wd = args.workdir or tempfile.mkdtemp(prefix="edus_bench_")
+    forms = [x.strip() for x in args.forms.split(",") if x.strip()]
+    if args.stepper != "rk4" and args.workload not in ("wannier", "wannier32"):
+        sys.exit("--stepper taylor4 applies to the wannier workloads")
     if args.contraction == "fp32" and args.workload not in ("wannier", "wannier32"):
-        ap.error("--contraction fp32 applies to the wannier workloads")
+        sys.exit("--contraction fp32 applies to the wannier workloads")
+    t_setup = time.perf_counter()
+    inp, wname = workload_input(args, wd)
     s = N.Setup(inp, fp32_contraction=args.contraction == "fp32")
     info = s.info
+    t_setup = time.perf_counter() - t_setup
+    nk, nb = info["nk"], info["nb"]
+    if nb < 5:
+        forms = [x for x in forms if x != "mixed"] or ["fp64"]
     if args.end_to_end:
         s.run(os.path.join(wd, "warm"), max_steps=args.warmup)
         t0 = time.perf_counter()
         n = s.run(os.path.join(wd, "run"), max_steps=args.steps)
         el = time.perf_counter() - t0
-        value = info["nk"] * n / el
         print(json.dumps({
-            "metric": "k-point\u00b7steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %",
-            "value": value, "unit": "k-point\u00b7steps/s", "n_gpus": 1, "steps": n, "warmup": args.warmup,
-            "ms_per_step": el * 1e3 / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic",
-            "config": {"workload": wname, "nk": info["nk"], "bands": info["nb"], "it_resolution": info["it_resolution"],
+            "metric": METRIC, "value": nk * n / el, "unit": "k-point·steps/s", "n_gpus": 1, "steps": n,
+            "warmup": args.warmup, "ms_per_step": el * 1e3 / n, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": wname, "nk": nk, "bands": nb, "it_resolution": info["it_resolution"],
                        "includes": "the drop-in's time loop: steps, observables and output files from t0",
                        "seconds": round(el, 3)}}))
         return
-    prec = N.PRECISION_MIXED if args.precision == "mixed" else N.PRECISION_FP64
-    eng = N.Engine(s.problem(), precision=prec)
-    t_setup = time.perf_counter() - t_setup
-    nk, nb = info["nk"], info["nb"]
+
+    probe = None
+    if not args.no_probe:
+        probe = N.stream_probe(1 << 30, 5)
 
     # Field schedule of the steps around the pulse peak, computed before timing.
+    nsteps_total = args.steps + args.warmup
     it0 = max(0, (info["itfi"] + 1) // 2 - nsteps_total // 2)
     f, g = s.fields(it0, nsteps_total)
-    eng.step(f[:args.warmup], g[:args.warmup])
-    t0 = time.perf_counter()
-    eng.step(f[args.warmup:], g[args.warmup:])
-    dt_wall = time.perf_counter() - t0
-    dev_ms, nlaunch = eng.last_step_time()
-    value = nk * args.steps / dt_wall
-
-    # Roofline of the dominant kernel (the fused RK4 RHS+stage kernel).  Algorithmic
-    # bytes per k and RK4 stage: S * (4 + n_op) with S = 16 nb^2 (one complex128
-    # nb x nb matrix), the 16 state transfers of a step spread over its 4 stages and
-    # n_op operator matrices per RHS (H, Dx, Dy, Dz; +U with dephasing).  The engine
-    # stores each Hermitian matrix packed (8 nb^2 bytes), so the bytes it has to move
-    # are half of that (U stays full complex).
-    # Adams-Bashforth s and Taylor s: S * (s + 2 + n_op) per step (read y_n, write
-    # f_n, read s-1 old f, write y_n+1), one launch per step, plus S for the
-    # Coulomb pass.
-    S = 16 * nb * nb
-    n_op = 4 + (1 if info["n_diss"] > 0 else 0)
-    rk4 = info["stepper"] == N.STEPPER_RK4
-    transfers = 4 if rk4 else info["order"] + 2 + (1 if info["ncut"] else 0)
-    bytes_launch = (transfers + n_op) * S * nk
-    # The engine reads a dipole component only when some field of the call has
-    # that component (a zero field multiplies it by an exact zero): H plus the
-    # nd active components.  (Mixed form: operators and U at half width.)
     ft = f[args.warmup:]
     nd = sum(1 for x in range(3) if np.any(ft[..., x] != 0) or np.any(ft[..., 3 + x] != 0))
-    opw = 1 if args.precision == "fp64" else 2
-    stored_launch = (transfers * (S // 2) * nk + (1 + nd) * (S // 2 // opw) * nk
-                     + (S // opw * nk if info["n_diss"] > 0 else 0))
-    stepper = "RK4" if rk4 else (f"Taylor{info['order']}" if info["stepper"] == N.STEPPER_TAYLOR else f"AB{info['order']}")
-    avg_launch_s = dev_ms * 1e-3 / nlaunch
-    achieved = bytes_launch / avg_launch_s / 1e9
-    stored = stored_launch / avg_launch_s / 1e9
-    traffic = None
-    suffix = "" if args.precision == "fp64" else "_mixed"
-    tfile = os.path.join(ROOT, "profiles", f"traffic_stage_wave_nb{nb}_nk{nk}{suffix}.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": (f"k_stage_wave<{8 if nb <= 8 else 16 if nb <= 16 else 32}>" if nb > 4 else f"k_stage_small<{nb}>")
-                      + (" (RK4 stages)" if rk4 else f" ({stepper} steps)")
-                      + ("" if args.precision == "fp64" else ", FP32 operators"),
-            "bytes_per_launch": bytes_launch,
-            "stored_bytes_per_launch": stored_launch, "dipole_components_read": nd,
-            "achieved_stored": round(stored, 2),
-            "frac_stored": round(stored / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_launch_s * 1e6, 3)}
+
+    legs, engines = {}, {}
+    for form in forms:
+        eng, t = time_form(s, form, f, g, args)
+        engines[form] = eng
+        m = stage_model(info, nd, form)
+        avg_launch_s = t["dev_ms"] * 1e-3 / t["nlaunch"]
+        bytes_launch = m["bytes_k"] * nk
+        achieved = bytes_launch / avg_launch_s / 1e9
+        if achieved > HBM_PEAK_GBS:
+            raise SystemExit(f"bench.py: {form} stage kernel at {achieved:.0f} GB/s above the HBM peak: the byte model is wrong")
+        peak_mx = FP64_MATRIX_PEAK_TFLOPS if form == "fp64" else FP32_MATRIX_PEAK_TFLOPS
+        tflops = m["flop_k"] * nk / avg_launch_s / 1e12
+        legs[form] = {
+            "value": nk * args.steps / t["wall_s"], "ms_per_step": t["wall_s"] * 1e3 / args.steps,
+            "device_ms_per_step": t["dev_ms"] / args.steps, "avg_launch_us": round(avg_launch_s * 1e6, 3),
+            "launches_per_step": t["nlaunch"] // args.steps, "engine_create_s": round(t["create_s"], 2),
+            "kernel": m["kernel"] + (" (RK4 stages)" if m["stepper"] == "RK4" else f" ({m['stepper']} steps)")
+                      + ("" if form == "fp64" else ", FP32 operators and products"),
+            "bytes_per_launch": bytes_launch, "achieved_gbs": round(achieved, 2),
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "mfma": ({"flop_per_launch": m["flop_k"] * nk, "achieved_tflops": round(tflops, 2),
+                      "peak_tflops": peak_mx, "frac": round(tflops / peak_mx, 4)} if m["flop_k"] else None),
+            "survey_bytes_per_launch": m["survey_bytes_k"] * nk,
+        }
+    head = forms[0]
+    lead = legs[head]
+    traffic, tsrc = None, None
+    suffix = "" if head == "fp64" else "_mixed"
+    tfile = os.path.join("profiles", f"traffic_stage_wave_nb{nb}_nk{nk}{suffix}.json")
+    if os.path.exists(os.path.join(ROOT, tfile)):
+        traffic = json.load(open(os.path.join(ROOT, tfile))).get("bytes_per_launch")
+        tsrc = tfile
+    roof = {"bound": "hbm", "achieved": lead["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": lead["frac"],
+            "traffic": traffic, "traffic_source": tsrc, "kernel": lead["kernel"],
+            "bytes_per_launch": lead["bytes_per_launch"], "bytes_model": "packed Hermitian state and operators, "
+            f"{nd} dipole component(s) read, mean state transfers per launch", "avg_launch_us": lead["avg_launch_us"],
+            "mfma": lead["mfma"],
+            "survey_bytes_per_launch": lead["survey_bytes_per_launch"],
+            "survey_rate_gbs": round(lead["survey_bytes_per_launch"] / (lead["avg_launch_us"] * 1e-6) / 1e9, 2)}
+    if probe:
+        roof["stream_probe_gbs"] = {k: round(v, 1) for k, v in probe.items()}
+        roof["frac_of_copy"] = round(lead["achieved_gbs"] / probe["copy"], 4)
+
+    mixed = None
+    if "mixed" in legs and args.acc_steps > 0:
+        acc = mixed_j_error(s, engines["mixed"], it0, args.acc_steps, ref=engines.get("fp64"),
+                            ref_setup=N.Setup(inp) if args.contraction == "fp32" else None)
+    else:
+        acc = None
+    if "mixed" in legs:
+        mixed = dict(legs["mixed"])
+        if "fp64" in legs:
+            mixed["speedup_vs_fp64"] = round(legs["mixed"]["value"] / legs["fp64"]["value"], 4)
+        mixed["error_vs_fp64"] = acc
+    for e in engines.values():
+        e.close()
 
     cpu = None
-    accuracy = None
-    if args.precision == "mixed":
-        accuracy = mixed_j_error(s, eng, it0, args.acc_steps,
-                                 ref_setup=N.Setup(inp) if args.contraction == "fp32" else None)
-    elif not args.no_cpu and rk4 and not info["ncut"]:  # the CPU sample runs the plain RK4 oracle
-        cpu = cpu_baseline(s, eng, args.cpu_steps)
+    rk4 = info["stepper"] == N.STEPPER_RK4
+    if not args.no_cpu and rk4 and not info["ncut"]:  # the CPU sample runs the plain RK4 oracle
+        if nk > 200 * 200:
+            sub_inp, _ = workload_input(args, os.path.join(wd, "cpu_subgrid"), nk_override=args.cpu_subgrid)
+            cpu = cpu_baseline(N.Setup(sub_inp), args.cpu_steps, subgrid=True)
+        else:
+            cpu = cpu_baseline(s, args.cpu_steps)
 
     out = {
-        "metric": "k-point\u00b7steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %",
-        "value": value,
-        "unit": "k-point\u00b7steps/s",
+        "metric": METRIC,
+        "value": lead["value"],
+        "unit": "k-point·steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt_wall * 1e3 / args.steps,
+        "ms_per_step": lead["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64/f32",
+        "dtype": "f64" if head == "fp64" else "f64/f32",
         "data": "synthetic",
-        "config": {"workload": wname,
-                   "nk": nk, "bands": nb, "stepper": stepper,
-                   "precision": "fp64" if args.precision == "fp64" else "mixed (FP32 operators and products, FP64 state)",
-                   "wannier_contraction": args.contraction if args.workload in ("wannier", "wannier32") else None,
+        "config": {"workload": wname, "nk": nk, "bands": nb, "stepper": stepper_name(info),
+                   "form": head, "wannier_contraction": args.contraction if args.workload in ("wannier", "wannier32") else None,
                    "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
-                   "device_ms_per_step": dev_ms / args.steps},
+                   "device_ms_per_step": lead["device_ms_per_step"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    if accuracy is not None:
-        out["j_error_vs_fp64"] = accuracy
+    if mixed is not None and head != "mixed":
+        out["mixed"] = mixed
+    elif head == "mixed":
+        out["j_error_vs_fp64"] = mixed["error_vs_fp64"] if mixed else None
     print(json.dumps(out))
+
+
+def stepper_name(info):
+    from edus_amd import native as N
+
+    if info["stepper"] == N.STEPPER_RK4:
+        return "RK4"
+    return f"Taylor{info['order']}" if info["stepper"] == N.STEPPER_TAYLOR else f"AB{info['order']}"
 
 
 def currents(ob):
@@ -220,18 +318,21 @@ def currents(ob):
     return ob["j1"][0::2] - 2.0 * ob["j2"][1::2]
 
 
-def mixed_j_error(s, eng, it0, nsteps, every=2, ref_setup=None):
+def mixed_j_error(s, eng, it0, nsteps, every=2, ref=None, ref_setup=None):
     """The mixed form's error against the FP64 drop-in path over nsteps from rho0
     in the pulse-peak window, both engines given the same fields: J(t) (every
     `every` steps), the final conduction population, and the harmonic spectrum
-    |FFT(J w)|^2 (Hann window w) of the J(t) series.  ref_setup: the FP64 path's
-    own tables (an FP64 Wannier contraction) when `s` was built otherwise; both
-    start from their own rho0."""
+    |FFT(J w)|^2 (Hann window w) of the J(t) series.  ref: an FP64 engine of the
+    same setup (made here if None); ref_setup: the FP64 path's own tables (an
+    FP64 Wannier contraction) when `s` was built otherwise; both start from
+    their own rho0."""
     from edus_amd import native as N
 
     info = s.info
     rs = ref_setup or s
-    ref = N.Engine(rs.problem(), precision=N.PRECISION_FP64)
+    own = ref is None or ref_setup is not None
+    if own:
+        ref = N.Engine(rs.problem(), precision=N.PRECISION_FP64)
     P0 = s.array("rho0", True).reshape(-1, info["nb"], info["nb"])
     P0r = rs.array("rho0", True).reshape(-1, info["nb"], info["nb"])
     f, g = s.fields(it0, nsteps)
@@ -256,7 +357,8 @@ def mixed_j_error(s, eng, it0, nsteps, every=2, ref_setup=None):
     im = np.abs(np.fft.rfft(jm * win, axis=0)) ** 2
     ir = np.abs(np.fft.rfft(jr * win, axis=0)) ** 2
     pm, pr = eng.get_state(), ref.get_state()
-    ref.close()
+    if own:
+        ref.close()
     return {"J_rel_err": float(np.max(np.abs(jm - jr)) / scale), "J_max": scale,
             "conduction_population_rel_err": abs(cm - cr) / max(abs(cr), 1e-300), "conduction_population": cr,
             "harmonic_intensity_rel_err": float(np.max(np.abs(im - ir)) / max(float(np.max(ir)), 1e-300)),
@@ -357,25 +459,27 @@ def bench_batch(args):
         "cpu_baseline": None}))
 
 
-def copy_calibration(mib):
-    """A plain device-to-device copy kernel of known size: its FETCH_SIZE and
-    WRITE_SIZE per dispatch calibrate the HBM counters (expected mib MiB each)."""
-    import torch
-    n = mib * (1 << 20) // 8
-    x = torch.rand(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(20):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_mib": mib, "dispatches": 20, "check": float(y[-1] - x[-1])}))
+def cpu_model():
+    """CPU model name of the host (from /proc/cpuinfo), for the baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
-def cpu_baseline(s, eng, steps):
+def cpu_baseline(s, steps, subgrid=False):
     """The CPU oracle (FP64 restatement of the reference loops, OpenMP over k) on a
-    bounded sample of the same workload: the full k-grid for a few RK4 steps from
-    the initial state at the pulse peak.  The GPU engine repeats the same steps, and
-    the J(t) error of the GPU path is stated from the two final states."""
+    bounded sample of the workload: a few RK4 steps of the setup's whole grid (the
+    workload's own grid, or a sub-grid of the same model when the full grid is too
+    large for the host) from the initial state at the pulse peak.  A GPU engine of
+    the same setup repeats the same steps, and the J(t) error of the GPU path is
+    stated from the two final states."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from edus_amd import native as N
     from oracle import oracle_py as O
     import helpers as Hh
 
@@ -398,6 +502,7 @@ def cpu_baseline(s, eng, steps):
         _, Pc = osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
         el = time.perf_counter() - t0
     # the same steps on the GPU, then J1 of both final states (GPU reductions)
+    eng = N.Engine(s.problem(), precision=N.PRECISION_FP64)
     f, g = s.fields(it0, n)
     g[0] = 1  # the oracle's run starts with E_non0 = true
     eng.set_state(P0)
@@ -406,12 +511,16 @@ def cpu_baseline(s, eng, steps):
     jg = eng.observables()["j1"].copy()
     eng.set_state(Pc)
     jc = eng.observables()["j1"].copy()
+    eng.close()
     j_err = float(np.max(np.abs(jg - jc)) / max(np.max(np.abs(jc)), 1e-300))
     p_err = float(np.max(np.abs(Pg - Pc)) / np.max(np.abs(Pc)))
+    grid = f"{info['nk1']}x{info['nk2']}"
     return {"value": info["nk"] * n / el, "unit": "k-point\u00b7steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} RK4 steps of the full {info['nk1']}x{info['nk2']} grid ({nb} bands) from rho0 at "
-                      f"t = {t_start:.3f} au, oracle, {cores} OpenMP threads",
-            "seconds": round(el, 2), "gpu_vs_oracle": {"J1_rel_err": j_err, "rho_rel_err": p_err, "steps": n}}
+            "cpu_model": cpu_model(),
+            "sample": f"{n} RK4 steps of the {'sub-grid ' if subgrid else 'full '}{grid} grid ({nb} bands"
+                      f"{', the same model' if subgrid else ''}) from rho0 at t = {t_start:.3f} au, oracle, "
+                      f"{cores} OpenMP threads",
+            "seconds": round(el, 2), "gpu_vs_oracle": {"J1_rel_err": j_err, "rho_rel_err": p_err, "steps": n, "grid": grid}}
 
 
 if __name__ == "__main__":

@@ -40,8 +40,8 @@ def _run(cmd):
 def build_engine():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libedus_hip.so")
-    src = os.path.join(CSRC, "hip", "engine.hip")
-    _run([HIPCC, *HIP_FLAGS, src, "-o", out, "-lrocsolver", "-lrocblas"])
+    srcs = [os.path.join(CSRC, "hip", f) for f in ("engine.hip", "stream.hip")]
+    _run([HIPCC, *HIP_FLAGS, *srcs, "-o", out, "-lrocsolver", "-lrocblas"])
     return out
 
 

@@ -188,7 +188,7 @@ int edus_setup_model(const edus_setup *h, char *name, int32_t len, double *param
         return 1;
     }
     edus::TBModel tb;
-    tb.init(s.in.TBtype, s.nch, s.nb, s.in.hopping);
+    tb.init(s.in.TBtype, s.nch, s.nb, s.in.hopping, s.in.b);
     std::snprintf(name, (size_t)len, "%s", tb.name.c_str());
     if (params) {
         const double v[8] = {tb.a, tb.g, tb.b, tb.t2, tb.phi0, tb.core, tb.rchv, tb.rchc};
@@ -204,7 +204,7 @@ int edus_setup_eval_tb(const edus_setup *h, const double *kcrys, int32_t n, doub
     const edus::Setup &s = h->s;
     if (s.in.iMode != "TB") { g_err = "not a tight-binding setup"; return EDUS_ERR_INVALID; }
     edus::TBModel tb;
-    tb.init(s.in.TBtype, s.nch, s.nb, s.in.hopping);
+    tb.init(s.in.TBtype, s.nch, s.nb, s.in.hopping, s.in.b);
     const int nb = s.nb;
     for (int q = 0; q < n; q++)
         for (int i = 0; i < nb; i++)

@@ -160,8 +160,10 @@ void eigh_jacobi(int n, const cplx *H, double *w, cplx *V);
 struct TBModel {
     std::string name;
     int Ncv = 0, Nch = 0;
-    double a = 0, b = 0, c = 0, d = 0, g = 0, t2 = 0, phi0 = 0, core = 0, rchv = 0, rchc = 0;
-    void init(const std::string &nm, int nch, int ncv, const std::vector<double> &hopping);
+    double a = 0, b = 0, c = 0, d = 0, f = 0, g = 0, t2 = 0, phi0 = 0, core = 0, rchv = 0, rchc = 0;
+    double vampa[2][3][6] = {};  // Vampa2015 band coefficients [band][direction][harmonic]
+    double kcart[3][3] = {};     // crystal -> cartesian k (columns b_1..b_3), for models written in cartesian k
+    void init(const std::string &nm, int nch, int ncv, const std::vector<double> &hopping, const double b_rows[3][3] = nullptr);
     bool supported() const;
     cplx energy(int ic, int jc, const double k[3]) const;
     cplx unitary(int ic, int jc, const double k[3]) const;

@@ -365,7 +365,8 @@ Input parse_input(const std::string &path) {
                             for (auto &row : in.a) row[0] = row[1] = row[2] = 0.;
                             in.a[0][0] = a00; in.a[1][1] = a11; in.a[1][2] = a12; in.a[2][1] = a21; in.a[2][2] = a22;
                         };
-                        // Built-in lattices and band counts of ReadInput.h:762-888.
+                        // Built-in lattices and band counts of ReadInput.h:762-888 (BN1b, DLGraphene,
+                        // Vampa2015, GeS: 800-835).
                         if (ieq(tb, "graphene") || ieq(tb, "GrapheneZurron")) {
                             double L = 2.46 * Units::space_A_au();
                             set_a(6.3303100000, L * std::cos(pi / 6.), L * std::sin(pi / 6.), L * std::cos(pi / 6.), -L * std::sin(pi / 6.));
@@ -412,6 +413,20 @@ Input parse_input(const std::string &path) {
                             double A = in.hopping[0];
                             set_a(6.3303100000, A * std::cos(pi / 6.), A * std::sin(pi / 6.), A * std::cos(pi / 6.), -A * std::sin(pi / 6.));
                             in.Nb[0] = 1; in.Nb[1] = 1; in.Nb[2] = 1;
+                        } else if (ieq(tb, "BN1b")) {  // the reference writes cos for both in-plane components
+                            double L = 2.5 * Units::space_A_au();
+                            set_a(6.3303100000, L * std::cos(pi / 6.), L * std::cos(pi / 6.), L * std::cos(pi / 6.), -L * std::cos(pi / 6.));
+                            in.Nb[0] = 0; in.Nb[1] = 1; in.Nb[2] = 0;
+                        } else if (ieq(tb, "DLGraphene")) {
+                            set_a(6.3303100000, 4.0257407400, -2.3242630000, 4.0257407400, 2.3242630000);
+                            in.Nb[0] = 0; in.Nb[1] = 2; in.Nb[2] = 2;
+                        } else if (ieq(tb, "Vampa2015")) {
+                            set_a(5.3200000000, 6.1400000000, 0.0, 0.0, 9.8300000000);
+                            in.Nb[0] = 0; in.Nb[1] = 1; in.Nb[2] = 1;
+                        } else if (ieq(tb, "GeS") || ieq(tb, "GeS_HSE06")) {
+                            double ax = (4.53 / 2.) * Units::space_A_au(), ay = (3.63 / 2.) * Units::space_A_au();
+                            set_a(6.3303100000, ax, ay, ax, -ay);
+                            in.Nb[0] = 0; in.Nb[1] = 1; in.Nb[2] = 1;
                         } else {
                             throw Error("TightBinding type " + tb + " is not supported by this build");
                         }

@@ -1,8 +1,10 @@
 // Band models feeding the H, U, D tables.
 //
-// Tight binding: the hexagonal boron-nitride family of headers/TightBinding.h
-// (init 81-89, set_hopping 153-173, energy 177-210 / 271-290 / 387-440,
-// unitary 894-1004, dipoles 1256-1421).  Each matrix element is the model's
+// Tight binding: the model families of headers/TightBinding.h (init 44-147,
+// set_hopping 153-173, energy 175-459, unitary 892-1213, dipoles 1256-1421):
+// the boron-nitride / Haldane / gen2d family, the graphene family (Graphene,
+// CoreGraphene, Graphene_costD, Graphene_symmetricD, GrapheneZurron, DLGraphene),
+// BN1b, Vampa2015 and GeS / GeS_HSE06.  Each matrix element is the model's
 // closed form; the expressions keep the reference's operand order so the values
 // agree to the last bit where libm agrees.
 // Wannier: reader of wannier90 <seed>_tb.dat as in headers/Wannier.h:47-149 and a
@@ -26,19 +28,56 @@ const cplx I1(0.0, 1.0);
 
 bool is_haldane(const std::string &n) { return n == "Haldane_CoreBN_Nedge" || n == "Haldane_CoreBN_Bedge"; }
 bool is_corebn(const std::string &n) { return n == "CoreBN_Nedge" || n == "CoreBN_Bedge"; }
+bool is_graphene_d(const std::string &n) { return n == "Graphene_costD" || n == "Graphene_symmetricD"; }
+bool is_ges(const std::string &n) { return n == "GeS" || n == "GeS_HSE06"; }
+bool is_graphene_family(const std::string &n) {
+    return n == "Graphene" || n == "CoreGraphene" || n == "GrapheneZurron" || is_graphene_d(n) || n == "DLGraphene";
+}
+
+// Vampa2015 band coefficients (TightBinding.h:98-121): [band][direction][harmonic j] of
+// sum_x sum_j c cos(j k_x); the y and z directions carry only j = 0, 1.
+const double kVampa[2][3][6] = {
+    {{-0.0928, 0.0705, 0.0200, -0.0012, 0.0029, 0.0006}, {-0.0307, 0.0307, 0, 0, 0, 0}, {-0.0059, 0.0059, 0, 0, 0, 0}},
+    {{0.0898, -0.0814, -0.0024, -0.0048, -0.0003, -0.0009}, {0.1147, -0.1147, 0, 0, 0, 0}, {0.0435, -0.0435, 0, 0, 0, 0}}};
+
+// Eigenvectors of a two-band block written as B0 + B.sigma, the form shared by
+// BoronNitride, the Haldane block and GeS (TightBinding.h:958-1029): from
+// (Bx, By, Bz), rows (e^{i phi} sin, -e^{-i phi} cos) and (e^{i phi} cos,
+// e^{-i phi} sin) of the half polar angle, phi half the azimuth.
+cplx bloch_unitary(int r, int col, double Bx, double By, double Bz) {
+    double B = std::sqrt(Bx * Bx + By * By + Bz * Bz);
+    double phi = std::atan2(By, Bx) / 2.;
+    double costh2 = std::sqrt((B + Bz) / 2. / B);
+    double sinth2 = std::sqrt((B - Bz) / 2. / B);
+    if (r == 0) return col == 0 ? std::exp(+I1 * phi) * sinth2 : -std::exp(-I1 * phi) * costh2;
+    if (r == 1) return col == 0 ? std::exp(+I1 * phi) * costh2 : std::exp(-I1 * phi) * sinth2;
+    return 0.;
+}
 }  // namespace
 
-void TBModel::init(const std::string &nm, int nch, int ncv, const std::vector<double> &hopping) {
+void TBModel::init(const std::string &nm, int nch, int ncv, const std::vector<double> &hopping, const double b_rows[3][3]) {
     name = nm;
     Nch = nch;
     Ncv = ncv;
     if (!supported()) throw Error("TightBinding type " + nm + " is not supported by this build");
-    // Boron-nitride family defaults (TightBinding.h:81-89).
-    a = 2.5 * Units::space_A_au();
-    b = 3.16 * Units::energy_eV_au();
-    c = 0;
-    d = 0;
-    g = 6.2 * Units::energy_eV_au();
+    if (b_rows)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) kcart[i][j] = b_rows[j][i];
+    const double A = Units::space_A_au(), eV = Units::energy_eV_au();
+    // Default parameters of TightBinding::init (TightBinding.h:44-147).
+    if (is_graphene_family(name)) {
+        a = 2.46 * A; b = 2.97 * eV; c = 0; d = 0; g = 0;
+        if (name == "DLGraphene") { c = 3.35 * A; d = 0.39 * eV; }
+    } else if (name == "Vampa2015") {
+        std::memcpy(vampa, kVampa, sizeof vampa);
+        g = 0.1213;
+    } else if (name == "GeS") {
+        g = 0.41 * eV; f = -2.33 * eV; t2 = 0.61 * eV; b = 0.13 * eV; c = 0.07 * eV; d = -0.09 * eV;
+    } else if (name == "GeS_HSE06") {
+        g = 0.62 * eV; f = 2.91 * eV; t2 = -0.797 * eV; b = -0.05 * eV; c = -0.14 * eV; d = 0.106 * eV;
+    } else {  // boron-nitride family, BN1b, Haldane, gen2d
+        a = 2.5 * A; b = 3.16 * eV; c = 0; d = 0; g = 6.2 * eV;
+    }
     if (is_haldane(name)) {
         if (hopping.size() < 4) throw Error("missing Haldane hopping parameters");
         g = hopping[0]; b = hopping[1]; t2 = hopping[2]; phi0 = hopping[3];
@@ -51,7 +90,8 @@ void TBModel::init(const std::string &nm, int nch, int ncv, const std::vector<do
 }
 
 bool TBModel::supported() const {
-    return name == "BoronNitride" || is_corebn(name) || is_haldane(name) || name == "gen2d_hexagonal";
+    return name == "BoronNitride" || is_corebn(name) || is_haldane(name) || name == "gen2d_hexagonal" || name == "BN1b" ||
+           is_graphene_family(name) || name == "Vampa2015" || is_ges(name);
 }
 
 cplx TBModel::energy(int ic, int jc, const double k[3]) const {
@@ -63,10 +103,64 @@ cplx TBModel::energy(int ic, int jc, const double k[3]) const {
         if (ic == 2) return (jc == 1) ? b * f : (jc == 2 ? cplx(g / 2.) : cplx(0.));
         return 0.;
     }
-    if (name == "BoronNitride") {
-        if (ic == jc) return ic == 0 ? cplx(-g / 2) : (ic == 1 ? cplx(g / 2) : cplx(0));
-        cplx f = std::exp(I1 * 2. * pi * k[1]) + 1. + std::exp(I1 * 2. * pi * k[2]);
-        return ic == 0 ? std::conj(f) * b : f * b;
+    // Nearest-neighbour structure factor of the hexagonal lattice in crystal k.
+    auto fhex = [&]() { return std::exp(I1 * 2. * pi * k[1]) + 1. + std::exp(I1 * 2. * pi * k[2]); };
+    if (name == "BoronNitride" || name == "Graphene") {
+        if (ic == jc) return name == "Graphene" ? cplx(0.) : (ic == 0 ? cplx(-g / 2) : (ic == 1 ? cplx(g / 2) : cplx(0)));
+        cplx fk = fhex();
+        return ic == 0 ? std::conj(fk) * b : fk * b;
+    }
+    if (is_graphene_d(name)) {  // two core levels, the graphene bands already diagonal
+        if (ic == jc && ic < 2) return -10.892178;
+        if (ic >= 2 && jc >= 2 && ic == jc) {
+            double modf = std::abs(fhex());
+            return ic == 2 ? cplx(-b * modf) : cplx(b * modf);
+        }
+        return 0.;
+    }
+    if (name == "CoreGraphene") {
+        if (ic == jc && ic < 2) return -10.892178;
+        if (ic >= 2 && jc >= 2 && ic != jc) {
+            cplx fk = fhex();
+            return ic == 2 ? std::conj(fk) * b : fk * b;
+        }
+        return 0.;
+    }
+    if (name == "GrapheneZurron") {  // the structure factor in cartesian k (TightBinding.h:251-264)
+        if (ic == jc) return 0.;
+        double kc[3];
+        for (int i = 0; i < 3; i++) kc[i] = kcart[i][0] * k[0] + kcart[i][1] * k[1] + kcart[i][2] * k[2];
+        cplx fz = std::exp(-I1 * a * kc[1] / std::sqrt(3.)) * (1. + 2. * std::exp(I1 * std::sqrt(3.) * a * kc[1] / 2.) * std::cos(a * kc[2] / 2.));
+        return ic == 0 ? fz * b : std::conj(fz) * b;
+    }
+    if (name == "DLGraphene") {  // two graphene layers coupled by d between orbitals 0 and 3
+        if (ic == jc) return 0.;
+        cplx fk = fhex();
+        if ((ic == 0 && jc == 1) || (ic == 2 && jc == 3)) return std::conj(fk) * b;
+        if ((ic == 1 && jc == 0) || (ic == 3 && jc == 2)) return fk * b;
+        if ((ic == 3 && jc == 0) || (ic == 0 && jc == 3)) return d;
+        return 0.;
+    }
+    if (name == "BN1b") {  // the conduction band of BoronNitride alone
+        cplx fk = fhex();
+        return 0.5 * std::sqrt(g * g + 4 * b * b * fk * std::conj(fk)).real();
+    }
+    if (name == "Vampa2015") {
+        if (ic != jc) return 0.;
+        cplx e = 0.;
+        for (int x = 0; x < 3; x++)
+            for (int j = 0; j < 6; j++) e += vampa[ic][x][j] * std::cos(j * k[x]);
+        if (ic == 1) e += g;
+        return e;
+    }
+    if (is_ges(name)) {
+        if (ic == jc) {
+            double tor = -2. * c * (std::cos(2. * pi * k[1]) + std::cos(2. * pi * k[2])) - 2. * d * (std::cos(2. * pi * (k[1] - k[2])));
+            return ic == 0 ? cplx(g + tor) : cplx(-g + tor);
+        }
+        cplx phix = std::exp(I1 * 2. * pi * k[1]) + std::exp(I1 * 2. * pi * k[2]);
+        cplx tor = -f - t2 * phix - b * std::conj(phix);
+        return ic == 0 ? std::conj(tor) : tor;
     }
     // Haldane_CoreBN_* and gen2d_hexagonal: core level, then a two-band Haldane
     // block with nearest (b) and next-nearest (t2, phi0) hopping.
@@ -78,14 +172,14 @@ cplx TBModel::energy(int ic, int jc, const double k[3]) const {
     }
     cplx f2 = std::exp(-2. * pi * I1 * k[2] + 2. * pi * I1 * k[1]) + std::exp(2. * pi * I1 * k[2]) +
               std::exp(-2. * pi * I1 * k[1]);
-    cplx f = 1. + std::exp(-2. * pi * I1 * k[1]) + std::exp(-2. * pi * I1 * k[2]);
+    cplx fm = 1. + std::exp(-2. * pi * I1 * k[1]) + std::exp(-2. * pi * I1 * k[2]);
     if (ic == 1) {
         if (jc == 1) return -g / 2. + 2. * std::real(t2 * std::exp(I1 * phi0) * f2);
-        if (jc == 2) return b * (f);
+        if (jc == 2) return b * (fm);
         return 0.;
     }
     if (ic == 2) {
-        if (jc == 1) return b * std::conj(f);
+        if (jc == 1) return b * std::conj(fm);
         if (jc == 2) return g / 2. + 2. * std::real(t2 * std::exp(-I1 * phi0) * f2);
         return 0.;
     }
@@ -97,13 +191,66 @@ cplx TBModel::unitary(int ic, int jc, const double k[3]) const {
     if (is_corebn(name)) {
         if (ic == 0) return jc == 0 ? cplx(1.) : cplx(0);
         if (jc == 0) return 0.;
-        cplx f = 1. + std::exp(2. * pi * I1 * k[1]) + std::exp(2. * pi * I1 * k[2]);
-        double E = std::sqrt(g * g + 4. * b * b * f * std::conj(f)).real() / 2.;
+        cplx fk = 1. + std::exp(2. * pi * I1 * k[1]) + std::exp(2. * pi * I1 * k[2]);
+        double E = std::sqrt(g * g + 4. * b * b * fk * std::conj(fk)).real() / 2.;
         double aa = std::sqrt((g + 2. * E) / (4. * E));
         double h = std::sqrt((2. * E - g) / (4. * E));
-        if (ic == 1) return jc == 1 ? cplx(aa) : -h * std::conj(f) / std::abs(f);
-        if (ic == 2) return jc == 1 ? cplx(h) : aa * std::conj(f) / std::abs(f);
+        if (ic == 1) return jc == 1 ? cplx(aa) : -h * std::conj(fk) / std::abs(fk);
+        if (ic == 2) return jc == 1 ? cplx(h) : aa * std::conj(fk) / std::abs(fk);
         return 0.;
+    }
+    if (is_ges(name)) {
+        cplx phix = std::exp(I1 * 2. * pi * k[1]) + std::exp(I1 * 2. * pi * k[2]);
+        cplx tor = -f - t2 * phix - b * std::conj(phix);
+        return bloch_unitary(ic, jc, std::real(tor), std::imag(tor), g);
+    }
+    // Graphene-like eigenvectors (1, +-f/|f|)/sqrt(2) with f of -k (TightBinding.h:1031-1071).
+    auto fneg = [&]() { return std::exp(-I1 * 2. * pi * k[1]) + 1. + std::exp(-I1 * 2. * pi * k[2]); };
+    if (name == "Graphene") {
+        if (jc == 1) {
+            cplx fk = fneg();
+            return ic == 0 ? -fk / (std::sqrt(2.) * std::abs(fk)) : fk / (std::sqrt(2.) * std::abs(fk));
+        }
+        return 1. / std::sqrt(2.);
+    }
+    if (name == "CoreGraphene") {
+        if (ic < 2) return ic == jc ? cplx(1.) : cplx(0.);
+        if (jc == 3) {
+            cplx fk = fneg();
+            return ic == 2 ? -fk / (std::sqrt(2.) * std::abs(fk)) : +fk / (std::sqrt(2.) * std::abs(fk));
+        }
+        return jc == 2 ? cplx(1. / std::sqrt(2.)) : cplx(0.);
+    }
+    if (is_graphene_d(name) || name == "BN1b" || name == "Vampa2015") return (ic == jc || name == "BN1b") ? cplx(1.) : cplx(0.);
+    if (name == "GrapheneZurron") {
+        if (jc == 1) {
+            double kc[3];
+            for (int i = 0; i < 3; i++) kc[i] = kcart[i][0] * k[0] + kcart[i][1] * k[1] + kcart[i][2] * k[2];
+            cplx fz = std::exp(-I1 * a * kc[1] / std::sqrt(3.)) * (1. + 2. * std::exp(I1 * std::sqrt(3.) * a * kc[1] / 2.) * std::cos(a * kc[2] / 2.));
+            return ic == 0 ? -fz / (std::sqrt(2.) * std::abs(fz)) : fz / (std::sqrt(2.) * std::abs(fz));
+        }
+        return 1. / std::sqrt(2.);
+    }
+    if (name == "DLGraphene") {
+        // Closed-form eigenvectors of the bilayer (TightBinding.h:1072-1138), with
+        // S = sqrt(d^2 + 4 b^2 |f|^2) and the two normalisations N1, N2.
+        cplx fk = std::exp(I1 * 2. * pi * k[1]) + 1. + std::exp(I1 * 2. * pi * k[2]);
+        cplx S = std::sqrt(std::pow(d, 2) + 4 * std::pow(b, 2) * fk * std::conj(fk));
+        cplx Sc = std::conj(S);
+        double N1 = std::sqrt(8 + std::pow(std::abs((d - S) / (b * std::conj(fk))), 2) + 16 * std::pow(std::abs((b * fk) / (d + S)), 2));
+        double N2 = std::sqrt(8 + 16 * std::pow(std::abs((b * fk) / (d - S)), 2) + std::pow(std::abs((d + S) / (b * std::conj(fk))), 2));
+        if (ic == 0 || ic == 3) {
+            const double s = ic == 0 ? -1. : 1.;
+            if (jc == 0) return s * 2 / N1;
+            if (jc == 1) return (-d + Sc) / (b * fk * N1);
+            if (jc == 2) return (s * 4 * b * fk) / (N1 * (d + Sc));
+            return 2 / N1;
+        }
+        const double s = ic == 1 ? 1. : -1.;
+        if (jc == 0) return s * 2 / N2;
+        if (jc == 1) return -((d + Sc) / (b * fk * N2));
+        if (jc == 2) return (s * 4 * b * fk) / (N2 * (d - Sc));
+        return 2 / N2;
     }
     int o = 0;  // first band of the two-band block
     if (name != "BoronNitride") {
@@ -117,13 +264,7 @@ cplx TBModel::unitary(int ic, int jc, const double k[3]) const {
     if (name != "BoronNitride")
         Bz = -g / 2. - 2. * t2 * std::sin(phi0) *
                            (std::sin(-2. * pi * k[2] + 2. * pi * k[1]) + std::sin(+2. * pi * k[2]) + std::sin(-2. * pi * k[1]));
-    double B = std::sqrt(Bx * Bx + By * By + Bz * Bz);
-    double phi = std::atan2(By, Bx) / 2.;
-    double costh2 = std::sqrt((B + Bz) / 2. / B);
-    double sinth2 = std::sqrt((B - Bz) / 2. / B);
-    if (ic == o) return jc == o ? std::exp(+I1 * phi) * sinth2 : -std::exp(-I1 * phi) * costh2;
-    if (ic == o + 1) return jc == o ? std::exp(+I1 * phi) * costh2 : std::exp(-I1 * phi) * sinth2;
-    return 0.;
+    return bloch_unitary(ic - o, jc - o, Bx, By, Bz);
 }
 
 cplx TBModel::dipole(int ic, int jc, int x, const double k[3]) const {
@@ -155,7 +296,38 @@ cplx TBModel::dipole(int ic, int jc, int x, const double k[3]) const {
         if (x == 1) return (ic == 1 && jc == 1) ? cplx(a / std::sqrt(3)) : cplx(0.);
         return 0.;
     }
-    return 0.;
+    // The graphene family and the other lattices (TightBinding.h:1256-1421).
+    const double y_pz = -2.68341111;  // position of the second graphene site along y
+    if (name == "Graphene") return (x == 1 && ic == 1 && jc == 1) ? cplx(y_pz) : cplx(0.);
+    if (name == "CoreGraphene") {
+        if (x == 0) return ((ic == 0 && jc == 2) || (ic == 1 && jc == 3) || (ic == 2 && jc == 0) || (ic == 3 && jc == 1)) ? cplx(0.0780088) : cplx(0.);
+        if (x == 1) return (ic == jc && ic % 2 == 1) ? cplx(y_pz) : cplx(0.);
+        return 0.;
+    }
+    if (is_graphene_d(name)) {
+        if (x == 2 && name == "Graphene_costD") return 0.;
+        if (x == 0) {
+            if (ic <= 1 && jc > 1) {
+                const double dx = 0.0780088 / std::sqrt(2.);
+                return (ic == 0 || (ic == 1 && jc == 2)) ? cplx(dx) : cplx(-dx);
+            }
+            if (ic > 1 && jc <= 1) return std::conj(dipole(jc, ic, 0, k));
+            return 0.;
+        }
+        // y, and z of Graphene_symmetricD (dipolez = dipoley)
+        if (ic == 1 && jc == 1) return y_pz;
+        if (ic > 1 && jc > 1) return ic == jc ? cplx(y_pz / 2) : cplx(-y_pz / 2);
+        return 0.;
+    }
+    if (name == "DLGraphene") {
+        if (ic != jc) return 0.;
+        if (x == 0) return (ic == 2 || ic == 3) ? cplx(c) : cplx(0.);
+        if (x == 1) return ic == 1 ? cplx(-a / std::sqrt(3)) : ((ic == 2 || ic == 3) ? cplx(a / (2 * std::sqrt(3))) : cplx(0.));
+        return ic == 2 ? cplx(a / 2.) : (ic == 3 ? cplx(-a / 2.) : cplx(0.));
+    }
+    if (name == "Vampa2015") return ic != jc ? cplx(x == 2 ? 3.94 : 3.46) : cplx(0.);
+    if (is_ges(name)) return (x == 1 && ic == 1 && jc == 1) ? cplx(0.52 * Units::space_A_au()) : cplx(0.);
+    return 0.;  // BN1b, GrapheneZurron
 }
 
 // ------------------------------------------------------------------ Wannier

@@ -107,7 +107,7 @@ void Setup::build(const std::string &input_path, int flags) {
     mark("table allocation");
     if (in.iMode == "TB") {
         TBModel tb;
-        tb.init(in.TBtype, nch, nb, in.hopping);
+        tb.init(in.TBtype, nch, nb, in.hopping, in.b);
 #pragma omp parallel for schedule(static)
         for (int ik = 0; ik < nk; ik++) {
             const double *k = &kpt[(size_t)ik * 3];
@@ -255,7 +255,7 @@ std::vector<double> Setup::disp_at_reference_point() const {
     std::vector<double> d((size_t)nb * 3, 0.0);
     if (in.iMode == "TB") {
         TBModel tb;
-        tb.init(in.TBtype, nch, nb, in.hopping);
+        tb.init(in.TBtype, nch, nb, in.hopping, in.b);
         for (int i = 0; i < nb; i++)
             for (int x = 0; x < 3; x++) d[(size_t)i * 3 + x] = tb.dipole(i, i, x, k).real();
     } else if (in.iMode == "CY") {

@@ -111,6 +111,7 @@ hip.edus_batch_step_rk4.argtypes = [C.c_void_p, C.c_int32, C.POINTER(_dp), C.POI
 hip.edus_batch_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
 hip.edus_batch_member.argtypes = [C.c_void_p, C.c_int32]
 hip.edus_batch_member.restype = C.c_void_p
+hip.edus_stream_probe.argtypes = [C.c_int64, C.c_int32, _dp, _dp, _dp]
 
 # ---- host
 host.edus_setup_from_input.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]
@@ -405,6 +406,14 @@ def wannier_hk(nk3, R, HR, rR, nw, fp32=False):
             _dptr(rR.view(np.float64)), _dptr(H.view(np.float64)), _dptr(D.view(np.float64)))
     _check_hip(rc, "edus_wannier_hk_fp32" if fp32 else "edus_wannier_hk")
     return H, D
+
+
+def stream_probe(nbytes=1 << 30, reps=10):
+    """Copy, triad and read rates (GB/s) of this library's streaming kernels on the
+    current device (edus_stream_probe)."""
+    cp, tr, rd = C.c_double(), C.c_double(), C.c_double()
+    _check_hip(hip.edus_stream_probe(int(nbytes), int(reps), C.byref(cp), C.byref(tr), C.byref(rd)), "edus_stream_probe")
+    return {"copy": cp.value, "triad": tr.value, "read": rd.value}
 
 
 def write_synthetic_tb(path, nw, rmax, a_ang=2.5, sigma_h_ev=1.5, sigma_r_ang=0.3, decay_ang=2.0, seed=1001):

@@ -135,17 +135,19 @@ def boron_nitride(directory, nk=48, nsteps=3000, dt=0.05, e_v_ang=0.3, photon_ev
     return write(os.path.join(directory, "input.txt"), text)
 
 
-def batch_scan(directory, n=256, seed=3003, nsteps=3000, dt=0.05):
+def batch_scan(directory, n=256, seed=3003, nsteps=3000, dt=0.05, grids=(48, 96, 144, 192), rmax=10):
     """Config 3: n independent instances.  Grids from {48, 96, 144, 192}^2, band
     counts from {2 (BoronNitride), 4, 8 (seeded Wannier, seed + i)}, and per
     instance a gaussian pump with peak field U[0.05, 0.6] V/A, CEP U[0, 2 pi),
     polarisation angle U[0, pi/3) in the yz plane, photon energy U[0.4, 1.0] eV
-    and intensity FWHM U[10, 20] fs; RK4.  Returns the input paths."""
+    and intensity FWHM U[10, 20] fs; RK4.  Returns the input paths.  `grids` and
+    `rmax` shrink the same draw for parity tests (the draws do not depend on
+    them)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     paths = []
     for i in range(n):
-        nk = int(rng.choice([48, 96, 144, 192]))
+        nk = int(grids[rng.choice(len(grids))])
         nb = int(rng.choice([2, 4, 8]))
         e0 = float(rng.uniform(0.05, 0.6))
         cep = float(rng.uniform(0.0, 2.0 * math.pi))
@@ -158,7 +160,7 @@ def batch_scan(directory, n=256, seed=3003, nsteps=3000, dt=0.05):
             paths.append(boron_nitride(d, nk=nk, nsteps=nsteps, dt=dt, e_v_ang=e0, photon_ev=w, fwhm_fs=fwhm, pol=pol,
                                        phase=cep))
         else:
-            paths.append(wannier_synthetic(d, nw=nb, rmax=10, nk=nk, nsteps=nsteps, dt=dt, e_v_ang=e0, nvalence=nb // 2,
+            paths.append(wannier_synthetic(d, nw=nb, rmax=rmax, nk=nk, nsteps=nsteps, dt=dt, e_v_ang=e0, nvalence=nb // 2,
                                            seed=seed + i, photon_ev=w, fwhm_fs=fwhm, pol=pol, phase=cep))
     return paths
 

@@ -251,6 +251,15 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
 int edus_wannier_hk_fp32(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R,
                          const double *HR, const double *rR, double *H_out, double *D_out);
 
+/* Streaming rates of the current device measured with this library's own
+ * kernels (16-byte accesses, grid-stride, the best of several workgroups-per-CU
+ * choices, each the mean of `reps` launches after one untimed): copy moves
+ * 2 x bytes per launch, triad a = b + s c 3 x bytes, read 1 x bytes; GB/s in
+ * 1e9 bytes.  No reference counterpart: the streaming ceiling the stage kernels'
+ * measured bandwidth is read against, and the known-size kernel the HBM
+ * counters are calibrated with.  Allocates three arrays of `bytes` bytes. */
+int edus_stream_probe(int64_t bytes, int32_t reps, double *copy_gbs, double *triad_gbs, double *read_gbs);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char *edus_last_error(void);
 

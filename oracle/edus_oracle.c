@@ -274,12 +274,182 @@ void orc_neighbours(const double *kpt, long nk, const double *b, const double *b
 
 /* ------------------------------------------------------------------ tight binding (TightBinding.h)
  * model: 0 BoronNitride, 1 CoreBN_Nedge, 2 CoreBN_Bedge, 3 Haldane_CoreBN_Nedge,
- * 4 Haldane_CoreBN_Bedge, 5 gen2d_hexagonal.
- * par: a, g, b, t2, phi0, core, rchv, rchc (a.u.) */
-typedef struct orc_tb { int model; double a, g, b, t2, phi0, core, rchv, rchc; } orc_tb;
+ * 4 Haldane_CoreBN_Bedge, 5 gen2d_hexagonal, 6 Graphene, 7 CoreGraphene,
+ * 8 Graphene_costD, 9 Graphene_symmetricD, 10 GrapheneZurron, 11 DLGraphene,
+ * 12 BN1b, 13 Vampa2015, 14 GeS, 15 GeS_HSE06.
+ * par: a, g, b, t2, phi0, core, rchv, rchc (a.u.) for models 0-5; models 6-15
+ * carry their own constants (TightBinding::init, TightBinding.h:44-147).
+ * kc: crystal -> cartesian k, kc_i = sum_j kc[3 i + j] k_j (GrapheneZurron). */
+typedef struct orc_tb { int model; double a, g, b, t2, phi0, core, rchv, rchc; double kc[9]; } orc_tb;
+
+#define ORC_EV (1.0 / 27.211396)
+#define ORC_ANG (1.0 / 0.52917721092)
+
+/* Constants of the models that take no input parameters (TightBinding.h:44-147). */
+typedef struct tb_const { double a, b, c, d, f, g, t2; } tb_const;
+static tb_const tb_constants(int model) {
+    tb_const q = {0, 0, 0, 0, 0, 0, 0};
+    if (model >= 6 && model <= 11) { q.a = 2.46 * ORC_ANG; q.b = 2.97 * ORC_EV; }
+    if (model == 11) { q.c = 3.35 * ORC_ANG; q.d = 0.39 * ORC_EV; }
+    if (model == 12) { q.a = 2.5 * ORC_ANG; q.b = 3.16 * ORC_EV; q.g = 6.2 * ORC_EV; }
+    if (model == 13) q.g = 0.1213;
+    if (model == 14) { q.g = 0.41 * ORC_EV; q.f = -2.33 * ORC_EV; q.t2 = 0.61 * ORC_EV; q.b = 0.13 * ORC_EV; q.c = 0.07 * ORC_EV; q.d = -0.09 * ORC_EV; }
+    if (model == 15) { q.g = 0.62 * ORC_EV; q.f = 2.91 * ORC_EV; q.t2 = -0.797 * ORC_EV; q.b = -0.05 * ORC_EV; q.c = -0.14 * ORC_EV; q.d = 0.106 * ORC_EV; }
+    return q;
+}
+
+/* Vampa2015 coefficients: [band][direction][j] of sum_x sum_j c cos(j k_x) (TightBinding.h:98-121). */
+static const double VAMPA[2][3][6] = {
+    {{-0.0928, 0.0705, 0.0200, -0.0012, 0.0029, 0.0006}, {-0.0307, 0.0307, 0, 0, 0, 0}, {-0.0059, 0.0059, 0, 0, 0, 0}},
+    {{0.0898, -0.0814, -0.0024, -0.0048, -0.0003, -0.0009}, {0.1147, -0.1147, 0, 0, 0, 0}, {0.0435, -0.0435, 0, 0, 0, 0}}};
+
+/* Structure factor of the hexagonal lattice, crystal k (and of -k). */
+static cplx fhex(const double *k) { return cexp(I * 2. * ORC_PI * k[1]) + 1. + cexp(I * 2. * ORC_PI * k[2]); }
+static cplx fhex_neg(const double *k) { return cexp(-I * 2. * ORC_PI * k[1]) + 1. + cexp(-I * 2. * ORC_PI * k[2]); }
+/* GrapheneZurron's structure factor in cartesian k (TightBinding.h:251-264). */
+static cplx fzurron(const orc_tb *m, const double *k, double a) {
+    double kc[3];
+    for (int i = 0; i < 3; i++) kc[i] = m->kc[3 * i] * k[0] + m->kc[3 * i + 1] * k[1] + m->kc[3 * i + 2] * k[2];
+    return cexp(-I * a * kc[1] / sqrt(3.)) * (1. + 2. * cexp(I * sqrt(3.) * a * kc[1] / 2.) * cos(a * kc[2] / 2.));
+}
+
+/* The two-band eigenvector form from (Bx, By, Bz) (TightBinding.h:958-1029). */
+static cplx tb_bloch_u(int r, int col, double Bx, double By, double Bz) {
+    double B = sqrt(Bx * Bx + By * By + Bz * Bz), phi = atan2(By, Bx) / 2.;
+    double c2 = sqrt((B + Bz) / 2. / B), s2 = sqrt((B - Bz) / 2. / B);
+    if (r == 0) return col == 0 ? cexp(I * phi) * s2 : -cexp(-I * phi) * c2;
+    return col == 0 ? cexp(I * phi) * c2 : cexp(-I * phi) * s2;
+}
+
+/* Energy, unitary and dipole of models 6-15 (TightBinding.h:175-459, 892-1213, 1256-1421). */
+static cplx tb_energy_more(const orc_tb *m, int ic, int jc, const double *k) {
+    const double pi = ORC_PI;
+    const tb_const q = tb_constants(m->model);
+    switch (m->model) {
+        case 6:
+            if (ic == jc) return 0.;
+            return ic == 0 ? conj(fhex(k)) * q.b : fhex(k) * q.b;
+        case 7:
+            if (ic == jc && ic < 2) return -10.892178;
+            if (ic >= 2 && jc >= 2 && ic != jc) return ic == 2 ? conj(fhex(k)) * q.b : fhex(k) * q.b;
+            return 0.;
+        case 8:
+        case 9:
+            if (ic == jc && ic < 2) return -10.892178;
+            if (ic >= 2 && ic == jc) return ic == 2 ? -q.b * cabs(fhex(k)) : q.b * cabs(fhex(k));
+            return 0.;
+        case 10:
+            if (ic == jc) return 0.;
+            return ic == 0 ? fzurron(m, k, q.a) * q.b : conj(fzurron(m, k, q.a)) * q.b;
+        case 11:
+            if (ic == jc) return 0.;
+            if ((ic == 0 && jc == 1) || (ic == 2 && jc == 3)) return conj(fhex(k)) * q.b;
+            if ((ic == 1 && jc == 0) || (ic == 3 && jc == 2)) return fhex(k) * q.b;
+            if ((ic == 3 && jc == 0) || (ic == 0 && jc == 3)) return q.d;
+            return 0.;
+        case 12: {
+            cplx f = fhex(k);
+            return 0.5 * creal(csqrt(q.g * q.g + 4 * q.b * q.b * f * conj(f)));
+        }
+        case 13: {
+            if (ic != jc) return 0.;
+            cplx e = 0.;
+            for (int x = 0; x < 3; x++)
+                for (int j = 0; j < 6; j++) e += VAMPA[ic][x][j] * cos(j * k[x]);
+            if (ic == 1) e += q.g;
+            return e;
+        }
+        default: {  /* GeS, GeS_HSE06 */
+            if (ic == jc) {
+                double tor = -2. * q.c * (cos(2. * pi * k[1]) + cos(2. * pi * k[2])) - 2. * q.d * (cos(2. * pi * (k[1] - k[2])));
+                return ic == 0 ? q.g + tor : -q.g + tor;
+            }
+            cplx phix = cexp(I * 2. * pi * k[1]) + cexp(I * 2. * pi * k[2]);
+            cplx tor = -q.f - q.t2 * phix - q.b * conj(phix);
+            return ic == 0 ? conj(tor) : tor;
+        }
+    }
+}
+
+static cplx tb_unitary_more(const orc_tb *m, int ic, int jc, const double *k) {
+    const double pi = ORC_PI;
+    const tb_const q = tb_constants(m->model);
+    switch (m->model) {
+        case 6:
+            if (jc == 1) return (ic == 0 ? -1. : 1.) * fhex_neg(k) / (sqrt(2.) * cabs(fhex_neg(k)));
+            return 1. / sqrt(2.);
+        case 7:
+            if (ic < 2) return ic == jc ? 1. : 0.;
+            if (jc == 3) return (ic == 2 ? -1. : 1.) * fhex_neg(k) / (sqrt(2.) * cabs(fhex_neg(k)));
+            return jc == 2 ? 1. / sqrt(2.) : 0.;
+        case 10:
+            if (jc == 1) return (ic == 0 ? -1. : 1.) * fzurron(m, k, q.a) / (sqrt(2.) * cabs(fzurron(m, k, q.a)));
+            return 1. / sqrt(2.);
+        case 11: {
+            /* eigenvectors of the bilayer block with S = sqrt(d^2 + 4 b^2 |f|^2) */
+            const double b = q.b, d = q.d;
+            cplx f = fhex(k);
+            double S = creal(csqrt(d * d + 4 * b * b * f * conj(f)));
+            double N1 = sqrt(8 + pow(cabs((d - S) / (b * conj(f))), 2) + 16 * pow(cabs((b * f) / (d + S)), 2));
+            double N2 = sqrt(8 + 16 * pow(cabs((b * f) / (d - S)), 2) + pow(cabs((d + S) / (b * conj(f))), 2));
+            static const double sgn[4] = {-1., 1., -1., 1.};
+            if (ic == 0 || ic == 3) {
+                if (jc == 0) return sgn[ic] * 2 / N1;
+                if (jc == 1) return (-d + S) / (b * f * N1);
+                if (jc == 2) return sgn[ic] * 4 * b * f / (N1 * (d + S));
+                return 2 / N1;
+            }
+            if (jc == 0) return sgn[ic] * 2 / N2;
+            if (jc == 1) return -(d + S) / (b * f * N2);
+            if (jc == 2) return sgn[ic] * 4 * b * f / (N2 * (d - S));
+            return 2 / N2;
+        }
+        case 12: return 1.;
+        case 14:
+        case 15: {
+            cplx phix = cexp(I * 2. * pi * k[1]) + cexp(I * 2. * pi * k[2]);
+            cplx tor = -q.f - q.t2 * phix - q.b * conj(phix);
+            return tb_bloch_u(ic, jc, creal(tor), cimag(tor), q.g);
+        }
+        default: return ic == jc ? 1. : 0.;  /* Graphene_costD / symmetricD, Vampa2015 */
+    }
+}
+
+static cplx tb_dipole_more(const orc_tb *m, int ic, int jc, int x) {
+    const tb_const q = tb_constants(m->model);
+    const double ypz = -2.68341111;
+    switch (m->model) {
+        case 6: return (x == 1 && ic == 1 && jc == 1) ? ypz : 0.;
+        case 7:
+            if (x == 0) return ((ic == 0 && jc == 2) || (ic == 1 && jc == 3) || (ic == 2 && jc == 0) || (ic == 3 && jc == 1)) ? 0.0780088 : 0.;
+            if (x == 1) return (ic == jc && ic % 2 == 1) ? ypz : 0.;
+            return 0.;
+        case 8:
+        case 9:
+            if (x == 0) {
+                int lo = ic < jc ? ic : jc, hi = ic < jc ? jc : ic;
+                if (!(lo <= 1 && hi > 1)) return 0.;
+                return (lo == 0 || hi == 2) ? 0.0780088 / sqrt(2.) : -(0.0780088 / sqrt(2.));
+            }
+            if (x == 2 && m->model == 8) return 0.;
+            if (ic == 1 && jc == 1) return ypz;
+            if (ic > 1 && jc > 1) return ic == jc ? ypz / 2 : -ypz / 2;
+            return 0.;
+        case 11:
+            if (ic != jc) return 0.;
+            if (x == 0) return ic >= 2 ? q.c : 0.;
+            if (x == 1) return ic == 1 ? -q.a / sqrt(3) : (ic >= 2 ? q.a / (2 * sqrt(3)) : 0.);
+            return ic == 2 ? q.a / 2. : (ic == 3 ? -q.a / 2. : 0.);
+        case 13: return ic != jc ? (x == 2 ? 3.94 : 3.46) : 0.;
+        case 14:
+        case 15: return (x == 1 && ic == 1 && jc == 1) ? 0.52 * ORC_ANG : 0.;
+        default: return 0.;  /* GrapheneZurron, BN1b */
+    }
+}
 
 static cplx tb_energy(const orc_tb *m, int ic, int jc, const double *k) {
     const double pi = ORC_PI;
+    if (m->model >= 6) return tb_energy_more(m, ic, jc, k);
     if (m->model == 1 || m->model == 2) {
         cplx f = 1. + cexp(2. * pi * I * k[1]) + cexp(2. * pi * I * k[2]);
         if (ic == 0) return jc == 0 ? (m->model == 1 ? -15.06 : -6.91) : 0;
@@ -309,6 +479,7 @@ static cplx tb_energy(const orc_tb *m, int ic, int jc, const double *k) {
 
 static cplx tb_unitary(const orc_tb *m, int ic, int jc, const double *k) {
     const double pi = ORC_PI;
+    if (m->model >= 6) return tb_unitary_more(m, ic, jc, k);
     if (m->model == 1 || m->model == 2) {
         if (ic == 0) return jc == 0 ? 1. : 0.;
         if (jc == 0) return 0.;
@@ -335,6 +506,7 @@ static cplx tb_unitary(const orc_tb *m, int ic, int jc, const double *k) {
 }
 
 static cplx tb_dipole(const orc_tb *m, int ic, int jc, int x) {
+    if (m->model >= 6) return tb_dipole_more(m, ic, jc, x);
     if (m->model == 5) {
         if (x == 0) {
             if ((ic == 0 && jc == 1) || (ic == 1 && jc == 0)) return m->rchv;

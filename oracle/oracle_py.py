@@ -32,7 +32,8 @@ class OPulse(C.Structure):
 
 class OTB(C.Structure):
     _fields_ = [("model", C.c_int), ("a", C.c_double), ("g", C.c_double), ("b", C.c_double), ("t2", C.c_double),
-                ("phi0", C.c_double), ("core", C.c_double), ("rchv", C.c_double), ("rchc", C.c_double)]
+                ("phi0", C.c_double), ("core", C.c_double), ("rchv", C.c_double), ("rchc", C.c_double),
+                ("kc", C.c_double * 9)]
 
 
 class OTables(C.Structure):
@@ -192,14 +193,23 @@ def neighbours(kpt, b, bvec):
 
 
 TB_MODELS = {"BoronNitride": 0, "CoreBN_Nedge": 1, "CoreBN_Bedge": 2, "Haldane_CoreBN_Nedge": 3,
-             "Haldane_CoreBN_Bedge": 4, "gen2d_hexagonal": 5}
+             "Haldane_CoreBN_Bedge": 4, "gen2d_hexagonal": 5, "Graphene": 6, "CoreGraphene": 7, "Graphene_costD": 8,
+             "Graphene_symmetricD": 9, "GrapheneZurron": 10, "DLGraphene": 11, "BN1b": 12, "Vampa2015": 13, "GeS": 14,
+             "GeS_HSE06": 15}
 
 
-def tb_tables(model, params, kpt, nb):
+def tb_tables(model, params, kpt, nb, kcart=None):
+    """Tables of a tight-binding model on crystal k points.  params: a, g, b, t2,
+    phi0, core, rchv, rchc (models with input parameters); kcart: 3x3 crystal ->
+    cartesian matrix (columns the reciprocal vectors), for models written in
+    cartesian k."""
     m = OTB()
     m.model = TB_MODELS[model]
-    for k, v in params.items():
+    for k, v in (params or {}).items():
         setattr(m, k, v)
+    if kcart is not None:
+        for i, v in enumerate(np.asarray(kcart, dtype=np.float64).reshape(9)):
+            m.kc[i] = v
     nk = len(kpt)
     H = np.zeros((nk, nb, nb), dtype=np.complex128)
     U = np.zeros_like(H)

@@ -179,7 +179,7 @@ def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, cou
         nbr = setup.nbr()
     name, par = setup.model()
     if par is not None and use_oracle_tables:
-        H, U, D = O.tb_tables(name, par, kpt, nb)
+        H, U, D = O.tb_tables(name, par, kpt, nb, kcart=b.T)
     else:
         H = setup.array("H", True).reshape(-1, nb, nb)
         U = setup.array("U", True).reshape(-1, nb, nb)

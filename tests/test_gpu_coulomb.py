@@ -34,7 +34,7 @@ def test_rhs_with_mean_field_matches_oracle(tmp_path):
         assert np.max(np.abs(pv - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("stepper", ["", "Taylor\nTaylorOrder 6\n"])
+@pytest.mark.parametrize("stepper", ["", "Taylor\nTaylorOrder 6\n", "Adams_Bashforth 4\n"])
 def test_trajectory_with_mean_field_matches_oracle(tmp_path, stepper):
     s = N.Setup(Hh.coulomb_input(str(tmp_path), nk=24, stepper=stepper, dt=0.1, nsteps=400, pump_int=5e13))
     osys = Hh.oracle_system(s, coulomb=True)
