@@ -39,7 +39,8 @@ void fill_problem(const Setup &s, edus_problem &p) {
     p.tabs_norm = std::fabs(s.L.jacobian) * dk[0] * dk[1] * dk[2];
     p.nktot = (double)s.nk;
     p.taylor_tau_inv = s.tau_inv.empty() ? nullptr : s.tau_inv.data();
-    for (int i = 0; i < 3; i++) p.nk_dims[i] = s.in.Nk[i];
+    // An explicit k list (nkPT read) is no grid: no strip order over it.
+    for (int i = 0; i < 3; i++) p.nk_dims[i] = s.in.kptread ? (i == 0 ? s.nk : 1) : s.in.Nk[i];
 }
 
 void fill_coulomb(const Setup &s, edus_coulomb &c) {

@@ -89,6 +89,7 @@ struct Input {
     double b[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     int Nk[3] = {0, 0, 0};
     bool kptread = false;
+    std::vector<double> kpt_list;          // nkPT read: [Nk0 Nk1 Nk2][3] crystal k points of the input
     int Nb[3] = {0, 0, 0};
     double change_gap_constant = 0;
     std::vector<Pulse> pumps;
@@ -145,8 +146,12 @@ struct Shells {
 
 // k-grid of BZ_parameters (InitParameters.h:13-40): ik -> (i0,i1,i2), i2 fastest.
 std::vector<double> make_kgrid(const int Nk[3]);
-Shells find_shells(const std::vector<double> &kpt, const int Nk[3], const Lattice &L);
+// grid: kpt is the uniform grid of Nk (shell vectors then carry integer offsets);
+// false for an explicit k list (nkPT read).
+Shells find_shells(const std::vector<double> &kpt, const int Nk[3], const Lattice &L, bool grid = true);
 std::vector<int32_t> neighbours_closed_form(const int Nk[3], const Shells &sh);
+// The reference's search of k + b in an explicit k list, by hashing (first match in list order).
+std::vector<int32_t> neighbours_search(const std::vector<double> &kpt, const Shells &sh, const Lattice &L);
 std::vector<int32_t> neighbours_bruteforce(const std::vector<double> &kpt, const Shells &sh, const Lattice &L);
 
 // ---------------------------------------------------------------- small linear algebra

@@ -275,8 +275,17 @@ Input parse_input(const std::string &path) {
             end_section(r, key);
         } else if (ieq(key, "nkpt")) {
             if (!(r.f >> in.Nk[0] >> in.Nk[1] >> in.Nk[2])) fail("Cannot read Nkpt because the file ended. Correct your input.%s%s");
-            if (str.size() == 2 && ieq(str[1], "read")) in.kptread = true;
-            if (in.kptread) throw Error("nkPT read (explicit k lists) is not supported by this build");
+            if (str.size() == 2) {
+                if (ieq(str[1], "read")) in.kptread = true;
+                else std::printf("%s can't have units %s. Please modify your input.", str[0].c_str(), str[1].c_str());
+            }
+            if (in.kptread) {  // the k points follow the sizes, three crystal coordinates each (ReadInput.h:432-439)
+                const long long n = (long long)in.Nk[0] * in.Nk[1] * in.Nk[2];
+                if (n < 1) fail("Cannot read Nkpt because the file ended. Correct your input.%s%s");
+                in.kpt_list.resize((size_t)n * 3);
+                for (long long i = 0; i < 3 * n; i++)
+                    if (!(r.f >> in.kpt_list[(size_t)i])) fail("Cannot read the k points of nkPT read: the list ended early.%s%s");
+            }
             end_section(r, key);
         } else if (ieq(key, "bands")) {
             s.clear();

@@ -10,7 +10,9 @@
 #include "host.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <unordered_map>
 
 namespace edus {
 
@@ -66,8 +68,8 @@ static double norm3(const double v[3]) { return std::sqrt(v[0] * v[0] + v[1] * v
 
 // Shells of grid points nearest Gamma, added until the B1 + fourth-moment system
 // A w = q (21 rows) is solved to 1e-7 (CalculateWeigths.h:18-133).
-Shells find_shells(const std::vector<double> &kpt, const int Nk[3], const Lattice &L) {
-    for (int c = 0; c < 3; c++)
+Shells find_shells(const std::vector<double> &kpt, const int Nk[3], const Lattice &L, bool grid) {
+    for (int c = 0; c < 3 && grid; c++)
         if (Nk[c] > 1 && (Nk[c] % 2) != 0)
             throw Error("k-grid sizes must be even (no Gamma point on the grid otherwise)");
     int plane = -1;
@@ -140,6 +142,7 @@ Shells find_shells(const std::vector<double> &kpt, const int Nk[3], const Lattic
         if (count + 1 >= 32) throw Error("More than 32 shells needed. The program could be very slow!");
     }
     // Integer grid offsets of each b-vector: b is the grid point ik relative to Gamma.
+    if (!grid) return sh;
     for (auto &mem : shell_members)
         for (int ik : mem) {
             int ii[3] = {(ik / (Nk[2] * Nk[1])) % Nk[0], (ik / Nk[2]) % Nk[1], ik % Nk[2]};
@@ -163,6 +166,67 @@ std::vector<int32_t> neighbours_closed_form(const int Nk[3], const Shells &sh) {
                 jj[c] = v < 0 ? v + Nk[c] : v;
             }
             nbr[(size_t)ik * nn + n] = (jj[0] * Nk[1] + jj[1]) * Nk[2] + jj[2];
+        }
+    }
+    return nbr;
+}
+
+// k + b in an explicit list: the reference's matching rule (k+b in cartesian, back
+// to crystal, wrapped with fmod(x+0.5,1)-0.5; a list point matches when every
+// coordinate differs by less than 1e-8 modulo 1; the first match in list order)
+// with the list hashed on a 1e-6 lattice: each point is entered with its
+// images at +-1 in every coordinate, and a query looks at the 27 cells around it.
+std::vector<int32_t> neighbours_search(const std::vector<double> &kpt, const Shells &sh, const Lattice &L) {
+    const int nk = (int)(kpt.size() / 3);
+    std::vector<std::array<double, 3>> bv;
+    for (auto &s : sh.bvec)
+        for (auto &v : s) bv.push_back(v);
+    const int nn = (int)bv.size();
+    const double q = 1.e6;
+    auto key = [](long long a, long long b, long long c) {
+        return (uint64_t)(a * 73856093LL) ^ (uint64_t)(b * 19349663LL) ^ (uint64_t)(c * 83492791LL);
+    };
+    std::unordered_multimap<uint64_t, int> cells;
+    cells.reserve((size_t)nk * 27);
+    for (int j = 0; j < nk; j++)
+        for (int a = -1; a <= 1; a++)
+            for (int b = -1; b <= 1; b++)
+                for (int c = -1; c <= 1; c++)
+                    cells.emplace(key(std::llround((kpt[(size_t)j * 3] + a) * q), std::llround((kpt[(size_t)j * 3 + 1] + b) * q),
+                                      std::llround((kpt[(size_t)j * 3 + 2] + c) * q)),
+                                  j);
+    std::vector<int32_t> nbr((size_t)nk * nn, -1);
+    for (int ik = 0; ik < nk; ik++) {
+        double kc[3];
+        L.to_cart(&kpt[(size_t)ik * 3], kc);
+        for (int n = 0; n < nn; n++) {
+            double pc[3] = {kc[0] + bv[n][0], kc[1] + bv[n][1], kc[2] + bv[n][2]};
+            double cr[3];
+            L.to_crys(pc, cr);
+            for (int c = 0; c < 3; c++) cr[c] = std::fmod(cr[c] + 0.5, 1.) - 0.5;
+            const long long c0 = std::llround(cr[0] * q), c1 = std::llround(cr[1] * q), c2 = std::llround(cr[2] * q);
+            int found = -1;
+            for (int a = -1; a <= 1; a++)
+                for (int b = -1; b <= 1; b++)
+                    for (int c = -1; c <= 1; c++) {
+                        auto range = cells.equal_range(key(c0 + a, c1 + b, c2 + c));
+                        for (auto it = range.first; it != range.second; ++it) {
+                            const int j = it->second;
+                            if (found >= 0 && j >= found) continue;
+                            bool ok = true;
+                            for (int d = 0; d < 3 && ok; d++) {
+                                const double dd = cr[d] - kpt[(size_t)j * 3 + d];
+                                ok = std::fabs(dd) < 1.e-08 || std::fabs(dd - 1.) < 1.e-08 || std::fabs(dd + 1.) < 1.e-08;
+                            }
+                            if (ok) found = j;
+                        }
+                    }
+            if (found < 0) {
+                char msg[256];
+                std::snprintf(msg, sizeof msg, "k + b = (%.6e, %.6e, %.6e) of k point %d is not in the k list", cr[0], cr[1], cr[2], ik);
+                throw Error(msg);
+            }
+            nbr[(size_t)ik * nn + n] = found;
         }
     }
     return nbr;

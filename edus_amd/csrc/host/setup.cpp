@@ -55,7 +55,7 @@ void Setup::build(const std::string &input_path, int flags) {
 
     for (int c = 0; c < 3; c++)
         if (in.Nk[c] < 1) throw Error("nkPT section missing or invalid");
-    kpt = make_kgrid(in.Nk);
+    kpt = in.kptread ? in.kpt_list : make_kgrid(in.Nk);
     nk = in.Nk[0] * in.Nk[1] * in.Nk[2];
     nb = in.Nb[0] + in.Nb[1] + in.Nb[2];
     nch = in.Nb[0];
@@ -78,9 +78,11 @@ void Setup::build(const std::string &input_path, int flags) {
         if (nb > 4) throw Error("the Coulomb term is implemented for up to 4 bands");
         if (in.coulomb_read) read_coulomb();
     }
-    sh = find_shells(kpt, in.Nk, L);
-    nbr = neighbours_closed_form(in.Nk, sh);
-    const int nn = (int)sh.offsets.size();
+    if (in.kptread && in.coulomb) throw Error("the Coulomb term needs the uniform k-grid (not nkPT read)");
+    sh = find_shells(kpt, in.Nk, L, !in.kptread);
+    nbr = in.kptread ? neighbours_search(kpt, sh, L) : neighbours_closed_form(in.Nk, sh);
+    int nn = 0;
+    for (auto &v : sh.bvec) nn += (int)v.size();
     if (flags & 1) {
         std::vector<int32_t> ref = neighbours_bruteforce(kpt, sh, L);
         if (ref != nbr) throw Error("closed-form neighbour table differs from the brute-force search");
@@ -144,10 +146,41 @@ void Setup::build(const std::string &input_path, int flags) {
         std::vector<double> HR((size_t)w.nR * nb * nb * 2), rR((size_t)w.nR * nb * nb * 6);
         std::memcpy(HR.data(), w.HR.data(), HR.size() * sizeof(double));
         std::memcpy(rR.data(), w.rR.data(), rR.size() * sizeof(double));
-        auto *contract = (flags & 2) ? edus_wannier_hk_fp32 : edus_wannier_hk;
-        int rc = contract(in.Nk, w.nR, nb, w.R.data(), HR.data(), rR.data(), reinterpret_cast<double *>(H.data()),
+        int rc = 0;
+        if (in.kptread) {
+            // An explicit k list has no per-direction phase tables: the sums of
+            // Wannier.h:278-304, 336-471 term by term (e^{2 pi i k.R} per k and R),
+            // H with a real diagonal, D = (r + r^T)/2 with a real diagonal.
+            const double tp = 2.0 * Units::pi();
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int ik = 0; ik < nk; ik++) {
+                const double *k = &kpt[(size_t)ik * 3];
+                std::vector<cplx> h((size_t)nb * nb, 0.), r((size_t)nb * nb * 3, 0.);
+                for (int q = 0; q < w.nR; q++) {
+                    const int32_t *R = &w.R[(size_t)q * 3];
+                    const cplx ph = std::exp(cplx(0.0, tp * (k[0] * R[0] + k[1] * R[1] + k[2] * R[2])));
+                    for (int t = 0; t < nb * nb; t++) {
+                        h[t] += ph * w.HR[(size_t)q * nb * nb + t];
+                        for (int x = 0; x < 3; x++) r[(size_t)t * 3 + x] += ph * w.rR[((size_t)q * nb * nb + t) * 3 + x];
+                    }
+                }
+                for (int i = 0; i < nb; i++)
+                    for (int j = 0; j < nb; j++) {
+                        cplx v = h[(size_t)i * nb + j];
+                        H[mi(nb, ik, i, j)] = (i == j) ? cplx(v.real(), 0.0) : v;
+                        for (int x = 0; x < 3; x++) {
+                            cplx a = r[((size_t)i * nb + j) * 3 + x], b = r[((size_t)j * nb + i) * 3 + x];
+                            if (i == j) { a = cplx(a.real(), 0.0); b = cplx(b.real(), 0.0); }
+                            D[mi(nb, ik, i, j) * 3 + x] = 0.5 * (a + b);
+                        }
+                    }
+            }
+        } else {
+            auto *contract = (flags & 2) ? edus_wannier_hk_fp32 : edus_wannier_hk;
+            rc = contract(in.Nk, w.nR, nb, w.R.data(), HR.data(), rR.data(), reinterpret_cast<double *>(H.data()),
                           reinterpret_cast<double *>(D.data()));
-        if (rc != 0) throw Error(std::string("Wannier contraction on the GPU failed: ") + edus_last_error());
+            if (rc != 0) throw Error(std::string("Wannier contraction on the GPU failed: ") + edus_last_error());
+        }
         mark("Wannier contraction");
         // U from the eigenvectors of H(k) (Wannier.h:496-542): rows are conj(v_a),
         // batched on the GPU like the contraction.
