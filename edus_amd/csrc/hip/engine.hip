@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1687,8 +1688,10 @@ __global__ __launch_bounds__(256) void k_wannier_mfma32(const cd *__restrict__ P
 // Contraction output [k][c][i][j] (c = 0 H, 1..3 r) to the caller's layouts,
 // one thread per (k, i, j): H with zero diagonal imaginary parts
 // (Wannier.h:489-492) and D = (r + r^T)/2 with zero diagonal imaginary parts
-// (Wannier.h:577-578, save_HUD.cpp:187-191), [k][i][j][x].
-__global__ void k_wannier_finish(const cd *__restrict__ Out, int kc, int nw, double *__restrict__ Hd, double *__restrict__ Dd) {
+// (Wannier.h:577-578, save_HUD.cpp:187-191), [k][i][j][x].  rtx: the r part of
+// each output row is [i][j][x] (the caller's own order) instead of [x][i][j].
+__global__ void k_wannier_finish(const cd *__restrict__ Out, int kc, int nw, double *__restrict__ Hd, double *__restrict__ Dd,
+                                 int rtx) {
     const int nw2 = nw * nw;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)kc * nw2) return;
@@ -1700,7 +1703,8 @@ __global__ void k_wannier_finish(const cd *__restrict__ Out, int kc, int nw, dou
     if (!Dd) return;
 #pragma unroll
     for (int x = 0; x < 3; x++) {
-        const cd a = src[(x + 1) * nw2 + i * nw + j], b = src[(x + 1) * nw2 + j * nw + i];
+        const cd a = rtx ? src[nw2 + (i * nw + j) * 3 + x] : src[(x + 1) * nw2 + i * nw + j];
+        const cd b = rtx ? src[nw2 + (j * nw + i) * 3 + x] : src[(x + 1) * nw2 + j * nw + i];
         const double ai = (i == j) ? 0.0 : a.y, bi = (i == j) ? 0.0 : b.y;
         Dd[(3 * t + x) * 2] = 0.5 * (a.x + b.x);
         Dd[(3 * t + x) * 2 + 1] = 0.5 * (ai + bi);
@@ -2102,6 +2106,37 @@ int check_ctx(edus_ctx *c) {
     if (!c) return set_err(EDUS_ERR_INVALID, "null context");
     hipError_t e = hipSetDevice(c->dev);
     if (e != hipSuccess) return set_err(EDUS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+// ==================================================================== fixed scratch
+// Device buffers of edus_wannier_hk_noalloc: one copy of this array per device,
+// held from the library's load until the process ends.  Host code never uses
+// the name itself as an address (on the host it is only a stand-in): the
+// device address comes from hipGetSymbolAddress, per device (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[EDUS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call with that device current and kept.
+int noalloc_scratch(void **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_err(EDUS_ERR_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
     return 0;
 }
 
@@ -3192,6 +3227,20 @@ int edus_coulomb_coefficients(const edus_coulomb_gen *g, double *Ao, double *Bo,
     return 0;
 }
 
+// Twiddles e^{2 pi i m / N_l} of the three grid directions, evaluated on the
+// host in long double, and the offset of each direction's run.
+static void wannier_twiddles(const int32_t nk3[3], std::vector<double> &tw, int offs[3]) {
+    tw.clear();
+    for (int l = 0; l < 3; l++) {
+        offs[l] = (int)(tw.size() / 2);
+        for (int m = 0; m < nk3[l]; m++) {
+            long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)nk3[l];
+            tw.push_back((double)cosl(ang));
+            tw.push_back((double)sinl(ang));
+        }
+    }
+}
+
 static int wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR, double *H_out,
                      double *D_out, bool fp32) {
     if (!nk3 || !R || !HR || !rR || !H_out || nR < 1 || nw < 1) return set_err(EDUS_ERR_INVALID, "bad Wannier arguments");
@@ -3212,17 +3261,9 @@ static int wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_
                 val[((size_t)r * nm + (x + 1) * nw2 + t) * 2 + 1] = rR[((size_t)r * nw2 + t) * 6 + 2 * x + 1];
             }
         }
-    // twiddles e^{2 pi i m / N_l}, evaluated on the host in long double
     std::vector<double> tw;
     int offs[3];
-    for (int l = 0; l < 3; l++) {
-        offs[l] = (int)(tw.size() / 2);
-        for (int m = 0; m < nk3[l]; m++) {
-            long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)nk3[l];
-            tw.push_back((double)cosl(ang));
-            tw.push_back((double)sinl(ang));
-        }
-    }
+    wannier_twiddles(nk3, tw, offs);
     int *dR = nullptr;
     double *dVal = nullptr, *dTw = nullptr, *dOut = nullptr, *dPT = nullptr, *dH = nullptr, *dD = nullptr;
     auto cleanup = [&]() {
@@ -3256,7 +3297,7 @@ static int wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_
             hipLaunchKernelGGL(k_wannier_mfma, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR, nm,
                                kc, (int)k0, (cd *)dOut);
         const long long nt = (long long)kc * nw2;
-        hipLaunchKernelGGL(k_wannier_finish, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, 0, (const cd *)dOut, kc, nw, dH, dD);
+        hipLaunchKernelGGL(k_wannier_finish, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, 0, (const cd *)dOut, kc, nw, dH, dD, 0);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpy(H_out + (size_t)k0 * nw2 * 2, dH, (size_t)nt * 16, hipMemcpyDeviceToHost);
         if (e == hipSuccess && D_out) e = hipMemcpy(D_out + (size_t)k0 * nw2 * 6, dD, (size_t)nt * 48, hipMemcpyDeviceToHost);
@@ -3278,6 +3319,68 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
 int edus_wannier_hk_fp32(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR,
                          double *H_out, double *D_out) {
     return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, true);
+}
+
+// The contraction of edus_wannier_hk with every device buffer carved from the
+// fixed scratch array: lattice vectors, twiddles, phase tables, H(R) and r(R)
+// (copied in with 2D copies, the r part of each row in the caller's [i][j][x]
+// order, so no host repacking), then the grid in as many k chunks as the rest
+// of the array holds.  Nothing in the array needs clearing: every buffer is
+// written before it is read.
+int edus_wannier_hk_noalloc(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR,
+                            double *H_out, double *D_out) {
+    if (!nk3 || !R || !HR || !rR || !H_out || nR < 1 || nw < 1) return set_err(EDUS_ERR_INVALID, "bad Wannier arguments");
+    for (int l = 0; l < 3; l++)
+        if (nk3[l] < 1) return set_err(EDUS_ERR_INVALID, "bad grid");
+    const long long nk = (long long)nk3[0] * nk3[1] * nk3[2];
+    const long long nw2 = (long long)nw * nw, nm = 4 * nw2, nsum = (long long)nk3[0] + nk3[1] + nk3[2];
+    auto al = [](long long b) { return (b + 255) / 256 * 256; };
+    const long long oR = 0, oTw = al(oR + 12LL * nR), oPT = al(oTw + 16 * nsum), oVal = al(oPT + 16LL * nR * nsum),
+                    oOut = al(oVal + 16LL * nR * nm);
+    const long long per_k = 16 * nm + 16 * nw2 + (D_out ? 48 * nw2 : 0), slack = 512;
+    const long long cap = EDUS_NOALLOC_SCRATCH_BYTES;
+    if (oOut + per_k + slack > cap)
+        return set_err(EDUS_ERR_UNSUPPORTED, "edus_wannier_hk_noalloc: the model needs " + std::to_string(oOut + per_k + slack) +
+                                                 " bytes of scratch, above the " + std::to_string(cap) +
+                                                 " of EDUS_NOALLOC_SCRATCH_BYTES (use edus_wannier_hk)");
+    const long long chunk = std::min<long long>(nk, (cap - oOut - slack) / per_k);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(EDUS_ERR_NODEVICE, "no HIP device visible");
+    void *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    char *b = static_cast<char *>(base);
+    int *dR = reinterpret_cast<int *>(b + oR);
+    double *dTw = reinterpret_cast<double *>(b + oTw), *dPT = reinterpret_cast<double *>(b + oPT);
+    cd *dVal = reinterpret_cast<cd *>(b + oVal), *dOut = reinterpret_cast<cd *>(b + oOut);
+    const long long oH = al(oOut + 16 * chunk * nm), oD = al(oH + 16 * chunk * nw2);
+    double *dH = reinterpret_cast<double *>(b + oH), *dD = D_out ? reinterpret_cast<double *>(b + oD) : nullptr;
+    std::vector<double> tw;
+    int offs[3];
+    wannier_twiddles(nk3, tw, offs);
+    HIP_TRY(hipMemcpy(dR, R, (size_t)nR * 3 * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dTw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+    // Val[R] = [H(R) (nw^2) | r(R) as [i][j][x] (3 nw^2)]
+    HIP_TRY(hipMemcpy2D(dVal, (size_t)nm * 16, HR, (size_t)nw2 * 16, (size_t)nw2 * 16, (size_t)nR, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy2D(dVal + nw2, (size_t)nm * 16, rR, (size_t)3 * nw2 * 16, (size_t)3 * nw2 * 16, (size_t)nR,
+                        hipMemcpyHostToDevice));
+    const size_t npt = (size_t)nR * nsum;
+    hipLaunchKernelGGL(k_wannier_ptab, dim3((unsigned)std::min<size_t>((npt + 255) / 256, 65536)), dim3(256), 0, 0, (const int *)dR,
+                       (const cd *)dTw, nR, nk3[0], nk3[1], nk3[2], offs[1], offs[2], (cd *)dPT);
+    HIP_TRY(hipGetLastError());
+    for (long long k0 = 0; k0 < nk; k0 += chunk) {
+        const int kc = (int)std::min<long long>(chunk, nk - k0);
+        const dim3 grid((unsigned)((nm + WMN - 1) / WMN), (unsigned)((kc + WMK - 1) / WMK));
+        hipLaunchKernelGGL(k_wannier_mfma, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR,
+                           (int)nm, kc, (int)k0, dOut);
+        const long long nt = (long long)kc * nw2;
+        hipLaunchKernelGGL(k_wannier_finish, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, 0, (const cd *)dOut, kc, nw, dH, dD, 1);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpy(H_out + (size_t)k0 * nw2 * 2, dH, (size_t)nt * 16, hipMemcpyDeviceToHost));
+        if (D_out) HIP_TRY(hipMemcpy(D_out + (size_t)k0 * nw2 * 6, dD, (size_t)nt * 48, hipMemcpyDeviceToHost));
+    }
+    g_err.clear();
+    return 0;
 }
 
 }  // extern "C"

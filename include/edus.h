@@ -251,6 +251,27 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
 int edus_wannier_hk_fp32(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R,
                          const double *HR, const double *rR, double *H_out, double *D_out);
 
+/* Size of the fixed device array behind edus_wannier_hk_noalloc (one copy per
+ * device, held from the library's load until the process ends). */
+#define EDUS_NOALLOC_SCRATCH_BYTES ((int64_t)128 << 20)
+
+/* edus_wannier_hk without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * device buffer (lattice vectors, phase tables, H(R) and r(R), output chunks)
+ * is carved from the library's fixed scratch array; the call allocates no device
+ * memory and uses no allocating library routine.  Same arguments and the same
+ * results as edus_wannier_hk, bit for bit.
+ * Limit: 12 nR + 16 (Nk0+Nk1+Nk2)(nR+1) + 64 nR nw^2 bytes, plus one k of
+ * output (128 nw^2 bytes with D_out, 64 nw^2 without) and 1 KiB of alignment,
+ * must fit EDUS_NOALLOC_SCRATCH_BYTES (a 32-band model with 961 lattice vectors
+ * needs 79 MB); a larger model returns EDUS_ERR_UNSUPPORTED before touching the
+ * device.  The grid is walked in as many k chunks as the rest of the array
+ * holds.  Calls run on the null stream and share the device's one array, so
+ * calls that overlap on one device race. */
+int edus_wannier_hk_noalloc(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR,
+                            double *H_out, double *D_out);
+
 /* Streaming rates of the current device measured with this library's own
  * kernels (16-byte accesses, grid-stride, the best of several workgroups-per-CU
  * choices, each the mean of `reps` launches after one untimed): copy moves
