@@ -21,7 +21,7 @@ void fill_problem(const Setup &s, edus_problem &p) {
     p.nb = s.nb;
     p.nch = s.nch;
     p.ncc = s.ncc;
-    p.n_nbr = (int32_t)s.sh.offsets.size();
+    p.n_nbr = (int32_t)(s.nbr_wb.size() / 3);  // b vectors over all shells (grid or list)
     p.stepper = s.stepper;
     p.order = s.order;
     p.nbr = s.nbr.data();
@@ -52,6 +52,7 @@ void fill_coulomb(const Setup &s, edus_coulomb &c) {
     c.D = reinterpret_cast<const double *>(s.cD.data());
     c.screen = reinterpret_cast<const double *>(s.cscreen.data());
     c.v_hartree = reinterpret_cast<const double *>(s.cvh.data());
+    c.diag_basis = s.in.coulomb_diag_basis ? 1 : 0;
 }
 }  // namespace edus
 
@@ -100,7 +101,7 @@ int edus_setup_get_info(const edus_setup *h, edus_setup_info *info) {
     info->nk0 = s.in.Nk[0]; info->nk1 = s.in.Nk[1]; info->nk2 = s.in.Nk[2]; info->nk = s.nk;
     info->nb = s.nb; info->nch = s.nch; info->ncc = s.ncc;
     for (int i = 0; i < 3; i++) info->nbands[i] = s.in.Nb[i];
-    info->n_nbr = (int32_t)s.sh.offsets.size();
+    info->n_nbr = (int32_t)(s.nbr_wb.size() / 3);
     info->n_shells = (int32_t)s.sh.bvec.size();
     info->stepper = s.stepper; info->order = s.order;
     info->it_resolution = s.in.it_resolution; info->i_current = s.in.iCurrent; info->i_tabs = s.in.iTAbs;

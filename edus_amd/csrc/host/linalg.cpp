@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 namespace edus {
 
@@ -29,37 +30,61 @@ void inv3(const double A[3][3], double Ainv[3][3], double *det) {
 
 // Least squares min ||A w - q|| for a full-column-rank m x n A (m >= n), by
 // Householder QR.  Equals pinv(A) q when A has full column rank.
+// Minimum-norm least squares w = pinv(A) q (the reference calls Armadillo's pinv,
+// CalculateWeigths.h:91-100): singular value decomposition by one-sided Jacobi
+// rotations of A's columns (A V = U S), then w = V S^+ U^T q with singular values
+// at or below max(m, n) * max(s) * eps treated as zero (pinv's default
+// tolerance).  The shell systems that are rank-deficient (lattices with shells of
+// two vectors, the orthorhombic and rhombic ones) need the minimum-norm solution.
 std::vector<double> lstsq(const std::vector<double> &A0, int m, int n, const std::vector<double> &q0) {
-    std::vector<double> A = A0, q = q0;
-    for (int j = 0; j < n; j++) {
-        double nrm = 0;
-        for (int i = j; i < m; i++) nrm += A[i * n + j] * A[i * n + j];
-        nrm = std::sqrt(nrm);
-        if (nrm == 0) continue;
-        double alpha = A[j * n + j] > 0 ? -nrm : nrm;
-        std::vector<double> v(m, 0.0);
-        for (int i = j; i < m; i++) v[i] = A[i * n + j];
-        v[j] -= alpha;
-        double vv = 0;
-        for (int i = j; i < m; i++) vv += v[i] * v[i];
-        if (vv == 0) continue;
-        for (int c = j; c < n; c++) {
-            double s = 0;
-            for (int i = j; i < m; i++) s += v[i] * A[i * n + c];
-            s = 2 * s / vv;
-            for (int i = j; i < m; i++) A[i * n + c] -= s * v[i];
-        }
-        double s = 0;
-        for (int i = j; i < m; i++) s += v[i] * q[i];
-        s = 2 * s / vv;
-        for (int i = j; i < m; i++) q[i] -= s * v[i];
+    std::vector<double> U = A0, V((size_t)n * n, 0.0);
+    for (int j = 0; j < n; j++) V[(size_t)j * n + j] = 1.0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int sweep = 0; sweep < 100; sweep++) {
+        bool rotated = false;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                double a = 0, b = 0, g = 0;
+                for (int i = 0; i < m; i++) {
+                    const double up = U[(size_t)i * n + p], uq = U[(size_t)i * n + q];
+                    a += up * up;
+                    b += uq * uq;
+                    g += up * uq;
+                }
+                if (g == 0.0 || std::fabs(g) <= eps * std::sqrt(a * b)) continue;
+                rotated = true;
+                const double zeta = (b - a) / (2.0 * g);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), sn = c * t;
+                for (int i = 0; i < m; i++) {
+                    const double up = U[(size_t)i * n + p], uq = U[(size_t)i * n + q];
+                    U[(size_t)i * n + p] = c * up - sn * uq;
+                    U[(size_t)i * n + q] = sn * up + c * uq;
+                }
+                for (int i = 0; i < n; i++) {
+                    const double vp = V[(size_t)i * n + p], vq = V[(size_t)i * n + q];
+                    V[(size_t)i * n + p] = c * vp - sn * vq;
+                    V[(size_t)i * n + q] = sn * vp + c * vq;
+                }
+            }
+        if (!rotated) break;
     }
+    std::vector<double> sv(n, 0.0);
+    double smax = 0;
+    for (int j = 0; j < n; j++) {
+        double s2 = 0;
+        for (int i = 0; i < m; i++) s2 += U[(size_t)i * n + j] * U[(size_t)i * n + j];
+        sv[j] = std::sqrt(s2);
+        smax = std::max(smax, sv[j]);
+    }
+    const double tol = std::max(m, n) * smax * eps;
     std::vector<double> w(n, 0.0);
-    for (int j = n - 1; j >= 0; j--) {
-        double s = q[j];
-        for (int c = j + 1; c < n; c++) s -= A[j * n + c] * w[c];
-        double d = A[j * n + j];
-        w[j] = (std::fabs(d) < 1e-300) ? 0.0 : s / d;
+    for (int j = 0; j < n; j++) {
+        if (!(sv[j] > tol)) continue;
+        double uq = 0;  // (U_j / s_j)^T q
+        for (int i = 0; i < m; i++) uq += U[(size_t)i * n + j] * q0[i];
+        const double coef = uq / (sv[j] * sv[j]);
+        for (int r = 0; r < n; r++) w[r] += V[(size_t)r * n + j] * coef;
     }
     return w;
 }

@@ -41,6 +41,8 @@ def test_listed_grid_reproduces_the_grid(tmp_path, model):
         text, kpt = _listed(str(tmp_path / "input.txt"), order)
         s = _setup(tmp_path, label, text)
         assert s.info["nk"] == nk
+        # the engine's stencil width: every b vector of every shell, as on the grid
+        assert s.info["n_nbr"] == g.info["n_nbr"] == s.problem().n_nbr == s.nbr().shape[1]
         np.testing.assert_array_equal(s.array("kpt").reshape(-1, 3), kpt)
         for name in ("H", "U", "D", "rho0"):
             np.testing.assert_array_equal(s.array(name, True), g.array(name, True).reshape(nk, -1)[order].reshape(-1))

@@ -131,6 +131,9 @@ def test_shells_and_neighbours_of_the_other_lattices(tmp_path, model):
     np.testing.assert_array_equal(o["nbr"], s.nbr())
     assert o["res"] < 1e-7
     w = np.repeat(o["w"], o["sizes"])
+    # the weights are pinv(A) q, the minimum-norm solution: these lattices' shell
+    # systems are rank-deficient, where another least-squares solution differs
+    np.testing.assert_allclose(s.array("nbr_wb").reshape(-1, 3), w[:, None] * o["bvec"], rtol=0, atol=1e-13)
     # sum_b w b_a b_b = delta_ab in the plane of the grid
     M = np.einsum("n,na,nb->ab", w, o["bvec"], o["bvec"])
     np.testing.assert_allclose(M[1:, 1:], np.eye(2), atol=1e-8)
