@@ -7,13 +7,14 @@ vectors (|R1|,|R2| <= 15, hexagonal, a = 2.5 A, exp(-|R|/3 A) decay, seed 4004),
 RK4 step (4 fused RHS+stage launches) over all k, with the state and every
 operator resident in HBM before the timed region.
 
-The line carries both forms of the stage kernel: the plain FP64 path (the
-drop-in, `value`) and the mixed form (FP32 operators and products, FP64 state;
-`mixed`), with the mixed form's J(t), final conduction population and
-harmonic-intensity errors against the FP64 path over a window of steps at the
-pulse peak.
+The line carries the forms of the stage kernel: the plain FP64 path (the
+drop-in, `value`), the mixed form (FP32 operators and products, FP64 state;
+`mixed`) and the mixed form with FP32 stage storage (`mixed32`: FP32 stage
+inputs and histories, FP64 state at t_n and accumulator), each mixed form with
+its J(t), final conduction population and harmonic-intensity errors against the
+FP64 path over a window of steps at the pulse peak.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload ...] [--forms fp64,mixed] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload ...] [--forms fp64,mixed,mixed32] [--no-cpu]
 
 Prints one JSON line.
 """
@@ -51,10 +52,11 @@ def main():
     ap.add_argument("--nw", type=int, default=16)
     ap.add_argument("--rmax", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,mixed",
+    ap.add_argument("--forms", default="fp64,mixed,mixed32",
                     help="stage-kernel forms to time, the first one is `value`: fp64 (the drop-in), mixed (FP32 "
-                         "operators and products, FP64 state; 5 bands or more)")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                         "operators and products, FP64 state), mixed32 (mixed plus FP32 stage inputs and histories, "
+                         "FP64 master state and accumulator); the mixed forms need 5 bands or more")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed32"], default=None,
                     help="time one form only (same as --forms <form>)")
     ap.add_argument("--stepper", choices=["rk4", "taylor4"], default="rk4",
                     help="wannier workloads: classic RK4 (four stage launches a step) or the Taylor-4 multistep "
@@ -70,6 +72,9 @@ def main():
     ap.add_argument("--cpu-subgrid", type=int, default=128,
                     help="grid points per direction of the CPU sample when the workload's grid is above 200x200")
     ap.add_argument("--no-probe", action="store_true", help="skip the streaming-rate probe")
+    ap.add_argument("--graph", action="store_true",
+                    help="RK4 steps from a device-resident field table through the captured step graph "
+                         "(edus_set_field_table + edus_step_rk4_graph) instead of per-call stage launches")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's streaming probe on arrays of MIB MiB (its copy kernel is the "
@@ -138,10 +143,16 @@ def stage_model(info, nd, form):
 
     nb = info["nb"]
     rk4 = info["stepper"] == N.STEPPER_RK4
-    transfers = 4 if rk4 else info["order"] + 2 + (1 if info["ncut"] else 0)
     opw = 8 if form == "fp64" else 4
     diss = info["n_diss"] > 0
-    per_k = transfers * 8 * nb * nb + (1 + nd) * opw * nb * nb + (2 * opw * nb * nb if diss else 0)
+    if form == "mixed32":
+        # FP32 stage inputs / histories (4 nb^2 B), FP64 state at t_n and accumulator (8 nb^2 B):
+        # RK4 stages move 20, 32, 32, 20 nb^2 B of state; order-s multistep 16 + 4 s
+        state_k = (26 if rk4 else 16 + 4 * info["order"]) * nb * nb
+    else:
+        transfers = 4 if rk4 else info["order"] + 2 + (1 if info["ncut"] else 0)
+        state_k = transfers * 8 * nb * nb
+    per_k = state_k + (1 + nd) * opw * nb * nb + (2 * opw * nb * nb if diss else 0)
     nbt = nb if nb <= 4 else 8 if nb <= 8 else 16 if nb <= 16 else 32
     flop_k = (8 * nbt ** 3) * (5 if diss else 1) if nbt >= 16 else 0
     S = 16 * nb * nb
@@ -157,15 +168,23 @@ def time_form(s, form, f, g, args):
     clock around a synchronised call; kernel time from the engine's HIP events)."""
     from edus_amd import native as N
 
-    prec = N.PRECISION_MIXED if form == "mixed" else N.PRECISION_FP64
+    prec = {"fp64": N.PRECISION_FP64, "mixed": N.PRECISION_MIXED, "mixed32": N.PRECISION_MIXED32}[form]
     t0 = time.perf_counter()
     eng = N.Engine(s.problem(), precision=prec)
     t_create = time.perf_counter() - t0
-    if args.warmup:
-        eng.step(f[:args.warmup], g[:args.warmup])
-    t0 = time.perf_counter()
-    eng.step(f[args.warmup:], g[args.warmup:])
-    wall = time.perf_counter() - t0
+    if getattr(args, "graph", False) and eng.stepper == N.STEPPER_RK4:
+        eng.set_field_table(f, g)
+        if args.warmup:
+            eng.step_graph(args.warmup)
+        t0 = time.perf_counter()
+        eng.step_graph(args.steps)
+        wall = time.perf_counter() - t0
+    else:
+        if args.warmup:
+            eng.step(f[:args.warmup], g[:args.warmup])
+        t0 = time.perf_counter()
+        eng.step(f[args.warmup:], g[args.warmup:])
+        wall = time.perf_counter() - t0
     dev_ms, nlaunch = eng.last_step_time()
     return eng, {"wall_s": wall, "dev_ms": dev_ms, "nlaunch": nlaunch, "create_s": t_create}
 
@@ -186,7 +205,7 @@ def bench_single(args):
     t_setup = time.perf_counter() - t_setup
     nk, nb = info["nk"], info["nb"]
     if nb < 5:
-        forms = [x for x in forms if x != "mixed"] or ["fp64"]
+        forms = [x for x in forms if x == "fp64"] or ["fp64"]
     if args.end_to_end:
         s.run(os.path.join(wd, "warm"), max_steps=args.warmup)
         t0 = time.perf_counter()
@@ -229,7 +248,8 @@ def bench_single(args):
             "device_ms_per_step": t["dev_ms"] / args.steps, "avg_launch_us": round(avg_launch_s * 1e6, 3),
             "launches_per_step": t["nlaunch"] // args.steps, "engine_create_s": round(t["create_s"], 2),
             "kernel": m["kernel"] + (" (RK4 stages)" if m["stepper"] == "RK4" else f" ({m['stepper']} steps)")
-                      + ("" if form == "fp64" else ", FP32 operators and products"),
+                      + {"fp64": "", "mixed": ", FP32 operators and products",
+                         "mixed32": ", FP32 operators, products and stage storage"}[form],
             "bytes_per_launch": bytes_launch, "achieved_gbs": round(achieved, 2),
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "mfma": ({"flop_per_launch": m["flop_k"] * nk, "achieved_tflops": round(tflops, 2),
@@ -255,17 +275,17 @@ def bench_single(args):
         roof["stream_probe_gbs"] = {k: round(v, 1) for k, v in probe.items()}
         roof["frac_of_copy"] = round(lead["achieved_gbs"] / probe["copy"], 4)
 
-    mixed = None
-    if "mixed" in legs and args.acc_steps > 0:
-        acc = mixed_j_error(s, engines["mixed"], it0, args.acc_steps, ref=engines.get("fp64"),
-                            ref_setup=N.Setup(inp) if args.contraction == "fp32" else None)
-    else:
-        acc = None
-    if "mixed" in legs:
-        mixed = dict(legs["mixed"])
+    mixed_legs = {}
+    ref_setup = N.Setup(inp) if args.contraction == "fp32" and any(f != "fp64" for f in legs) else None
+    for form in legs:
+        if form == "fp64":
+            continue
+        leg = dict(legs[form])
         if "fp64" in legs:
-            mixed["speedup_vs_fp64"] = round(legs["mixed"]["value"] / legs["fp64"]["value"], 4)
-        mixed["error_vs_fp64"] = acc
+            leg["speedup_vs_fp64"] = round(legs[form]["value"] / legs["fp64"]["value"], 4)
+        leg["error_vs_fp64"] = (mixed_j_error(s, engines[form], it0, args.acc_steps, ref=engines.get("fp64"),
+                                              ref_setup=ref_setup) if args.acc_steps > 0 else None)
+        mixed_legs[form] = leg
     for e in engines.values():
         e.close()
 
@@ -290,18 +310,24 @@ def bench_single(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f64/f32",
+        "forms": {"fp64": "the drop-in: FP64 throughout",
+                  "mixed": "FP32 operators and commutator/dephasing products on the matrix cores, FP64 state",
+                  "mixed32": "mixed plus FP32 stage inputs and derivative histories; FP64 state at t_n, FP64 "
+                             "accumulator and updates"},
         "data": "synthetic",
         "config": {"workload": wname, "nk": nk, "bands": nb, "stepper": stepper_name(info),
-                   "form": head, "wannier_contraction": args.contraction if args.workload in ("wannier", "wannier32") else None,
+                   "form": head, "launch": "captured step graph" if args.graph else "stream launches per call",
+                   "wannier_contraction": args.contraction if args.workload in ("wannier", "wannier32") else None,
                    "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
                    "device_ms_per_step": lead["device_ms_per_step"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    if mixed is not None and head != "mixed":
-        out["mixed"] = mixed
-    elif head == "mixed":
-        out["j_error_vs_fp64"] = mixed["error_vs_fp64"] if mixed else None
+    for form, leg in mixed_legs.items():
+        if form == head:
+            out["j_error_vs_fp64"] = leg["error_vs_fp64"]
+        else:
+            out[form] = leg
     print(json.dumps(out))
 
 
@@ -496,8 +522,9 @@ def cpu_baseline(s, steps, subgrid=False):
     t0 = time.perf_counter()
     _, Pc = osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
     el = time.perf_counter() - t0
-    if not steps and el < 10.0:  # scale the sample to ~10-30 s of CPU work
-        n = max(2, int(n * 12.0 / max(el, 1e-3)))
+    n_left = info["itfi"] + 1 - it0  # the steps the run has from it0 on
+    if not steps and el < 10.0:  # scale the sample to ~10-30 s of CPU work, within the run
+        n = max(2, min(n_left, int(n * 12.0 / max(el, 1e-3))))
         t0 = time.perf_counter()
         _, Pc = osys["sys"].run(P0, 0, 4, info["dt"], t_start, n, 10 ** 9, pumps, probe)
         el = time.perf_counter() - t0

@@ -7,6 +7,7 @@ Outputs (git-ignored, shipped with the tree):
     edus_amd/lib/libedus_host.so   C++ host (input, setup, time loop), include/edus_host.h
     edus_amd/lib/edus_run          drop-in executable:  edus_run input.txt [outdir]
 """
+import hashlib
 import os
 import subprocess
 import sys
@@ -37,11 +38,32 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+def _digest(paths):
+    h = hashlib.sha256()
+    for p in paths:
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def _run_stable(cmd, sources):
+    """hipcc reads each source once per pass (host, then each device target): a
+    source edited while it runs gives a library whose host part launches kernels
+    that its device code object lacks.  Such a build is removed and fails."""
+    before = _digest(sources)
+    _run(cmd)
+    if _digest(sources) != before:
+        out = cmd[cmd.index("-o") + 1]
+        os.remove(out)
+        raise RuntimeError(f"sources changed while {os.path.basename(out)} was built; build again")
+
+
 def build_engine():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libedus_hip.so")
     srcs = [os.path.join(CSRC, "hip", f) for f in ("engine.hip", "stream.hip")]
-    _run([HIPCC, *HIP_FLAGS, *srcs, "-o", out, "-lrocsolver", "-lrocblas"])
+    headers = [os.path.join(ROOT, "include", "edus.h")]
+    _run_stable([HIPCC, *HIP_FLAGS, *srcs, "-o", out, "-lrocsolver", "-lrocblas"], srcs + headers)
     return out
 
 

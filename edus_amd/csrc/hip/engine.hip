@@ -80,12 +80,22 @@ struct ConstArgs {
     const double *EH;          // [nb] Hartree shift (minus equilibrium)
     const double *screen;      // [nb*nb] complex
     double *Hr;                // packed renormalised H (written)
+    int coul_diag;             // Coulomb_diag_basis: the sums ran on PD, corrections in the eigenbasis
+    const double *PD;          // packed state in the eigenbasis, conj(U) P U^T (diagonal basis)
+    // A stencil that is the same integer offset at every k of the uniform grid
+    // (every grid of the reference's BZ_parameters): the kernels form k + b from k
+    // and the offset instead of loading it from nbr, which saves the dependent
+    // round trip to memory before the gather.  gn0 = 0: read the table.
+    int gn0, gn1, gn2;
+    signed char goff[kMaxNbr][3];
 };
 
 // Per-launch stage data.  `tab` points at the stage's row of the field table:
 // [0..2] pump sum, [3..5] half probe, [6..6+n_nbr) gradient coefficients
 // c_n = sum_x E_pump[x] * w_shell b_n[x], [6+n_nbr] gradient flag.
 struct StageArgs {
+    const int *ctr;      // device step counter (field table path), or null: the row is `tab` itself
+    long long ctr_stride;  // doubles per counted step of the field table
     const double *Y;     // stage input (neighbours gathered from it)
     const double *P0;    // RK: state at t_n
     double *ACC;         // RK: accumulator
@@ -101,6 +111,13 @@ struct StageArgs {
     double dtp[EDUS_MAX_ORDER];
     double ca, cb, dt;
 };
+
+// The field-table row of a stage: `tab`, advanced by the device step counter when
+// the launch belongs to a captured step (its arguments are fixed at capture, the
+// counter moves on between replays).
+__device__ __forceinline__ const double *stage_row(const StageArgs &a) {
+    return a.ctr ? a.tab + (size_t)(*a.ctr) * (size_t)a.ctr_stride : a.tab;
+}
 
 // ---------------------------------------------------------------- complex helpers
 struct cd {
@@ -164,6 +181,28 @@ __device__ __forceinline__ int strip_k(int L, int nk, int sw, int scol) {
     return (u / wr) * scol + nfull * sw + u % wr;
 }
 
+// Index of neighbour n of k (k < nk): from the grid offsets when the context has
+// them (offsets are within half the grid, so one wrap per axis suffices), else
+// from the table.
+struct GridPos {
+    int i0, i1, i2;
+};
+__device__ __forceinline__ GridPos grid_pos(const ConstArgs &c, int k) {
+    if (!c.gn0) return GridPos{0, 0, 0};
+    const int n12 = c.gn1 * c.gn2, i0 = k / n12, r = k - i0 * n12, i1 = r / c.gn2;
+    return GridPos{i0, i1, r - i1 * c.gn2};
+}
+__device__ __forceinline__ int wrap1(int v, int n) {
+    v += v < 0 ? n : 0;
+    return v >= n ? v - n : v;
+}
+__device__ __forceinline__ int nbr_of(const ConstArgs &c, const GridPos &g, int k, int n) {
+    if (c.gn0)
+        return (wrap1(g.i0 + c.goff[n][0], c.gn0) * c.gn1 + wrap1(g.i1 + c.goff[n][1], c.gn1)) * c.gn2 +
+               wrap1(g.i2 + c.goff[n][2], c.gn2);
+    return c.nbr[(size_t)n * c.nkp + k];
+}
+
 // Pair index of (i<j) in row-major upper-triangle order.
 __host__ __device__ __forceinline__ int pair_index(int i, int j, int nb) { return i * nb - i * (i + 1) / 2 + (j - i - 1); }
 
@@ -173,13 +212,17 @@ __host__ __device__ __forceinline__ int pair_index(int i, int j, int nb) { retur
 __device__ __forceinline__ double ldnt(const double *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void stnt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void stnt(float *p, float v) { __builtin_nontemporal_store(v, p); }
 
 // ---------------------------------------------------------------- stage update
 // The elementwise tail shared by all kernels.  `kv` is one packed double of the
 // right-hand side, `y` the same double of the stage input, `o` its offset in the
 // state arrays and `h` the stride between history slots.
-template <int MODE>
+// S: storage type of the Adams-Bashforth / Taylor history (double, or float in
+// the mixed form with FP32 stage storage); the state and its update stay FP64.
+template <int MODE, typename S = double>
 __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_t h, double kv, double y) {
+    S *const hist = reinterpret_cast<S *>(a.HIST);
     if constexpr (MODE == RK1) {
         stnt(a.ACC + o, y + kv * a.ca);
         stnt(a.OUT + o, y + kv * a.cb);
@@ -191,10 +234,10 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_
     } else if constexpr (MODE == ABM) {
         // the order-1 history loads are issued together (unrolled to the largest
         // order, 5) before the sum, which keeps the order of the coefficients
-        stnt(a.HIST + (size_t)a.slot_out * h + o, kv);
+        stnt(hist + (size_t)a.slot_out * h + o, (S)kv);
         double hv[5];
 #pragma unroll
-        for (int s = 1; s < 5; s++) hv[s] = (s < a.order) ? ldnt(a.HIST + (size_t)a.slots[s] * h + o) : 0.0;
+        for (int s = 1; s < 5; s++) hv[s] = (s < a.order) ? (double)ldnt(hist + (size_t)a.slots[s] * h + o) : 0.0;
         double delta = a.coef[0] * kv;
 #pragma unroll
         for (int s = 1; s < 5; s++)
@@ -203,11 +246,11 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_
     } else if constexpr (MODE == RHSO) {
         a.OUT[o] = kv;
     } else {  // TAY
-        stnt(a.HIST + (size_t)a.slot_out * h + o, kv);
+        stnt(hist + (size_t)a.slot_out * h + o, (S)kv);
         const int n = a.order - 1;
         double dp[EDUS_MAX_ORDER];
 #pragma unroll
-        for (int i = 0; i < EDUS_MAX_ORDER - 1; i++) dp[i] = (i < n) ? kv - ldnt(a.HIST + (size_t)a.slots[i + 1] * h + o) : 0.0;
+        for (int i = 0; i < EDUS_MAX_ORDER - 1; i++) dp[i] = (i < n) ? kv - (double)ldnt(hist + (size_t)a.slots[i + 1] * h + o) : 0.0;
         double delta = kv * a.dt;
         for (int i = 1; i <= n; i++) {
             double al = 0.0;
@@ -226,7 +269,7 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
     constexpr int NP = NB * NB;
     if (k >= c.nk) return;
     const size_t nkp = c.nkp;
-    const double *tab = a.tab;
+    const double *tab = stage_row(a);
     const double Ep[3] = {tab[0], tab[1], tab[2]}, Eq[3] = {tab[3], tab[4], tab[5]};
     const int nn = c.n_nbr;
     const bool grad = nn > 0 && tab[6 + nn] != 0.0;
@@ -234,14 +277,29 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
     // the end waits for one round trip, not two.
     constexpr int kPreNbr = NB <= 3 ? 12 : 0;  // a multiple of the batch size below
     int kpre[kPreNbr > 0 ? kPreNbr : 1];
+    const GridPos gp = grid_pos(c, k);
     if (grad) {
 #pragma unroll
-        for (int n = 0; n < kPreNbr; n++) kpre[n] = c.nbr[(size_t)min(n, nn - 1) * nkp + k];
+        for (int n = 0; n < kPreNbr; n++) kpre[n] = nbr_of(c, gp, k, min(n, nn - 1));
     }
 
-    double y[NP];
+    double y[NP], pre0[NP], pre1[NP];
 #pragma unroll
     for (int e = 0; e < NP; e++) y[e] = a.Y[e * nkp + k];
+    // the RK stage tails' reads of the accumulator and the step's base state,
+    // requested with the stage input (up to 3 bands; at 4 the registers they
+    // take cost more occupancy than the early request gains)
+    constexpr bool kPreRK = NB <= 3;
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+        pre0[e] = pre1[e] = 0.0;
+        if constexpr (kPreRK && (MODE == RK2 || MODE == RK3)) {
+            pre0[e] = ldnt(a.ACC + e * nkp + k);
+            pre1[e] = ldnt(a.P0 + e * nkp + k);
+        } else if constexpr (kPreRK && MODE == RK4S) {
+            pre0[e] = ldnt(a.ACC + e * nkp + k);
+        }
+    }
     cd P[NB][NB], M[NB][NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) {
@@ -257,7 +315,66 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
     // X_W[i][j] = -sum_mn (X_mn)_{ji} trig(k), minus Screen_const * (P - P0)_{ji}
     // when n_cond > 1e-20, real diagonal, plus the Hartree shift.
     cd Hren[NB][NB];
-    if (c.coul) {
+    if (c.coul && c.coul_diag) {
+        // Diagonal basis (RungeKutta.h:97-124): X_d from the sums over PD, minus
+        // Screen_const[0][0] (PD - P_Bloch_0) on the diagonal (P_Bloch_0: the
+        // filled bands) and conj(Screen_const[0][1]) PD_ji off it; back to the
+        // orbital basis as U^+ X_d U (InitParameters.h:340-370).
+        cd Xd[NB][NB], U[NB][NB];
+        const double *uk = c.U + (size_t)k * NB * NB * 2;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) U[i][j] = mk(uk[(i * NB + j) * 2], uk[(i * NB + j) * 2 + 1]);
+        const cd scr0 = mk(c.screen[0], c.screen[1]), scr1c = mk(c.screen[2], -c.screen[3]);
+        int u = 0;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = i; j < NB; j++, u++) {
+                cd xk = mk(c.Xw[(size_t)(2 * u) * nkp + k], c.Xw[(size_t)(2 * u + 1) * nkp + k]);
+                if (i == j) {
+                    xk = csub(xk, cscale(scr0, c.PD[i * nkp + k] - (i < c.ncc ? 1.0 : 0.0)));
+                    Xd[i][i] = xk;
+                } else {
+                    const int p = pair_index(i, j, NB);
+                    // PD_ji = conj(PD_ij)
+                    const cd pdji = mk(c.PD[(NB + 2 * p) * nkp + k], -c.PD[(NB + 2 * p + 1) * nkp + k]);
+                    xk = csub(xk, cmul(scr1c, pdji));
+                    Xd[i][j] = xk;
+                    Xd[j][i] = cconj(xk);
+                }
+            }
+        cd T1[NB][NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                cd t = mk(0.0, 0.0);
+#pragma unroll
+                for (int l = 0; l < NB; l++) t = cadd(t, cmul(Xd[i][l], U[l][j]));
+                T1[i][j] = t;
+            }
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = i; j < NB; j++) {
+                cd xw = mk(0.0, 0.0);
+#pragma unroll
+                for (int l = 0; l < NB; l++) xw = cadd(xw, cmul(cconj(U[l][i]), T1[l][j]));
+                cd h;
+                if (i == j) {
+                    h = mk(c.H[i * nkp + k] + xw.x + c.EH[i], 0.0);
+                    c.Hr[i * nkp + k] = h.x;
+                } else {
+                    const int p = pair_index(i, j, NB);
+                    h = mk(c.H[(NB + 2 * p) * nkp + k] + xw.x, c.H[(NB + 2 * p + 1) * nkp + k] + xw.y);
+                    c.Hr[(NB + 2 * p) * nkp + k] = h.x;
+                    c.Hr[(NB + 2 * p + 1) * nkp + k] = h.y;
+                }
+                Hren[i][j] = h;
+            }
+    } else if (c.coul) {
         int u = 0;
 #pragma unroll
         for (int i = 0; i < NB; i++)
@@ -424,14 +541,24 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
         for (int n0 = kPreNbr; n0 < nn; n0 += GB) {
             int kk[GB];
 #pragma unroll
-            for (int j = 0; j < GB; j++) kk[j] = c.nbr[(size_t)min(n0 + j, nn - 1) * nkp + k];
+            for (int j = 0; j < GB; j++) kk[j] = nbr_of(c, gp, k, min(n0 + j, nn - 1));
             batch(n0, kk);
         }
 #pragma unroll
         for (int e = 0; e < NP; e++) pv[e] += g[e];
     }
 #pragma unroll
-    for (int e = 0; e < NP; e++) stage_update<MODE>(a, e * nkp + k, (size_t)NP * nkp, pv[e], y[e]);
+    for (int e = 0; e < NP; e++) {
+        const size_t o = e * nkp + k;
+        if constexpr (kPreRK && (MODE == RK2 || MODE == RK3)) {
+            stnt(a.ACC + o, pre0[e] + pv[e] * a.ca);
+            stnt(a.OUT + o, pre1[e] + pv[e] * a.cb);
+        } else if constexpr (kPreRK && MODE == RK4S) {
+            stnt(a.OUT + o, pre0[e] + pv[e] * a.ca);
+        } else {
+            stage_update<MODE>(a, o, (size_t)NP * nkp, pv[e], y[e]);
+        }
+    }
 }
 
 template <int NB, int MODE, bool DISS>
@@ -440,6 +567,52 @@ __global__ __launch_bounds__(256) void k_stage_small(ConstArgs c, StageArgs a) {
 }
 
 // ================================================================ Coulomb mean field
+// P in the eigenbasis for the Coulomb_diag_basis form (get_P_in_dia_vect,
+// InitParameters.h:264-335): PD = conj(U) P U^T per k in the transposed storage,
+// diagonal real, packed like the state (thread per k, nb <= 4).
+template <int NB>
+__global__ __launch_bounds__(256) void k_coul_pdiag(const double *__restrict__ Y, const double *__restrict__ Uall, int nk, int nkp,
+                                                    double *__restrict__ PD) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    cd P[NB][NB], U[NB][NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        P[i][i] = mk(Y[(size_t)i * nkp + k], 0.0);
+#pragma unroll
+        for (int j = i + 1; j < NB; j++) {
+            const int p = pair_index(i, j, NB);
+            P[i][j] = mk(Y[(size_t)(NB + 2 * p) * nkp + k], Y[(size_t)(NB + 2 * p + 1) * nkp + k]);
+            P[j][i] = cconj(P[i][j]);
+        }
+    }
+    const double *uk = Uall + (size_t)k * NB * NB * 2;
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) U[i][j] = mk(uk[(i * NB + j) * 2], uk[(i * NB + j) * 2 + 1]);
+#pragma unroll
+    for (int a = 0; a < NB; a++)
+#pragma unroll
+        for (int b = a; b < NB; b++) {
+            cd acc = mk(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+                cd t = mk(0.0, 0.0);
+#pragma unroll
+                for (int j = 0; j < NB; j++) t = cadd(t, cmul(P[i][j], U[b][j]));
+                acc = cadd(acc, cmul(cconj(U[a][i]), t));
+            }
+            if (a == b) {
+                PD[(size_t)a * nkp + k] = acc.x;
+            } else {
+                const int p = pair_index(a, b, NB);
+                PD[(size_t)(NB + 2 * p) * nkp + k] = acc.x;
+                PD[(size_t)(NB + 2 * p + 1) * nkp + k] = acc.y;
+            }
+        }
+}
+
 // Separable partial 2D Fourier sums of the packed state over the grid
 // k = (i1, i2) (Coulomb.h:273-350):
 //   S_y[i1][n][e] = sum_i2 y_n(i2) P_e(i1, i2)                     (y = cos, sin)
@@ -937,10 +1110,17 @@ __host__ __device__ constexpr int wave_lds_doubles(int diss) {
 // the group; smem: the group's LDS; store: false for a group past the end of the
 // grid, which computes its clamped k again but writes nothing (some stage modes
 // update their accumulator in place).
-template <int NBT, int MODE, typename R>
+// S: storage type of the RK stage inputs/outputs (YA, YB) and of the multistep
+// histories: double, or float in the mixed form with FP32 stage storage, where
+// the state at t_n (P0), the RK accumulator and every update stay FP64.
+template <int NBT, int MODE, typename R, typename S = double>
 __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &a, const int k, const int lane, double *smem,
                                            const bool store) {
     using CT = typename CplxOf<R>::T;
+    // stage input in S for the RK stages 2-4 (they read YA / YB); the FP64 state otherwise
+    constexpr bool kInS = std::is_same<S, float>::value && (MODE == RK2 || MODE == RK3 || MODE == RK4S);
+    using YT = typename std::conditional<kInS, S, double>::type;
+    const YT *const Yin = reinterpret_cast<const YT *>(a.Y);
     constexpr int TPK = WaveGeom<NBT>::TPK, RB = WaveGeom<NBT>::RB, PER = WaveGeom<NBT>::PER;
     constexpr int LD = WaveGeom<NBT>::LD, MAT = NBT * LD;
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
@@ -949,7 +1129,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     const R *Hop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.H : (const void *)c.Hf);
     const R *Dop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.D : (const void *)c.Df);
     const R *Uop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.U : (const void *)c.Uf);
-    const double *tab = a.tab;
+    const double *tab = stage_row(a);
     const R Ep[3] = {(R)tab[0], (R)tab[1], (R)tab[2]}, Eq[3] = {(R)tab[3], (R)tab[4], (R)tab[5]};
     const size_t base = (size_t)k * NP;
     const int nn = c.n_nbr;
@@ -970,7 +1150,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         ej[m] = (code >> 8) & 255;
         ec[m] = code >> 16;
         g[m] = pre0[m] = pre1[m] = 0.0;
-        y[m] = a.Y[base + e];
+        y[m] = (double)Yin[base + e];
         hv[m] = ldnt(Hop + base + e);
 #pragma unroll
         for (int x = 0; x < 3; x++) dv[m][x] = (a.dmask >> x & 1) ? ldnt(Dop + (size_t)x * c.nk * NP + base + e) : (R)0;
@@ -986,15 +1166,16 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     // flight together, and the sum keeps the stencil order.
     if (grad) {
         constexpr int GB = WaveGeom<NBT>::GATHER;
+        const GridPos gp = grid_pos(c, k);
         for (int n0 = 0; n0 < nn; n0 += GB) {
             size_t kb[GB];
 #pragma unroll
-            for (int j = 0; j < GB; j++) kb[j] = (size_t)c.nbr[(size_t)min(n0 + j, nn - 1) * c.nkp + k] * NP;
+            for (int j = 0; j < GB; j++) kb[j] = (size_t)nbr_of(c, gp, k, min(n0 + j, nn - 1)) * NP;
             double v[GB][PER];
 #pragma unroll
             for (int j = 0; j < GB; j++)
 #pragma unroll
-                for (int m = 0; m < PER; m++) v[j][m] = a.Y[kb[j] + eo[m]];
+                for (int m = 0; m < PER; m++) v[j][m] = (double)Yin[kb[j] + eo[m]];
 #pragma unroll
             for (int j = 0; j < GB; j++)
                 if (n0 + j < nn) {
@@ -1141,31 +1322,32 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     for (int m = 0; m < PER; m++) {
         const double kv = pv[m] + g[m];
         const size_t o = base + eo[m];
+        S *const outs = reinterpret_cast<S *>(a.OUT);  // RK stages 1-3 write the next stage input
         if constexpr (MODE == RK1) {
             stnt(a.ACC + o, y[m] + kv * a.ca);
-            stnt(a.OUT + o, y[m] + kv * a.cb);
+            stnt(outs + o, (S)(y[m] + kv * a.cb));
         } else if constexpr (MODE == RK2 || MODE == RK3) {
             stnt(a.ACC + o, pre0[m] + kv * a.ca);
-            stnt(a.OUT + o, pre1[m] + kv * a.cb);
+            stnt(outs + o, (S)(pre1[m] + kv * a.cb));
         } else if constexpr (MODE == RK4S) {
             stnt(a.OUT + o, pre0[m] + kv * a.ca);
         } else {
-            stage_update<MODE>(a, o, (size_t)c.nk * NP, kv, y[m]);
+            stage_update<MODE, S>(a, o, (size_t)c.nk * NP, kv, y[m]);
         }
     }
 }
 
-template <int NBT, int MODE, typename R>
+template <int NBT, int MODE, typename R, typename S = double>
 __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_stage_wave(ConstArgs c, StageArgs a) {
     extern __shared__ double smem_d[];
     constexpr int TPK = WaveGeom<NBT>::TPK;
     if constexpr (WaveGeom<NBT>::KPB == 1) {
         // xcd_block and strip_k: bijections of [0, nk)
-        stage_wave<NBT, MODE, R>(c, a, strip_k(xcd_block(), c.nk, c.sw, c.scol), threadIdx.x, smem_d, true);
+        stage_wave<NBT, MODE, R, S>(c, a, strip_k(xcd_block(), c.nk, c.sw, c.scol), threadIdx.x, smem_d, true);
     } else {
         const int sub = threadIdx.x / TPK;
         const int kk = xcd_block() * WaveGeom<NBT>::KPB + sub;
-        stage_wave<NBT, MODE, R>(c, a, strip_k(min(kk, c.nk - 1), c.nk, c.sw, c.scol), threadIdx.x % TPK,
+        stage_wave<NBT, MODE, R, S>(c, a, strip_k(min(kk, c.nk - 1), c.nk, c.sw, c.scol), threadIdx.x % TPK,
                                  smem_d + sub * wave_lds_doubles<NBT, R>(c.diss), kk < c.nk);
     }
 }
@@ -1451,6 +1633,16 @@ __global__ void k_build_q(const double *Hp, const double *Dp, int nk, int nb, si
     Q[2 * id + 1] = acc.y;
 }
 
+// ================================================================ captured steps
+// The step counter of the field-table path: set by the host's table upload (in
+// stream order) and advanced by one at the end of every captured step.
+__global__ void k_counter_set(int *ctr, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ctr = v;
+}
+__global__ void k_counter_step(int *ctr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ctr += 1;
+}
+
 // ================================================================ adaptive dt
 // Diagonal of the state before a step (P_W_prev, RungeKutta.cpp:1-9,139-147 and
 // Taylor_DE_solver_deriv_Polynom.cpp:75-84).  Element e of k sits at k*sk + e*se.
@@ -1685,6 +1877,89 @@ __global__ __launch_bounds__(256) void k_wannier_mfma32(const cd *__restrict__ P
         }
 }
 
+// The contraction as BF16x3 split products on the bf16 matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation): every FP32 operand x is split
+// into hi = bf16(x) and lo = bf16(x - hi), and each real product a b is taken as
+// a_hi b_hi + a_hi b_lo + a_lo b_hi (the lo lo term, ~2^-16 |a b|, dropped).
+// Phases and H(R), r(R) are formed in FP64 and rounded once to FP32.  Tiling as
+// k_wannier_mfma32 with R consumed 32 at a time (one K slice of the
+// instruction); lane l supplies A[row l%16][8 (l/16) .. +7] and B[same k][col l%16]
+// and holds D[4 (l/16) + i][l%16].
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+constexpr int WBR = 32;
+
+__device__ __forceinline__ void bf16_split(float x, __bf16 &hi, __bf16 &lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+__global__ __launch_bounds__(256) void k_wannier_mfma_bf16x3(const cd *__restrict__ PT, const cd *__restrict__ Val, int N0, int N1,
+                                                             int N2, int nR, int nm, int nk, int kbase, cd *__restrict__ Out) {
+    // [hi, lo] planes of the real and imaginary parts, K (R) innermost with one pad
+    __shared__ __bf16 Ph[2][2][WMK][WBR + 8];
+    __shared__ __bf16 Vs[2][2][WMN][WBR + 8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k0 = blockIdx.y * WMK, m0 = blockIdx.x * WMN;
+    v4f are[2], aim[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) are[t] = aim[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int r0 = 0; r0 < nR; r0 += WBR) {
+        for (int t = threadIdx.x; t < WMK * WBR; t += 256) {
+            const int kk = t / WBR, rr = t % WBR, k = k0 + kk, r = r0 + rr;
+            const cd ph = (k < nk && r < nR) ? wannier_phase(PT, nR, kbase + k, r, N0, N1, N2) : mk(0.0, 0.0);
+            bf16_split((float)ph.x, Ph[0][0][kk][rr], Ph[0][1][kk][rr]);
+            bf16_split((float)ph.y, Ph[1][0][kk][rr], Ph[1][1][kk][rr]);
+        }
+        for (int t = threadIdx.x; t < WBR * WMN; t += 256) {
+            const int rr = t / WMN, mm = t % WMN, r = r0 + rr, m = m0 + mm;
+            const cd v = (r < nR && m < nm) ? Val[(size_t)r * nm + m] : mk(0.0, 0.0);
+            bf16_split((float)v.x, Vs[0][0][mm][rr], Vs[0][1][mm][rr]);
+            bf16_split((float)v.y, Vs[1][0][mm][rr], Vs[1][1][mm][rr]);
+        }
+        __syncthreads();
+        const int ka = 8 * (lane >> 4);
+        v8bf a[2][2], an[2];  // [re/im][hi/lo]; an: -Im (hi, lo)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[c][h][q] = Ph[c][h][wave * 16 + (lane & 15)][ka + q];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) an[h][q] = (__bf16)(-(float)a[1][h][q]);
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            v8bf b[2][2];
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int q = 0; q < 8; q++) b[c][h][q] = Vs[c][h][t * 16 + (lane & 15)][ka + q];
+            // Re += PhR VR - PhI VI, Im += PhR VI + PhI VR, each as hi hi + hi lo + lo hi
+            auto prod3 = [&](const v8bf &xh, const v8bf &xl, const v8bf &yh, const v8bf &yl, v4f acc) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, yh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, yl, acc, 0, 0, 0);
+                return __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, yh, acc, 0, 0, 0);
+            };
+            are[t] = prod3(a[0][0], a[0][1], b[0][0], b[0][1], are[t]);
+            are[t] = prod3(an[0], an[1], b[1][0], b[1][1], are[t]);
+            aim[t] = prod3(a[0][0], a[0][1], b[1][0], b[1][1], aim[t]);
+            aim[t] = prod3(a[1][0], a[1][1], b[0][0], b[0][1], aim[t]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = k0 + wave * 16 + 4 * (lane >> 4) + i, m = m0 + t * 16 + (lane & 15);
+            if (k < nk && m < nm) Out[(size_t)k * nm + m] = mk((double)are[t][i], (double)aim[t][i]);
+        }
+}
+
 // Contraction output [k][c][i][j] (c = 0 H, 1..3 r) to the caller's layouts,
 // one thread per (k, i, j): H with zero diagonal imaginary parts
 // (Wannier.h:489-492) and D = (r + r^T)/2 with zero diagonal imaginary parts
@@ -1870,6 +2145,8 @@ struct edus_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
     int sw = 0, scol = 0;  // strip order of the wave kernels (strip_k)
+    int gn[3] = {0, 0, 0};  // grid sizes of a translation-invariant stencil (ConstArgs::gn0..2), or 0
+    signed char goff[kMaxNbr][3] = {};
     int precision = EDUS_PRECISION_FP64;
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
     int *eidx = nullptr;  // packed element -> (i, j, comp) of the wave layout
@@ -1893,7 +2170,8 @@ struct edus_ctx {
     int last_launches = 0;
     std::vector<void *> allocs;
     // Coulomb mean field
-    int coul = 0, N1 = 0, N2 = 0, ncut = 0, nu = 0;
+    int coul = 0, coul_diag = 0, N1 = 0, N2 = 0, ncut = 0, nu = 0;
+    double *cPD = nullptr;  // diagonal basis: packed state in the eigenbasis
     double n_cond = 0.0;
     double *cxm = nullptr, *sxm = nullptr, *cyn = nullptr, *syn = nullptr, *cS = nullptr, *cF = nullptr;
     double *cX = nullptr, *cX0 = nullptr, *cA = nullptr, *cB = nullptr, *cC = nullptr, *cD = nullptr, *cvh = nullptr;
@@ -1903,6 +2181,14 @@ struct edus_ctx {
     double *snap = nullptr;  // [nk] complex, k-resolved snapshot element
     // adaptive dt: diagonal before the step, block maxima
     double *DG = nullptr, *dpart = nullptr;
+    // device-resident field table of a run and the captured RK4 step reading it
+    double *gtab = nullptr;
+    size_t gtab_cap = 0;
+    int *gctr = nullptr;          // rows consumed, in steps (advanced by the captured step)
+    long long gtab_steps = 0, gpos = 0;
+    int gdmask = 7;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -2013,6 +2299,52 @@ int launch_coulomb(edus_ctx *c, const double *Y) {
     return 0;
 }
 
+// The mean-field sums of the stage input Y: over Y itself (Wannier basis) or over
+// Y in the eigenbasis (Coulomb_diag_basis, RungeKutta.h:41-55).
+int launch_coulomb_of(edus_ctx *c, const double *Y) {
+    if (!c->coul_diag) return launch_coulomb(c, Y);
+    const dim3 grid((c->nk + 255) / 256), block(256);
+    switch (c->nb) {
+        case 1: hipLaunchKernelGGL(k_coul_pdiag<1>, grid, block, 0, c->stream, Y, (const double *)c->U, c->nk, c->nkp, c->cPD); break;
+        case 2: hipLaunchKernelGGL(k_coul_pdiag<2>, grid, block, 0, c->stream, Y, (const double *)c->U, c->nk, c->nkp, c->cPD); break;
+        case 3: hipLaunchKernelGGL(k_coul_pdiag<3>, grid, block, 0, c->stream, Y, (const double *)c->U, c->nk, c->nkp, c->cPD); break;
+        default: hipLaunchKernelGGL(k_coul_pdiag<4>, grid, block, 0, c->stream, Y, (const double *)c->U, c->nk, c->nkp, c->cPD); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return launch_coulomb(c, c->cPD);
+}
+
+// Whether the neighbour table is one integer offset per neighbour on the grid
+// nk_dims, the same at every k (k + b wrapped per axis): then the kernels form
+// the neighbours from the offsets (ConstArgs::goff) and skip the table.  Explicit
+// k lists and tables of any other form keep the table.
+void grid_stencil(edus_ctx *c, const edus_problem *p) {
+    const int N[3] = {p->nk_dims[0], p->nk_dims[1], p->nk_dims[2]};
+    const int nn = p->n_nbr;
+    if (nn < 1 || nn > kMaxNbr || N[0] < 1 || N[1] < 1 || N[2] < 1 || (long long)N[0] * N[1] * N[2] != p->nk) return;
+    signed char off[kMaxNbr][3] = {};
+    for (int n = 0; n < nn; n++) {  // the offsets at k = 0
+        const int j = p->nbr[n];
+        const int jj[3] = {j / (N[1] * N[2]), (j / N[2]) % N[1], j % N[2]};
+        for (int x = 0; x < 3; x++) {
+            const int o = jj[x] > N[x] / 2 ? jj[x] - N[x] : jj[x];
+            if (o < -127 || o > 127) return;
+            off[n][x] = (signed char)o;
+        }
+    }
+    auto wrap = [](int v, int n) { v %= n; return v < 0 ? v + n : v; };
+    for (int k = 0; k < p->nk; k++) {
+        const int ii[3] = {k / (N[1] * N[2]), (k / N[2]) % N[1], k % N[2]};
+        for (int n = 0; n < nn; n++) {
+            const int want = (wrap(ii[0] + off[n][0], N[0]) * N[1] + wrap(ii[1] + off[n][1], N[1])) * N[2] +
+                             wrap(ii[2] + off[n][2], N[2]);
+            if (p->nbr[(size_t)k * nn + n] != want) return;
+        }
+    }
+    for (int x = 0; x < 3; x++) c->gn[x] = N[x];
+    std::memcpy(c->goff, off, sizeof off);
+}
+
 ConstArgs const_args(const edus_ctx *c) {
     ConstArgs ca;
     std::memset(&ca, 0, sizeof ca);
@@ -2023,6 +2355,10 @@ ConstArgs const_args(const edus_ctx *c) {
     ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
     ca.cxm = c->cxm; ca.sxm = c->sxm; ca.Tt = c->cT; ca.PW0 = c->cPW0; ca.EH = c->cEH; ca.screen = c->cscreen; ca.Hr = c->cHr;
     ca.Xw = c->cXw;
+    ca.coul_diag = c->coul_diag;
+    ca.PD = c->cPD;
+    ca.gn0 = c->gn[0]; ca.gn1 = c->gn[1]; ca.gn2 = c->gn[2];
+    std::memcpy(ca.goff, c->goff, sizeof ca.goff);
     return ca;
 }
 
@@ -2031,7 +2367,7 @@ int launch_stage(edus_ctx *c, StageArgs a) {
     a.dmask = c->dmask;
     const ConstArgs ca = const_args(c);
     if (c->coul) {
-        int rc = launch_coulomb(c, a.Y);
+        int rc = launch_coulomb_of(c, a.Y);
         if (rc) return rc;
     }
     if (c->soa) {
@@ -2057,7 +2393,9 @@ int launch_stage(edus_ctx *c, StageArgs a) {
             const dim3 grid((c->nk + KPB - 1) / KPB), block(WaveGeom<NBT>::TPK * KPB);
             const size_t lds_f = (size_t)KPB * wave_lds_doubles<NBT, float>(c->diss) * 8;
             const size_t lds_d = (size_t)KPB * wave_lds_doubles<NBT, double>(c->diss) * 8;
-            if (c->precision == EDUS_PRECISION_MIXED)
+            if (c->precision == EDUS_PRECISION_MIXED32)
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float, float>), grid, block, lds_f, c->stream, ca, a);
+            else if (c->precision == EDUS_PRECISION_MIXED)
                 hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float>), grid, block, lds_f, c->stream, ca, a);
             else
                 hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double>), grid, block, lds_d, c->stream, ca, a);
@@ -2193,9 +2531,9 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
     c->stepper = p->stepper; c->order = (p->stepper == EDUS_STEPPER_RK4) ? 4 : p->order;
     c->soa = nb <= 4;
     c->precision = p->precision;
-    if (c->precision != EDUS_PRECISION_FP64 && c->precision != EDUS_PRECISION_MIXED)
-        return fail(set_err(EDUS_ERR_INVALID, "precision must be EDUS_PRECISION_FP64 or EDUS_PRECISION_MIXED"));
-    if (c->precision == EDUS_PRECISION_MIXED && c->soa)
+    if (c->precision != EDUS_PRECISION_FP64 && c->precision != EDUS_PRECISION_MIXED && c->precision != EDUS_PRECISION_MIXED32)
+        return fail(set_err(EDUS_ERR_INVALID, "precision must be EDUS_PRECISION_FP64, _MIXED or _MIXED32"));
+    if (c->precision != EDUS_PRECISION_FP64 && c->soa)
         return fail(set_err(EDUS_ERR_UNSUPPORTED, "mixed precision is implemented for nb >= 5"));
     c->nkp = c->soa ? ((p->nk + 255) / 256) * 256 : p->nk;
     // Wave kernels: walk the grid in strips (strip_k) when a strip's two previous
@@ -2299,7 +2637,7 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
             hipMemcpyAsync(c->T, p->T, (size_t)nb * nb * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "upload U, T"));
     }
-    if (c->precision == EDUS_PRECISION_MIXED) {
+    if (c->precision != EDUS_PRECISION_FP64) {
         const size_t nh = se, nd = 3 * se, nu = (size_t)p->nk * nb * nb * 2;
         if ((rc = dalloc(c, &c->Hf, nh)) || (rc = dalloc(c, &c->Df, nd)) || (rc = dalloc(c, &c->Uf, nu))) return fail(rc);
         const double *src[3] = {c->H, c->D, c->U};
@@ -2325,6 +2663,7 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
         if ((rc = dalloc(c, &c->nbr, nbr.size()))) return fail(rc);
         if (hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "upload nbr"));
+        grid_stencil(c, p);
     }
     c->obs_blocks = std::max(1, std::min(kObsMaxBlocks, p->nk));
     if ((rc = dalloc(c, &c->part, (size_t)c->obs_blocks * (nb + 17))) || (rc = dalloc(c, &c->obs, nb + 17))) return fail(rc);
@@ -2392,8 +2731,20 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
         if (hipMemcpyAsync(c->cPW0, c->P0, se * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
             hipMemcpyAsync(c->cHr, c->H, se * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "copy P0/H"));
-        // equilibrium: X0 and the Hartree shift of P0 (main_MPI.cpp:276-321)
-        if ((rc = launch_coulomb(c, c->P0))) return fail(rc);
+        // equilibrium: X0 and the Hartree shift of P0 (main_MPI.cpp:276-321); in the
+        // diagonal basis of the filled bands, occupation 1 (main_MPI.cpp:291-303)
+        if (q->diag_basis) {
+            c->coul_diag = 1;
+            if ((rc = dalloc(c, &c->cPD, se))) return fail(rc);
+            std::vector<double> peq(se, 0.0);
+            for (int i = 0; i < c->ncc; i++)
+                for (int k = 0; k < p->nk; k++) peq[(size_t)i * c->nkp + k] = 1.0;
+            if ((rc = upload(c, c->cPD, peq))) return fail(rc);
+            if ((rc = launch_coulomb(c, c->cPD))) return fail(rc);
+            if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_err(EDUS_ERR_HIP, "diagonal-basis equilibrium"));
+        } else if ((rc = launch_coulomb(c, c->P0))) {
+            return fail(rc);
+        }
         if (hipMemcpyAsync(c->cX0, c->cX, (size_t)4 * nmm * nb2 * 2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
             hipMemcpyAsync(c->cEH0, c->cEH, nb * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "copy equilibrium"));
@@ -2409,6 +2760,9 @@ int edus_destroy(edus_ctx *c) {
     if (!c) return 0;
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->gtab) (void)hipFree(c->gtab);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->tab) (void)hipFree(c->tab);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -2478,6 +2832,93 @@ int edus_step_rk4(edus_ctx *c, int32_t nsteps, const double *fields, const uint8
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; s++)
         if ((rc = rk4_step(c, c->tab + (size_t)s * 4 * rs, rs, c->dt))) return rc;
+    c->nstep += nsteps;
+    return finish_step(c, 4 * nsteps);
+}
+
+static void drop_graph(edus_ctx *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+int edus_set_field_table(edus_ctx *c, int32_t nsteps, const double *fields, const uint8_t *grad_on) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->stepper != EDUS_STEPPER_RK4) return set_err(EDUS_ERR_UNSUPPORTED, "the field-table path runs RK4 contexts");
+    if (nsteps < 1 || !fields || !grad_on) return set_err(EDUS_ERR_INVALID, "bad field table");
+    std::vector<double> rows;
+    rk4_rows(c, nsteps, fields, grad_on, rows);
+    const size_t rs = 7 + (size_t)c->n_nbr;
+    int mask = 0;
+    for (size_t r = 0; r + rs <= rows.size(); r += rs)
+        for (int x = 0; x < 3; x++)
+            if (rows[r + x] != 0.0 || rows[r + 3 + x] != 0.0) mask |= 1 << x;
+    if (rows.size() > c->gtab_cap || mask != c->gdmask) drop_graph(c);  // the graph holds the table address and the mask
+    if (rows.size() > c->gtab_cap) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->gtab) HIP_TRY(hipFree(c->gtab));
+        c->gtab = nullptr;
+        c->gtab_cap = 0;
+        HIP_TRY(hipMalloc(&c->gtab, rows.size() * sizeof(double)));
+        c->gtab_cap = rows.size();
+    }
+    if (!c->gctr) {
+        int *q = nullptr;
+        HIP_TRY(hipMalloc(&q, sizeof(int)));
+        c->allocs.push_back(q);
+        c->gctr = q;
+    }
+    c->gdmask = mask;
+    HIP_TRY(hipMemcpyAsync(c->gtab, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_counter_set, dim3(1), dim3(1), 0, c->stream, c->gctr, 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));  // the host rows are released on return
+    c->gtab_steps = nsteps;
+    c->gpos = 0;
+    return 0;
+}
+
+// One RK4 step whose stage rows come from the field table at the device step
+// counter, captured once into a graph (four stage launches and the counter
+// advance; Coulomb passes included) and replayed for every step.
+static int build_step_graph(edus_ctx *c) {
+    const int rs = 7 + c->n_nbr;
+    const int saved = c->dmask;
+    c->dmask = c->gdmask;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int st = 0; st < 4 && rc == 0; st++) {
+        StageArgs a = rk4_stage_args(c, st, c->gtab + (size_t)st * rs, c->dt);
+        a.ctr = c->gctr;
+        a.ctr_stride = 4LL * rs;
+        rc = st == 0 ? launch_stage<RK1>(c, a) : st == 1 ? launch_stage<RK2>(c, a) : st == 2 ? launch_stage<RK3>(c, a) : launch_stage<RK4S>(c, a);
+    }
+    if (rc == 0) hipLaunchKernelGGL(k_counter_step, dim3(1), dim3(1), 0, c->stream, c->gctr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->dmask = saved;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return set_err(EDUS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    c->graph = g;
+    HIP_TRY(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+int edus_step_rk4_graph(edus_ctx *c, int32_t nsteps) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->stepper != EDUS_STEPPER_RK4) return set_err(EDUS_ERR_UNSUPPORTED, "the field-table path runs RK4 contexts");
+    if (!c->gtab) return set_err(EDUS_ERR_INVALID, "no field table: call edus_set_field_table first");
+    if (nsteps < 0 || c->gpos + nsteps > c->gtab_steps) return set_err(EDUS_ERR_INVALID, "steps beyond the end of the field table");
+    if (!c->gexec && (rc = build_step_graph(c))) return rc;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < nsteps; s++) HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+    c->gpos += nsteps;
     c->nstep += nsteps;
     return finish_step(c, 4 * nsteps);
 }
@@ -2824,6 +3265,8 @@ int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch
         if (!problems[i]) return set_err(EDUS_ERR_INVALID, "null problem in batch");
         if (problems[i]->stepper != EDUS_STEPPER_RK4) return set_err(EDUS_ERR_UNSUPPORTED, "batches run RK4 problems");
         if (problems[i]->coulomb) return set_err(EDUS_ERR_UNSUPPORTED, "batches do not carry the Coulomb term");
+        if (problems[i]->precision == EDUS_PRECISION_MIXED32)
+            return set_err(EDUS_ERR_UNSUPPORTED, "batches run FP64 or EDUS_PRECISION_MIXED members");
     }
     auto *b = new edus_batch();
     auto fail = [&](int rc) { edus_batch_destroy(b); return rc; };
@@ -3241,8 +3684,9 @@ static void wannier_twiddles(const int32_t nk3[3], std::vector<double> &tw, int 
     }
 }
 
+// form: 0 FP64 matrix cores, 1 FP32, 2 BF16x3 split products.
 static int wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR, double *H_out,
-                     double *D_out, bool fp32) {
+                     double *D_out, int form) {
     if (!nk3 || !R || !HR || !rR || !H_out || nR < 1 || nw < 1) return set_err(EDUS_ERR_INVALID, "bad Wannier arguments");
     for (int l = 0; l < 3; l++)
         if (nk3[l] < 1) return set_err(EDUS_ERR_INVALID, "bad grid");
@@ -3290,7 +3734,10 @@ static int wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_
     for (long long k0 = 0; k0 < nk; k0 += chunk) {
         const int kc = (int)std::min<long long>(chunk, nk - k0);
         const dim3 grid((nm + WMN - 1) / WMN, (unsigned)((kc + WMK - 1) / WMK));
-        if (fp32)
+        if (form == 2)
+            hipLaunchKernelGGL(k_wannier_mfma_bf16x3, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR,
+                               nm, kc, (int)k0, (cd *)dOut);
+        else if (form == 1)
             hipLaunchKernelGGL(k_wannier_mfma32, grid, dim3(256), 0, 0, (const cd *)dPT, (const cd *)dVal, nk3[0], nk3[1], nk3[2], nR, nm,
                                kc, (int)k0, (cd *)dOut);
         else
@@ -3313,12 +3760,17 @@ static int wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_
 
 int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR, double *H_out,
                     double *D_out) {
-    return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, false);
+    return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, 0);
 }
 
 int edus_wannier_hk_fp32(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR,
                          double *H_out, double *D_out) {
-    return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, true);
+    return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, 1);
+}
+
+int edus_wannier_hk_bf16x3(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR,
+                           double *H_out, double *D_out) {
+    return wannier_hk(nk3, nR, nw, R, HR, rR, H_out, D_out, 2);
 }
 
 // The contraction of edus_wannier_hk with every device buffer carved from the

@@ -41,8 +41,6 @@ void Setup::build(const std::string &input_path, int flags) {
         t_last = t;
     };
     in = parse_input(input_path);
-    if (in.coulomb && in.coulomb_diag_basis)
-        throw Error("Coulomb_diag_basis is not supported by this build (Wannier basis only)");
     if (in.iMode != "TB" && in.iMode != "W" && in.iMode != "CY") throw Error("iMode = " + in.iMode + " is incorrect");
     // The crystal interface replaces the lattice with the one of its k-mesh file
     // (input.cpp:57, Crystal_interface.h:58-96).

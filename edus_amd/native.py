@@ -14,7 +14,7 @@ import numpy as np
 LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 
 STEPPER_RK4, STEPPER_AB, STEPPER_TAYLOR = 0, 1, 2
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED32 = 0, 1, 2
 FIELDS_PER_EVAL = 6
 
 _dp = C.POINTER(C.c_double)
@@ -112,6 +112,8 @@ hip.edus_batch_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
 hip.edus_batch_member.argtypes = [C.c_void_p, C.c_int32]
 hip.edus_batch_member.restype = C.c_void_p
 hip.edus_stream_probe.argtypes = [C.c_int64, C.c_int32, _dp, _dp, _dp]
+hip.edus_set_field_table.argtypes = [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_uint8)]
+hip.edus_step_rk4_graph.argtypes = [C.c_void_p, C.c_int32]
 
 # ---- host
 host.edus_setup_from_input.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]
@@ -343,6 +345,17 @@ class Engine:
         fn = {STEPPER_RK4: hip.edus_step_rk4, STEPPER_AB: hip.edus_step_ab, STEPPER_TAYLOR: hip.edus_step_taylor}[self.stepper]
         _check_hip(fn(self._h, n, _dptr(fields), grad_on.ctypes.data_as(C.POINTER(C.c_uint8))), "edus_step")
 
+    def set_field_table(self, fields, grad_on):
+        """Uploads the RK4 field schedule of a run (edus_set_field_table)."""
+        fields = np.ascontiguousarray(fields, dtype=np.float64)
+        grad_on = np.ascontiguousarray(grad_on, dtype=np.uint8)
+        _check_hip(hip.edus_set_field_table(self._h, len(grad_on), _dptr(fields), grad_on.ctypes.data_as(C.POINTER(C.c_uint8))),
+                   "edus_set_field_table")
+
+    def step_graph(self, nsteps):
+        """nsteps RK4 steps from the field table through the captured step graph."""
+        _check_hip(hip.edus_step_rk4_graph(self._h, int(nsteps)), "edus_step_rk4_graph")
+
     def step_dyn(self, dt, fields, grad_on, tau_inv=None, cond=0.0):
         """One adaptive-dt step; returns the step-size measure max_dP."""
         fields = np.ascontiguousarray(fields, dtype=np.float64)
@@ -391,9 +404,10 @@ class Engine:
         return ms.value, n.value
 
 
-def wannier_hk(nk3, R, HR, rR, nw, fp32=False):
+def wannier_hk(nk3, R, HR, rR, nw, fp32=False, bf16x3=False):
     """GPU Wannier contraction: returns H [nk][nw][nw], D [nk][nw][nw][3] (complex).
-    fp32: the FP32 matrix-core form (edus_wannier_hk_fp32)."""
+    fp32: the FP32 matrix-core form (edus_wannier_hk_fp32); bf16x3: the BF16x3
+    split-product form (edus_wannier_hk_bf16x3)."""
     nk3 = np.ascontiguousarray(nk3, dtype=np.int32)
     R = np.ascontiguousarray(R, dtype=np.int32)
     HR = np.ascontiguousarray(HR, dtype=np.complex128)
@@ -401,10 +415,14 @@ def wannier_hk(nk3, R, HR, rR, nw, fp32=False):
     nk = int(np.prod(nk3))
     H = np.zeros((nk, nw, nw), dtype=np.complex128)
     D = np.zeros((nk, nw, nw, 3), dtype=np.complex128)
-    fn = hip.edus_wannier_hk_fp32 if fp32 else hip.edus_wannier_hk
+    if bf16x3:
+        fn = hip.edus_wannier_hk_bf16x3
+        fn.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
+    else:
+        fn = hip.edus_wannier_hk_fp32 if fp32 else hip.edus_wannier_hk
     rc = fn(nk3.ctypes.data_as(_ip), len(R), nw, R.ctypes.data_as(_ip), _dptr(HR.view(np.float64)),
             _dptr(rR.view(np.float64)), _dptr(H.view(np.float64)), _dptr(D.view(np.float64)))
-    _check_hip(rc, "edus_wannier_hk_fp32" if fp32 else "edus_wannier_hk")
+    _check_hip(rc, fn.__name__)
     return H, D
 
 

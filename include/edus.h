@@ -45,18 +45,26 @@ extern "C" {
  * reference counterpart: an option for long runs where FP32 operators suffice. */
 #define EDUS_PRECISION_FP64  0
 #define EDUS_PRECISION_MIXED 1
+/* MIXED32: MIXED plus FP32 storage of what only feeds a right-hand side or an
+ * FP64 sum: the RK4 stage inputs (y_n + c dt k, read by stages 2-4 and their
+ * stencils) and the Adams-Bashforth / Taylor derivative histories.  The state at
+ * t_n, the RK4 accumulator and every update stay FP64 (the master copy).  Band
+ * counts 5..32; not for batches.  No reference counterpart. */
+#define EDUS_PRECISION_MIXED32 2
 
 #define EDUS_MAX_NB          32  /* largest band count with a kernel in this build */
 #define EDUS_MAX_ORDER        8  /* largest Adams-Bashforth / Taylor order */
 
 typedef struct edus_ctx edus_ctx;
 
-/* Hartree-Fock mean field in the Wannier basis as a truncated separable Fourier
- * series of V(k-k') (Coulomb.h:273-541, coefficients as read by
- * Fourier_Coulomb.cpp:505-622).  Sample in the yz plane (grid 1 x n1 x n2). */
+/* Hartree-Fock mean field as a truncated separable Fourier series of V(k-k')
+ * (Coulomb.h:273-541, coefficients as read by Fourier_Coulomb.cpp:505-622), in
+ * the Wannier basis or, with diag_basis, in the eigenbasis.  Sample in the yz
+ * plane (grid 1 x n1 x n2). */
 typedef struct edus_coulomb {
     int32_t ncut;            /* Fourier modes per direction (Coulomb_set.Ncut)       */
-    int32_t reserved;
+    int32_t diag_basis;      /* Coulomb_diag_basis: sums over P in the eigenbasis of U,
+                                equilibrium the filled bands (RungeKutta.h:41-55, 97-124) */
     const double *A, *B, *C, *D;  /* [ncut*ncut][nb*nb] complex                      */
     const double *screen;    /* [nb*nb] complex Screen_const[0][*]                   */
     const double *v_hartree; /* [nb][nb] complex V_Hartree                           */
@@ -116,6 +124,21 @@ int edus_destroy(edus_ctx *ctx);
  * get_derivative_Df (headers/RungeKutta.h:7-297), Runge_Kutta_Ad/Ac
  * (RungeKutta.h:301-355) and Mpi_communication (MPIcommunication.h:1-20). */
 int edus_step_rk4(edus_ctx *ctx, int32_t nsteps, const double *fields, const uint8_t *grad_on);
+
+/* The field schedule of a whole run, uploaded once and kept on the device:
+ * nsteps RK4 steps laid out as for edus_step_rk4 (fields [nsteps][3][6],
+ * grad_on [nsteps]).  Resets the table position to its first step.  Part of the
+ * captured-step path below; replaces the per-stage field evaluation of
+ * RungeKutta.cpp:24-30 feeding get_derivative_Df.  RK4 contexts. */
+int edus_set_field_table(edus_ctx *ctx, int32_t nsteps, const double *fields, const uint8_t *grad_on);
+
+/* Advances `nsteps` RK4 steps reading their fields from the table, from its
+ * current position on: one RK4 step (RungeKutta.cpp:11-137, the four fused
+ * stage launches, Coulomb passes included) is captured once into a HIP graph
+ * that reads its rows through a device step counter, then replayed once per
+ * step.  The results are bit-identical to edus_step_rk4 with the same fields.
+ * Fails with EDUS_ERR_INVALID past the end of the table. */
+int edus_step_rk4_graph(edus_ctx *ctx, int32_t nsteps);
 
 /* Advances `nsteps` Adams-Bashforth steps of the context's order.
  * fields: [nsteps][1][EDUS_FIELDS_PER_EVAL].  Replaces
@@ -250,6 +273,17 @@ int edus_wannier_hk(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t 
  * element carries at most (2 nR + 4) 2^-24 sum_R |X(R)_mn| of error. */
 int edus_wannier_hk_fp32(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R,
                          const double *HR, const double *rR, double *H_out, double *D_out);
+
+/* The same contraction as BF16x3 split products on the bf16 matrix cores: each
+ * FP32-rounded operand x is split into hi = bf16(x) and lo = bf16(x - hi) and every
+ * real product is taken as hi*hi + hi*lo + lo*hi with FP32 accumulation (phases
+ * formed in FP64).  The BF16x3 form of Wannier::energy / dipolex/y/z
+ * (headers/Wannier.h:278-304, 336-471); each element carries at most
+ * sqrt(2) (3 2^-18 + (2 nR + 8) 2^-24) sum_R |X(R)_mn| of error (the split keeps 16
+ * of FP32's 24 mantissa bits).  No reference counterpart as a form: for setups
+ * where that accuracy suffices. */
+int edus_wannier_hk_bf16x3(const int32_t nk3[3], int32_t nR, int32_t nw, const int32_t *R, const double *HR, const double *rR,
+                           double *H_out, double *D_out);
 
 /* Size of the fixed device array behind edus_wannier_hk_noalloc (one copy per
  * device, held from the library's load until the process ends). */

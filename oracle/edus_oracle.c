@@ -14,17 +14,26 @@
  *   orc_shells           CalculateWeigths.h:1-138  b-vectors and residuals printed in the AB5 output.log;
  *                                                  B1 condition sum w b_a b_b = delta_ab
  *   orc_neighbours       CalculateWeigths.h:145-226 closed form k+b on small grids (tests), invariants
- *   orc_tb_tables        TightBinding.h            Test_matrices.txt of the AB5 run; U H U^+ diagonal, U U^+ = 1
+ *   orc_tb_tables        TightBinding.h            Test_matrices.txt of the AB5 run; U H U^+ diagonal, U U^+ = 1;
+ *                                                  every family: closed-form bands (graphene +-b|f| and its
+ *                                                  Dirac point, the bilayer's +-(S +- d)/2, BN +-sqrt(g^2/4 +
+ *                                                  b^2|f|^2) with the gaps at Gamma and K, BN1b, GeS, Vampa2015's
+ *                                                  gap at Gamma), periodicity, GrapheneZurron's cartesian f
  *   orc_wannier_tables   Wannier.h:278-471         brute-force DFT on tiny NR, Hermiticity, periodicity
  *   orc_rhs / orc_run    RungeKutta.h:7-355, RungeKutta.cpp, Linear_Multistep_ODE_solvers.cpp,
  *                        Taylor_DE_solver_deriv_Polynom.cpp, const_dt_evolution.cpp,
  *                        Observables_MPI.h          AB5 J1/J2/Losses/TransientAbs (AB5 + core-hole decay);
- *                                                  RK4 / Taylor / dephasing: free precession, T2 decay,
- *                                                  trace preservation, scalar-ODE order tests
+ *                                                  RK4: free precession, a driven mode converging at fourth
+ *                                                  order (stage field times), the one-step E_non0 lag;
+ *                                                  dephasing: Initialize_T restated independently and the
+ *                                                  exact RK4 amplification of every element at zero field;
+ *                                                  AB / Taylor: polynomial exactness, the scheme as written
  *   Coulomb mean field   Coulomb.h:273-541, RungeKutta.h:29-138
  *                                                  brute-force convolution with the separable kernel of
  *                                                  the A..D coefficients (example files, Ncut 20);
- *                                                  RHS reduces to the bare RHS at equilibrium
+ *                                                  RHS reduces to the bare RHS at equilibrium; diagonal basis
+ *                                                  (RungeKutta.h:41-55, 97-124): the composition of the pinned
+ *                                                  sums with eigenbasis transforms done in numpy
  *   orc_run_dyn          dyn_dt_evolution.cpp      reduces bit for bit to orc_run when the bound is never
  *                                                  reached; its dt sequence replayed step by step in numpy;
  *                                                  Taylor with a dt history = the polynomial identity
@@ -36,10 +45,10 @@
  *                                                  (no reference data ship for the crystal interface)
  *
  * Parity unpinned: the eigenvector phases of Armadillo's eig_sym (Wannier and crystal U,
- * which only matter for dephasing and for off-diagonal snapshot elements), and the
- * reference's own trajectories for RK4, Taylor, Wannier, Coulomb and adaptive dt (its
- * build needs Armadillo/LAPACK, which the image lacks): those are pinned through the
- * invariants above, not against reference output.
+ * which only matter for dephasing and for off-diagonal snapshot elements).  The
+ * reference's own trajectories for RK4, Taylor, Wannier, Coulomb and adaptive dt cannot
+ * be produced here (its build needs Armadillo/LAPACK, which the image lacks): those
+ * paths are pinned through the invariants above, not against reference output.
  */
 #include <complex.h>
 #include <math.h>
@@ -606,6 +615,7 @@ typedef struct orc_coul_params {
     const double *A, *B, *C, *D;   /* [ncut^2][nb^2] complex (Fourier_Coulomb.cpp:505-622) */
     const double *screen;          /* [nb^2] complex Screen_const[0][*] */
     const double *vh;              /* [nb][nb] complex V_Hartree */
+    int diag_basis;                /* Coulomb_diag_basis: the sums run on P in the eigenbasis (RungeKutta.h:41-55, 97-124) */
 } orc_coul_params;
 
 typedef struct orc_coul {
@@ -615,6 +625,7 @@ typedef struct orc_coul {
     double EH[ORC_MAXNB], EH0[ORC_MAXNB];
     cplx *PW0;                          /* P at t = 0 */
     cplx *Hren;                         /* renormalised H of the last RHS */
+    cplx *PD;                           /* diagonal basis: P of the current RHS in the eigenbasis */
     double n_cond;
 } orc_coul;
 
@@ -696,8 +707,29 @@ static void coulomb_coefficients(const orc_sys *s, const cplx *P) {
     }
 }
 
+/* get_P_in_dia_vect (InitParameters.h:264-335): Pd = conj(U) P U^T of one k (the
+ * transposed storage kept), its diagonal made real. */
+static void p_in_eigenbasis(int nb, const cplx *P, const cplx *U, cplx *Pd) {
+    cplx M[ORC_MAXNB][ORC_MAXNB];
+    for (int ii = 0; ii < nb; ii++)
+        for (int jc = 0; jc < nb; jc++) {
+            cplx acc = 0;
+            for (int jj = 0; jj < nb; jj++) acc += P[ii * nb + jj] * U[jc * nb + jj];
+            M[jc][ii] = acc;
+        }
+    for (int ic = 0; ic < nb; ic++)
+        for (int jc = 0; jc < nb; jc++) {
+            cplx acc = 0;
+            for (int ii = 0; ii < nb; ii++) acc += conj(U[ic * nb + ii]) * M[jc][ii];
+            Pd[ic * nb + jc] = acc;
+        }
+    for (int ic = 0; ic < nb; ic++) Pd[ic * nb + ic] = creal(Pd[ic * nb + ic]);
+}
+
 /* Calculate_X_MPI (Coulomb.h:510-541) + peak correction and renormalised H
- * (RungeKutta.h:69-138) for one k; returns Hk (full). */
+ * (RungeKutta.h:69-138) for one k; returns Hk (full).  Diagonal basis: the
+ * eigenbasis corrections against P_Bloch_0 = the filled bands (main_MPI.cpp:291-331)
+ * and X_W = U^+ X_d U (get_Xd_in_Wannier_vect, InitParameters.h:340-370). */
 static void coulomb_hk(const orc_sys *s, long ik, const cplx *P, cplx Hk[ORC_MAXNB][ORC_MAXNB]) {
     orc_coul *c = s->coul;
     const int Ncv = s->nb, Ncut = c->p->ncut, nb2 = Ncv * Ncv;
@@ -717,7 +749,28 @@ static void coulomb_hk(const orc_sys *s, long ik, const cplx *P, cplx Hk[ORC_MAX
     }
     const cplx *Pk = &P[ik * nb2], *P0k = &c->PW0[ik * nb2];
     const cplx *scr = (const cplx *)c->p->screen;
-    if (c->n_cond > pow(10, -20)) {
+    if (c->p->diag_basis) {
+        const cplx *Pd = &c->PD[ik * nb2], *Uk = &s->U[ik * nb2];
+        for (int ic = 0; ic < Ncv; ic++) {
+            Xk[ic][ic] -= scr[0] * (Pd[ic * Ncv + ic] - (ic < s->ncc ? 1.0 : 0.0));
+            for (int jc = ic + 1; jc < Ncv; jc++) {
+                Xk[ic][jc] -= conj(scr[1]) * Pd[jc * Ncv + ic];
+                Xk[jc][ic] = conj(Xk[ic][jc]);
+            }
+        }
+        cplx M[ORC_MAXNB][ORC_MAXNB], XW[ORC_MAXNB][ORC_MAXNB];
+        memset(M, 0, sizeof M);
+        memset(XW, 0, sizeof XW);
+        for (int ii = 0; ii < Ncv; ii++)
+            for (int jj = 0; jj < Ncv; jj++)
+                for (int jc = 0; jc < Ncv; jc++) M[ii][jc] += Xk[ii][jj] * Uk[jj * Ncv + jc];
+        for (int ic = 0; ic < Ncv; ic++)
+            for (int ii = 0; ii < Ncv; ii++) {
+                const cplx cu = conj(Uk[ii * Ncv + ic]);
+                for (int jc = 0; jc < Ncv; jc++) XW[ic][jc] += cu * M[ii][jc];
+            }
+        memcpy(Xk, XW, sizeof Xk);
+    } else if (c->n_cond > pow(10, -20)) {
         for (int ic = 0; ic < Ncv; ic++)
             for (int jc = ic; jc < Ncv; jc++) {
                 const int idx = (ic != jc) ? ic * Ncv + jc : 0;
@@ -784,7 +837,14 @@ static void herm_eig(int n, const cplx *H, cplx *V) {
 /* EF[0] = pump sum, EF[1] = half probe */
 static void rhs(const orc_sys *s, const cplx *P, cplx *Pv, const double EF[2][3], int E_non0) {
     const int Ncv = s->nb, Nch = s->nch;
-    if (s->coul) coulomb_coefficients(s, P);
+    if (s->coul && s->coul->p->diag_basis) {
+        const int nb2 = Ncv * Ncv;
+#pragma omp parallel for schedule(static)
+        for (long ik = 0; ik < s->nk; ik++) p_in_eigenbasis(Ncv, &P[ik * nb2], &s->U[ik * nb2], &s->coul->PD[ik * nb2]);
+        coulomb_coefficients(s, s->coul->PD);
+    } else if (s->coul) {
+        coulomb_coefficients(s, P);
+    }
 #pragma omp parallel for schedule(static)
     for (long ik = 0; ik < s->nk; ik++) {
         const cplx *Pk = &P[ik * Ncv * Ncv];
@@ -922,8 +982,16 @@ static orc_coul *coul_init(orc_sys *s, const orc_coul_params *p, const cplx *P0)
     memcpy(c->PW0, P0, sizeof(cplx) * nk * nb2);
     c->Hren = malloc(sizeof(cplx) * nk * nb2);
     memcpy(c->Hren, s->H, sizeof(cplx) * nk * nb2);
+    c->PD = calloc((size_t)nk * nb2, sizeof(cplx));
     s->coul = c;
-    coulomb_coefficients(s, P0);
+    if (p->diag_basis) {
+        /* equilibrium in the eigenbasis: the filled bands, occupation 1 (main_MPI.cpp:291-303) */
+        for (long ik = 0; ik < nk; ik++)
+            for (int ic = 0; ic < s->ncc; ic++) c->PD[ik * nb2 + ic * s->nb + ic] = 1.0;
+        coulomb_coefficients(s, c->PD);
+    } else {
+        coulomb_coefficients(s, P0);
+    }
     for (int q = 0; q < 4; q++) memcpy(c->X0[q], c->X[q], sizeof(cplx) * Ncut * Ncut * nb2);
     for (int i = 0; i < s->nb; i++) c->EH0[i] = c->EH[i];
     c->n_cond = 0;
@@ -934,7 +1002,7 @@ static void coul_free(orc_coul *c) {
     if (!c) return;
     free(c->cosx); free(c->sinx); free(c->cosy); free(c->siny);
     for (int q = 0; q < 4; q++) { free(c->X[q]); free(c->X0[q]); }
-    free(c->PW0); free(c->Hren); free(c);
+    free(c->PW0); free(c->Hren); free(c->PD); free(c);
 }
 
 /* Test hook: X(k) of a state P with X0 = 0 and no peak correction:

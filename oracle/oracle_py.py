@@ -44,7 +44,7 @@ class OTables(C.Structure):
 
 class OCoul(C.Structure):
     _fields_ = [("ncut", C.c_int), ("dir1", C.c_int), ("dir2", C.c_int), ("kpt", _dp), ("A", _dp), ("B", _dp),
-                ("C", _dp), ("D", _dp), ("screen", _dp), ("vh", _dp)]
+                ("C", _dp), ("D", _dp), ("screen", _dp), ("vh", _dp), ("diag_basis", C.c_int)]
 
 
 class OObs(C.Structure):
@@ -253,14 +253,15 @@ class System:
         if coulomb is not None:
             self.set_coulomb(**coulomb)
 
-    def set_coulomb(self, ncut, kpt, A, B, C_, D, screen, vh, dir1=1, dir2=2):
-        """Coulomb mean-field tables (Fourier_Coulomb.cpp:505-622 layout)."""
+    def set_coulomb(self, ncut, kpt, A, B, C_, D, screen, vh, dir1=1, dir2=2, diag_basis=False):
+        """Coulomb mean-field tables (Fourier_Coulomb.cpp:505-622 layout); diag_basis:
+        the Coulomb_diag_basis form (sums over P in the eigenbasis)."""
         self._ckeep = [np.ascontiguousarray(kpt, dtype=np.float64)] + [
             np.ascontiguousarray(x, dtype=np.complex128) for x in (A, B, C_, D, screen, vh)]
         k, a, b, c, d, sc, v = self._ckeep
         self._coul = OCoul(ncut, dir1, dir2, _d(k), _d(a.view(np.float64)), _d(b.view(np.float64)),
                            _d(c.view(np.float64)), _d(d.view(np.float64)), _d(sc.view(np.float64)),
-                           _d(v.view(np.float64)))
+                           _d(v.view(np.float64)), 1 if diag_basis else 0)
         self.t.coul = C.cast(C.pointer(self._coul), C.c_void_p)
 
     def coulomb_x(self, P):

@@ -158,7 +158,7 @@ def write_crystal_input(d, nkf=25, nk=24, norb=3, valence=1, stepper="", nsteps=
 
 
 def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, coulomb=False, coul_dir=COULOMB,
-                  coul_label=COULOMB_LABEL):
+                  coul_label=COULOMB_LABEL, diag_basis=False):
     """Oracle tables for a setup: k-grid, shells, neighbours and model from the oracle's
     own restatement (tight binding), the setup's parameters and pulses.  With
     coulomb, the mean-field coefficients are parsed from the files in coul_dir
@@ -189,7 +189,7 @@ def oracle_system(setup, use_oracle_tables=True, bruteforce_neighbours=True, cou
     if coulomb:
         ncut = info["ncut"]
         cf = read_coulomb_files(coul_dir, coul_label, nb, ncut)
-        sys_.set_coulomb(ncut, kpt, cf["A"], cf["B"], cf["C"], cf["D"], cf["screen"], cf["vh"])
+        sys_.set_coulomb(ncut, kpt, cf["A"], cf["B"], cf["C"], cf["D"], cf["screen"], cf["vh"], diag_basis=diag_basis)
     return {"sys": sys_, "kpt": kpt, "bvec": bvec, "w": w, "sizes": sizes, "res": res, "nbr": nbr, "H": H, "U": U, "D": D,
             "T": T}
 

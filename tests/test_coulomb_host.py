@@ -27,12 +27,19 @@ def test_coefficient_files_read_like_numpy(tmp_path):
     assert np.max(np.abs(ref["A"][:, [0, 4, 8]].imag)) == 0.0
 
 
-def test_coulomb_diag_basis_rejected(tmp_path):
-    p = Hh.coulomb_input(str(tmp_path), nk=12)
-    text = open(p).read().replace("Read_Coulomb_from_files\n", "Read_Coulomb_from_files\nCoulomb_diag_basis\n")
-    open(p, "w").write(text)
-    with pytest.raises(RuntimeError, match="Coulomb_diag_basis"):
-        N.Setup(p)
+def test_coulomb_diag_basis_accepted_with_the_same_tables(tmp_path):
+    """Coulomb_diag_basis changes only where the mean-field sums run, not the
+    setup's tables."""
+    (tmp_path / "w").mkdir()
+    (tmp_path / "d").mkdir()
+    pw = Hh.coulomb_input(str(tmp_path / "w"), nk=12)
+    pd = Hh.coulomb_input(str(tmp_path / "d"), nk=12)
+    text = open(pd).read().replace("Read_Coulomb_from_files\n", "Read_Coulomb_from_files\nCoulomb_diag_basis\n")
+    open(pd, "w").write(text)
+    sw, sd = N.Setup(pw), N.Setup(pd)
+    for name in ("H", "U", "D", "rho0"):
+        np.testing.assert_array_equal(sd.array(name), sw.array(name))
+    assert sd.info["ncut"] == sw.info["ncut"] == 20
 
 
 def test_missing_coefficient_file(tmp_path):

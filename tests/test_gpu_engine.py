@@ -184,6 +184,30 @@ def test_wannier_contraction_fp32_within_rounding_bound(tmp_path, nw):
     np.testing.assert_array_equal(s32.array("H", True).reshape(H.shape), H)
 
 
+@pytest.mark.parametrize("nw,rmax", [(5, 3), (16, 3), (32, 2)])
+def test_wannier_contraction_bf16x3_within_split_bound(tmp_path, nw, rmax):
+    """BF16x3 split products (hi hi + hi lo + lo hi of bf16 halves, FP32 sums)
+    against the FP64 direct sum.  Bound per element: sqrt(2) (3 2^-18 + (2 nR + 8)
+    2^-24) sum_R |X(R)_mn| (the split keeps 16 mantissa bits; products of bf16
+    are exact in FP32; the FP32 sums); a wrong lane layout or dropped hi-lo term
+    errs far above it (the hi*hi part alone errs by ~2^-9)."""
+    s = _wannier_setup(str(tmp_path), nw, rmax=rmax)
+    seed = [f for f in os.listdir(tmp_path) if f.endswith("_tb.dat")][0]
+    R, HR, rR = _read_tb(os.path.join(tmp_path, seed), nw)
+    kpt = O.kgrid(1, 12, 12)
+    oH, oD = O.wannier_tables(kpt, R, HR, rR, nw)
+    H, D = N.wannier_hk(np.array([1, 12, 12]), R, HR, rR, nw, bf16x3=True)
+    g = np.sqrt(2.0) * (3 * 2.0 ** -18 + (2 * len(R) + 8) * 2.0 ** -24)
+    bH = g * np.sum(np.abs(HR), axis=0)
+    bR = g * np.sum(np.abs(rR), axis=0)
+    bD = 0.5 * (bR + np.swapaxes(bR, 0, 1))
+    eH, eD = np.abs(H - oH), np.abs(D - oD)
+    print(f"nw={nw} nR={len(R)} max err H {eH.max():.3e} (bound {bH.max():.3e}), D {eD.max():.3e} (bound {bD.max():.3e})",
+          flush=True)
+    assert np.all(eH <= bH[None]) and np.all(eD <= bD[None])
+    assert eH.max() > 2.0 ** -24 * np.abs(oH).max()  # the split form is the one that ran
+
+
 @pytest.mark.parametrize("nw,t2", [(5, 0.0), (8, 10.0), (12, 0.0), (16, 0.0), (16, 10.0), (20, 10.0), (32, 0.0), (32, 10.0)])
 def test_rhs_wave_kernel_matches_oracle(tmp_path, nw, t2):
     s = _wannier_setup(str(tmp_path), nw, t2=t2)

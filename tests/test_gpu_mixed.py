@@ -7,8 +7,9 @@ computed from the inputs: every FP32 product C = A B of length nb carries at mos
 (nb + 2) u sum_l |A_il||B_lj| (u = 2^-24, operands rounded once), and the
 commutator doubles it; the dephasing sandwich chains four such products of the
 unitary.  A wrong index, sign or operand gives errors of order |M||P| itself,
-far above this bound.  The trajectory check is looser (rounding accumulates over
-the steps): 1e-4 of each series' maximum.
+far above this bound.  Trajectories of both mixed forms (EDUS_PRECISION_MIXED and
+EDUS_PRECISION_MIXED32, the latter with FP32 stage inputs and histories) keep
+J(t) within 1e-6 of its maximum.
 """
 import numpy as np
 import pytest
@@ -66,21 +67,48 @@ def test_mixed_rejected_for_small_band_counts(tmp_path):
         N.Engine(s.problem(), precision=N.PRECISION_MIXED)
 
 
-def test_trajectory_mixed_close_to_fp64_oracle(tmp_path):
+@pytest.mark.parametrize("prec", ["mixed", "mixed32"])
+@pytest.mark.parametrize("case", ["nw16_rk4", "nw32_2pumps_rk4", "nw32_2pumps_taylor4"])
+def test_trajectory_mixed_close_to_fp64_oracle(tmp_path, case, prec):
+    """J1, the band populations and the final state of the mixed forms against the
+    FP64 oracle over the steps around the pulse peak of a 16-band model and of the
+    two-pump 32-band model (RK4 and Taylor-4): J(t) within 1e-6 of its maximum (the
+    J(t) bar of the mixed form), the final state within 1e-6."""
     from edus_amd import workloads
-    p = workloads.wannier_synthetic(str(tmp_path), nw=16, rmax=2, nk=12, nsteps=400, dt=0.05, stepper="RK4",
-                                    order=4, nvalence=4, seed=11, e_v_ang=0.5, it_resolution=4)
+    if case == "nw16_rk4":
+        p = workloads.wannier_synthetic(str(tmp_path), nw=16, rmax=2, nk=12, nsteps=400, dt=0.05, stepper="RK4",
+                                        order=4, nvalence=4, seed=11, e_v_ang=0.5, it_resolution=4)
+    else:
+        p = workloads.wannier_synthetic(str(tmp_path), nw=32, rmax=2, nk=12, nsteps=400, dt=0.04,
+                                        stepper="RK4" if case.endswith("rk4") else "Taylor", order=4, nvalence=8,
+                                        seed=4004, decay_ang=3.0, e_v_ang=0.4, photon_ev=0.4, fwhm_fs=20.0,
+                                        extra_pumps=((0.8, 0.1),), it_resolution=4)
     s = N.Setup(p)
+    nb = s.info["nb"]
     osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=False)
-    n = min(80, s.info["itfi"] + 1)
+    n = min(120, s.info["itfi"] + 1)
     ora, Pora = Hh.oracle_run(s, osys, n)
     prob = s.problem()
-    prob.precision = N.PRECISION_MIXED
+    prob.precision = N.PRECISION_MIXED if prec == "mixed" else N.PRECISION_MIXED32
     gpu, eng = run_engine_rows(s, n, problem=prob)
-    g, o = Hh.split_rows(gpu, 16), Hh.split_rows(ora, 16)
-    for key in ("J1", "pop"):
+    g, o = Hh.split_rows(gpu, nb), Hh.split_rows(ora, nb)
+    # J(t) within 1e-6 of its maximum (measured ~6e-8); the conduction-band
+    # populations, which the pulse raises to a few 1e-4, within 1e-5 of their
+    # maximum (measured up to 2e-6, i.e. ~1e-9 absolute)
+    for key, bar in (("J1", 1e-6), ("pop", 1e-5)):
         scale = np.max(np.abs(o[key]))
         err = np.max(np.abs(g[key] - o[key]))
-        print(key, "max err / series max", err / scale, flush=True)
-        assert err <= 1e-4 * scale, key
-    assert np.max(np.abs(eng.get_state() - Pora)) <= 1e-4 * np.max(np.abs(Pora))
+        print(case, prec, key, "max err / series max", err / scale, flush=True)
+        assert err <= bar * scale, key
+        assert err > 0  # the FP32 path is the one that ran
+    err = np.max(np.abs(eng.get_state() - Pora)) / np.max(np.abs(Pora))
+    print(case, prec, "final state", err, flush=True)
+    assert err <= 1e-6
+
+
+def test_mixed32_rejected_in_batches(tmp_path):
+    s = _wannier_setup(str(tmp_path), 8)
+    p = s.problem()
+    p.precision = N.PRECISION_MIXED32
+    with pytest.raises(RuntimeError, match="batches run FP64"):
+        N.Batch([p])
