@@ -82,12 +82,6 @@ struct ConstArgs {
     double *Hr;                // packed renormalised H (written)
     int coul_diag;             // Coulomb_diag_basis: the sums ran on PD, corrections in the eigenbasis
     const double *PD;          // packed state in the eigenbasis, conj(U) P U^T (diagonal basis)
-    // A stencil that is the same integer offset at every k of the uniform grid
-    // (every grid of the reference's BZ_parameters): the kernels form k + b from k
-    // and the offset instead of loading it from nbr, which saves the dependent
-    // round trip to memory before the gather.  gn0 = 0: read the table.
-    int gn0, gn1, gn2;
-    signed char goff[kMaxNbr][3];
 };
 
 // Per-launch stage data.  `tab` points at the stage's row of the field table:
@@ -181,28 +175,6 @@ __device__ __forceinline__ int strip_k(int L, int nk, int sw, int scol) {
     return (u / wr) * scol + nfull * sw + u % wr;
 }
 
-// Index of neighbour n of k (k < nk): from the grid offsets when the context has
-// them (offsets are within half the grid, so one wrap per axis suffices), else
-// from the table.
-struct GridPos {
-    int i0, i1, i2;
-};
-__device__ __forceinline__ GridPos grid_pos(const ConstArgs &c, int k) {
-    if (!c.gn0) return GridPos{0, 0, 0};
-    const int n12 = c.gn1 * c.gn2, i0 = k / n12, r = k - i0 * n12, i1 = r / c.gn2;
-    return GridPos{i0, i1, r - i1 * c.gn2};
-}
-__device__ __forceinline__ int wrap1(int v, int n) {
-    v += v < 0 ? n : 0;
-    return v >= n ? v - n : v;
-}
-__device__ __forceinline__ int nbr_of(const ConstArgs &c, const GridPos &g, int k, int n) {
-    if (c.gn0)
-        return (wrap1(g.i0 + c.goff[n][0], c.gn0) * c.gn1 + wrap1(g.i1 + c.goff[n][1], c.gn1)) * c.gn2 +
-               wrap1(g.i2 + c.goff[n][2], c.gn2);
-    return c.nbr[(size_t)n * c.nkp + k];
-}
-
 // Pair index of (i<j) in row-major upper-triangle order.
 __host__ __device__ __forceinline__ int pair_index(int i, int j, int nb) { return i * nb - i * (i + 1) / 2 + (j - i - 1); }
 
@@ -277,10 +249,9 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
     // the end waits for one round trip, not two.
     constexpr int kPreNbr = NB <= 3 ? 12 : 0;  // a multiple of the batch size below
     int kpre[kPreNbr > 0 ? kPreNbr : 1];
-    const GridPos gp = grid_pos(c, k);
     if (grad) {
 #pragma unroll
-        for (int n = 0; n < kPreNbr; n++) kpre[n] = nbr_of(c, gp, k, min(n, nn - 1));
+        for (int n = 0; n < kPreNbr; n++) kpre[n] = c.nbr[(size_t)min(n, nn - 1) * nkp + k];
     }
 
     double y[NP], pre0[NP], pre1[NP];
@@ -541,7 +512,7 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
         for (int n0 = kPreNbr; n0 < nn; n0 += GB) {
             int kk[GB];
 #pragma unroll
-            for (int j = 0; j < GB; j++) kk[j] = nbr_of(c, gp, k, min(n0 + j, nn - 1));
+            for (int j = 0; j < GB; j++) kk[j] = c.nbr[(size_t)min(n0 + j, nn - 1) * nkp + k];
             batch(n0, kk);
         }
 #pragma unroll
@@ -1166,11 +1137,10 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     // flight together, and the sum keeps the stencil order.
     if (grad) {
         constexpr int GB = WaveGeom<NBT>::GATHER;
-        const GridPos gp = grid_pos(c, k);
         for (int n0 = 0; n0 < nn; n0 += GB) {
             size_t kb[GB];
 #pragma unroll
-            for (int j = 0; j < GB; j++) kb[j] = (size_t)nbr_of(c, gp, k, min(n0 + j, nn - 1)) * NP;
+            for (int j = 0; j < GB; j++) kb[j] = (size_t)c.nbr[(size_t)min(n0 + j, nn - 1) * c.nkp + k] * NP;
             double v[GB][PER];
 #pragma unroll
             for (int j = 0; j < GB; j++)
@@ -2145,8 +2115,6 @@ struct edus_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
     int sw = 0, scol = 0;  // strip order of the wave kernels (strip_k)
-    int gn[3] = {0, 0, 0};  // grid sizes of a translation-invariant stencil (ConstArgs::gn0..2), or 0
-    signed char goff[kMaxNbr][3] = {};
     int precision = EDUS_PRECISION_FP64;
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
     int *eidx = nullptr;  // packed element -> (i, j, comp) of the wave layout
@@ -2314,37 +2282,6 @@ int launch_coulomb_of(edus_ctx *c, const double *Y) {
     return launch_coulomb(c, c->cPD);
 }
 
-// Whether the neighbour table is one integer offset per neighbour on the grid
-// nk_dims, the same at every k (k + b wrapped per axis): then the kernels form
-// the neighbours from the offsets (ConstArgs::goff) and skip the table.  Explicit
-// k lists and tables of any other form keep the table.
-void grid_stencil(edus_ctx *c, const edus_problem *p) {
-    const int N[3] = {p->nk_dims[0], p->nk_dims[1], p->nk_dims[2]};
-    const int nn = p->n_nbr;
-    if (nn < 1 || nn > kMaxNbr || N[0] < 1 || N[1] < 1 || N[2] < 1 || (long long)N[0] * N[1] * N[2] != p->nk) return;
-    signed char off[kMaxNbr][3] = {};
-    for (int n = 0; n < nn; n++) {  // the offsets at k = 0
-        const int j = p->nbr[n];
-        const int jj[3] = {j / (N[1] * N[2]), (j / N[2]) % N[1], j % N[2]};
-        for (int x = 0; x < 3; x++) {
-            const int o = jj[x] > N[x] / 2 ? jj[x] - N[x] : jj[x];
-            if (o < -127 || o > 127) return;
-            off[n][x] = (signed char)o;
-        }
-    }
-    auto wrap = [](int v, int n) { v %= n; return v < 0 ? v + n : v; };
-    for (int k = 0; k < p->nk; k++) {
-        const int ii[3] = {k / (N[1] * N[2]), (k / N[2]) % N[1], k % N[2]};
-        for (int n = 0; n < nn; n++) {
-            const int want = (wrap(ii[0] + off[n][0], N[0]) * N[1] + wrap(ii[1] + off[n][1], N[1])) * N[2] +
-                             wrap(ii[2] + off[n][2], N[2]);
-            if (p->nbr[(size_t)k * nn + n] != want) return;
-        }
-    }
-    for (int x = 0; x < 3; x++) c->gn[x] = N[x];
-    std::memcpy(c->goff, off, sizeof off);
-}
-
 ConstArgs const_args(const edus_ctx *c) {
     ConstArgs ca;
     std::memset(&ca, 0, sizeof ca);
@@ -2357,8 +2294,6 @@ ConstArgs const_args(const edus_ctx *c) {
     ca.Xw = c->cXw;
     ca.coul_diag = c->coul_diag;
     ca.PD = c->cPD;
-    ca.gn0 = c->gn[0]; ca.gn1 = c->gn[1]; ca.gn2 = c->gn[2];
-    std::memcpy(ca.goff, c->goff, sizeof ca.goff);
     return ca;
 }
 
@@ -2663,7 +2598,6 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
         if ((rc = dalloc(c, &c->nbr, nbr.size()))) return fail(rc);
         if (hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "upload nbr"));
-        grid_stencil(c, p);
     }
     c->obs_blocks = std::max(1, std::min(kObsMaxBlocks, p->nk));
     if ((rc = dalloc(c, &c->part, (size_t)c->obs_blocks * (nb + 17))) || (rc = dalloc(c, &c->obs, nb + 17))) return fail(rc);
