@@ -464,7 +464,19 @@ def bench_batch(args):
     seq_wall = time.perf_counter() - t1
     seq_value = float(nk.sum()) * seq_steps / seq_wall
     S = 16 * nb * nb
-    bytes_step = float(np.sum((16 + 4 * n_op) * S * nk))   # RK4: S (16 + 4 n_op) per k and step
+    survey_step = float(np.sum((16 + 4 * n_op) * S * nk))   # survey figure: S (16 + 4 n_op) per k and RK4 step
+    # own layout (as for the single workloads): 16 packed FP64 state transfers per RK4
+    # step, and per stage packed H, the dipole components the instance's fields use
+    # and full complex U with dephasing, at 4 bytes for members in mixed precision
+    own = []
+    for s_, (f, _), p in zip(setups, fg, problems):
+        b = s_.info["nb"]
+        ft = f[args.warmup:]
+        nd = sum(1 for x in range(3) if np.any(ft[..., x] != 0) or np.any(ft[..., 3 + x] != 0))
+        opw = 4 if p.precision != N.PRECISION_FP64 else 8
+        per_k = 16 * 8 * b * b + 4 * (opw * b * b * (1 + nd) + (2 * opw * b * b if s_.info["n_diss"] > 0 else 0))
+        own.append(per_k * s_.info["nk"])
+    bytes_step = float(sum(own))
     achieved = bytes_step * args.steps / (dev_ms * 1e-3) / 1e9
     classes = sorted({(int(b) if b <= 4 else (8 if b <= 8 else 16 if b <= 16 else 32)) for b in nb})
     print(json.dumps({
@@ -481,7 +493,10 @@ def bench_batch(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_stage_small_batch / k_stage_wave_batch (RK4 stages)",
-                     "bytes_per_step": bytes_step},
+                     "bytes_per_step": bytes_step,
+                     "bytes_model": "packed Hermitian state and operators, the dipole components each member's fields use",
+                     "survey_bytes_per_step": survey_step,
+                     "survey_rate_gbs": round(survey_step * args.steps / (dev_ms * 1e-3) / 1e9, 2)},
         "cpu_baseline": None}))
 
 
