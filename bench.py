@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--cpu-subgrid", type=int, default=128,
                     help="grid points per direction of the CPU sample when the workload's grid is above 200x200")
     ap.add_argument("--no-probe", action="store_true", help="skip the streaming-rate probe")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch schedule instead of timing the candidates (edus_autotune)")
     ap.add_argument("--graph", action="store_true",
                     help="RK4 steps from a device-resident field table through the captured step graph "
                          "(edus_set_field_table + edus_step_rk4_graph) instead of per-call stage launches")
@@ -172,6 +174,9 @@ def time_form(s, form, f, g, args):
     t0 = time.perf_counter()
     eng = N.Engine(s.problem(), precision=prec)
     t_create = time.perf_counter() - t0
+    schedule = None
+    if not getattr(args, "no_autotune", False):  # the timed steps' own field, before any step
+        schedule = eng.autotune(f[args.warmup, 0], bool(g[args.warmup]), reps=3)
     if getattr(args, "graph", False) and eng.stepper == N.STEPPER_RK4:
         eng.set_field_table(f, g)
         if args.warmup:
@@ -186,7 +191,7 @@ def time_form(s, form, f, g, args):
         eng.step(f[args.warmup:], g[args.warmup:])
         wall = time.perf_counter() - t0
     dev_ms, nlaunch = eng.last_step_time()
-    return eng, {"wall_s": wall, "dev_ms": dev_ms, "nlaunch": nlaunch, "create_s": t_create}
+    return eng, {"wall_s": wall, "dev_ms": dev_ms, "nlaunch": nlaunch, "create_s": t_create, "schedule": schedule}
 
 
 def bench_single(args):
@@ -247,6 +252,8 @@ def bench_single(args):
             "value": nk * args.steps / t["wall_s"], "ms_per_step": t["wall_s"] * 1e3 / args.steps,
             "device_ms_per_step": t["dev_ms"] / args.steps, "avg_launch_us": round(avg_launch_s * 1e6, 3),
             "launches_per_step": t["nlaunch"] // args.steps, "engine_create_s": round(t["create_s"], 2),
+            "schedule": (None if t["schedule"] is None else
+                         {("strip_width" if nb > 4 else "workgroup_threads"): t["schedule"], "picked_by": "edus_autotune"}),
             "kernel": m["kernel"] + (" (RK4 stages)" if m["stepper"] == "RK4" else f" ({m['stepper']} steps)")
                       + {"fp64": "", "mixed": ", FP32 operators and products",
                          "mixed32": ", FP32 operators, products and stage storage"}[form],
@@ -319,7 +326,7 @@ def bench_single(args):
                    "form": head, "launch": "captured step graph" if args.graph else "stream launches per call",
                    "wannier_contraction": args.contraction if args.workload in ("wannier", "wannier32") else None,
                    "state_layout": "packed-hermitian", "setup_s": round(t_setup, 2),
-                   "device_ms_per_step": lead["device_ms_per_step"]},
+                   "device_ms_per_step": lead["device_ms_per_step"], "schedule": lead["schedule"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -2115,6 +2115,7 @@ struct edus_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
     int sw = 0, scol = 0;  // strip order of the wave kernels (strip_k)
+    int tpb = 0;           // threads per workgroup of the nb <= 4 kernel (0: by grid size)
     int precision = EDUS_PRECISION_FP64;
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
     int *eidx = nullptr;  // packed element -> (i, j, comp) of the wave layout
@@ -2309,7 +2310,7 @@ int launch_stage(edus_ctx *c, StageArgs a) {
         // A grid of fewer k than 256 per CU runs 64-thread workgroups, so that
         // more CUs share it (the k past nk, up to the padded nkp, are computed
         // and never read).
-        const int tpb = c->nk < 256 * 256 ? 64 : 256, blocks = (c->nk + tpb - 1) / tpb;
+        const int tpb = c->tpb ? c->tpb : (c->nk < 256 * 256 ? 64 : 256), blocks = (c->nk + tpb - 1) / tpb;
         auto go = [&](auto dtag) {
             constexpr bool DISS = decltype(dtag)::value;
             switch (c->nb) {
@@ -3075,6 +3076,52 @@ int edus_set_state(edus_ctx *c, const double *rho) {
     pack_state(c, rho, st);
     HIP_TRY(hipMemcpyAsync(const_cast<double *>(current_state(c)), st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Schedule of the stage kernels picked by timing the context's own right-hand
+// side: every candidate gives the same sums for every k (only the order of the
+// workgroups or their size changes), so the choice cannot change a result.
+int edus_autotune(edus_ctx *c, const double *field6, uint8_t grad_on, int32_t reps, int32_t *chosen) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!field6 || reps < 1 || reps > 1000) return set_err(EDUS_ERR_INVALID, "bad autotune arguments");
+    std::vector<double> row(7 + c->n_nbr);
+    table_row(c, field6, grad_on, row.data());
+    if ((rc = upload_table(c, row))) return rc;
+    StageArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.tab = c->tab;
+    a.Y = current_state(c);
+    a.OUT = c->YB;  // scratch between steps: the state is not touched
+    std::vector<int> cand;
+    if (c->soa) {
+        cand = {64, 128, 256};
+    } else {
+        cand.push_back(0);
+        if (c->scol > 0)
+            for (int w : {16, 32, 64, 128, 256})
+                if (w < c->scol) cand.push_back(w);
+    }
+    int &knob = c->soa ? c->tpb : c->sw;
+    const int before = knob;
+    int best = cand[0];
+    float best_ms = 0.f;
+    for (size_t i = 0; i < cand.size(); i++) {
+        knob = cand[i];
+        if ((rc = launch_stage<RHSO>(c, a))) { knob = before; return rc; }  // untimed
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < reps; r++)
+            if ((rc = launch_stage<RHSO>(c, a))) { knob = before; return rc; }
+        HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (i == 0 || ms < best_ms) { best_ms = ms; best = cand[i]; }
+    }
+    knob = best;
+    if (best != before) drop_graph(c);  // a captured step holds the old launch shape
+    if (chosen) *chosen = best;
     return 0;
 }
 

@@ -100,6 +100,7 @@ hip.edus_rhs.argtypes = [C.c_void_p, _dp, C.c_uint8, _dp]
 hip.edus_get_state.argtypes = [C.c_void_p, _dp]
 hip.edus_set_state.argtypes = [C.c_void_p, _dp]
 hip.edus_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
+hip.edus_autotune.argtypes = [C.c_void_p, _dp, C.c_uint8, C.c_int32, C.POINTER(C.c_int32)]
 hip.edus_wannier_hk.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
 hip.edus_wannier_hk_fp32.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
 hip.edus_last_error.restype = C.c_char_p
@@ -402,6 +403,14 @@ class Engine:
         n = C.c_int32()
         hip.edus_last_step_time(self._h, C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+    def autotune(self, field6, grad_on=True, reps=5):
+        """Launch schedule picked by timing (edus_autotune); returns the kept strip
+        width (5..32 bands) or workgroup size (1..4 bands)."""
+        f = np.ascontiguousarray(field6, dtype=np.float64)
+        ch = C.c_int32()
+        _check_hip(hip.edus_autotune(self._h, _dptr(f), 1 if grad_on else 0, reps, C.byref(ch)), "edus_autotune")
+        return ch.value
 
 
 def wannier_hk(nk3, R, HR, rR, nw, fp32=False, bf16x3=False):

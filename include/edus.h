@@ -198,6 +198,17 @@ int edus_set_state(edus_ctx *ctx, const double *rho);
  * with HIP events on the context's stream; *nlaunch receives the kernel count. */
 int edus_last_step_time(edus_ctx *ctx, double *ms, int32_t *nlaunch);
 
+/* Picks the stage kernels' launch schedule by measurement on the context's own
+ * data and keeps it for every later launch: for 5..32 bands the strip width of
+ * the workgroup walk over the grid (0 = k order, or 16..256 columns narrower than
+ * a grid row), for 1..4 bands the threads per workgroup (64, 128, 256).  Each
+ * candidate runs `reps` right-hand-side launches with the field `field6` (as for
+ * edus_rhs) into scratch memory; the state and the step results are unaffected
+ * (every candidate forms the same sums for every k).  `chosen` (may be NULL)
+ * receives the strip width or the workgroup size kept.  No reference
+ * counterpart: the reference has no launch schedule. */
+int edus_autotune(edus_ctx *ctx, const double *field6, uint8_t grad_on, int32_t reps, int32_t *chosen);
+
 /*
  * Batches: n independent RK4 problems (different grids, band counts, models and
  * pulses; no Coulomb term) advanced together.  No reference counterpart: the

@@ -13,7 +13,9 @@
  *   orc_kgrid            InitParameters.h:13-40    k_grid.txt of the AB5 run (4900 rows)
  *   orc_shells           CalculateWeigths.h:1-138  b-vectors and residuals printed in the AB5 output.log;
  *                                                  B1 condition sum w b_a b_b = delta_ab
- *   orc_neighbours       CalculateWeigths.h:145-226 closed form k+b on small grids (tests), invariants
+ *   orc_neighbours       CalculateWeigths.h:145-226 closed form k+b on small grids (tests), invariants;
+ *                                                  on k lists (nkPT read) the uniform grid written as a list,
+ *                                                  in order and shuffled, gives the grid's table permuted
  *   orc_tb_tables        TightBinding.h            Test_matrices.txt of the AB5 run; U H U^+ diagonal, U U^+ = 1;
  *                                                  every family: closed-form bands (graphene +-b|f| and its
  *                                                  Dirac point, the bilayer's +-(S +- d)/2, BN +-sqrt(g^2/4 +

@@ -69,3 +69,26 @@ def test_list_missing_a_neighbour_is_an_error(tmp_path):
     text = text.replace("nkPT read {\n1 8 8\n", "nkPT read {\n1 9 7\n")  # 63 points listed
     with pytest.raises(N.EdusError, match="is not in the k list"):
         _setup(tmp_path, "missing", text)
+
+
+def test_oracle_search_on_a_shuffled_list(tmp_path):
+    """The oracle's own shell and neighbour search (brute force, pinned on grids by
+    the closed form) on the shuffled list: the same shells as on the grid (b vectors
+    taken in list order) and the grid's neighbour table, permuted like the points."""
+    import helpers as Hh
+    g = N.Setup(tb_input(str(tmp_path), nk=12, t2=0.01))
+    nk = g.info["nk"]
+    order = np.random.default_rng(11).permutation(nk)
+    text, kpt = _listed(str(tmp_path / "input.txt"), order)
+    s = _setup(tmp_path, "shuffled", text)
+    og = Hh.oracle_system(g, use_oracle_tables=True, bruteforce_neighbours=True)
+    ol = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=True)
+    np.testing.assert_array_equal(ol["sizes"], og["sizes"])
+    np.testing.assert_allclose(ol["w"], og["w"], rtol=1e-12)
+    wl, wg = ol["bvec"], og["bvec"]
+    col = [int(np.flatnonzero(np.all(np.abs(wg - row) <= 1e-12 * np.abs(wg).max(), axis=1))[0]) for row in wl]
+    assert sorted(col) == list(range(len(wg)))
+    inv = np.argsort(order)
+    np.testing.assert_array_equal(ol["nbr"], inv[og["nbr"][order][:, col]])
+    # and the host's hashed search gives the oracle's table
+    np.testing.assert_array_equal(s.nbr(), ol["nbr"])
