@@ -186,6 +186,38 @@ __device__ __forceinline__ float ldnt(const float *p) { return __builtin_nontemp
 __device__ __forceinline__ void stnt(double *p, double v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void stnt(float *p, float v) { __builtin_nontemporal_store(v, p); }
 
+// Two adjacent packed elements in one access (16 bytes in FP64, 8 in FP32), when
+// `vec` (the pair lies inside the matrix and the array offset is even); else two
+// single accesses at o0 and o1.  Halves the vector memory instructions of the
+// wave kernels, whose address unit is their busiest memory stage.
+template <typename T> struct Pair;
+template <> struct Pair<double> { typedef double V __attribute__((ext_vector_type(2))); };
+template <> struct Pair<float> { typedef float V __attribute__((ext_vector_type(2))); };
+template <bool NT, typename T>
+__device__ __forceinline__ void ld2(const T *p, size_t o0, size_t o1, bool vec, T &a, T &b) {
+    if (vec) {
+        const typename Pair<T>::V *q = reinterpret_cast<const typename Pair<T>::V *>(p + o0);
+        const typename Pair<T>::V v = NT ? __builtin_nontemporal_load(q) : *q;
+        a = v.x;
+        b = v.y;
+    } else {
+        a = NT ? ldnt(p + o0) : p[o0];
+        b = NT ? ldnt(p + o1) : p[o1];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void st2(T *p, size_t o0, size_t o1, bool vec, T a, T b) {
+    if (vec) {
+        typename Pair<T>::V v;
+        v.x = a;
+        v.y = b;
+        __builtin_nontemporal_store(v, reinterpret_cast<typename Pair<T>::V *>(p + o0));
+    } else {
+        stnt(p + o0, a);
+        stnt(p + o1, b);
+    }
+}
+
 // ---------------------------------------------------------------- stage update
 // The elementwise tail shared by all kernels.  `kv` is one packed double of the
 // right-hand side, `y` the same double of the stage input, `o` its offset in the
@@ -1109,27 +1141,63 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     // Lanes past the last packed element (nb < NBT) repeat element NP-1: every
     // load stays in bounds without a branch, and their LDS and global stores
     // write the same value to the same address as the lane that owns it.
+    // With an even number of doubles per thread, a lane owns adjacent pairs of
+    // elements (2q, 2q+1), q = lane + TPK m', read and written 16 bytes at a time
+    // when nb is even (NP even keeps every pair inside one k and aligned); the
+    // pair is a real diagonal pair or the (re, im) of one upper element.
+    constexpr bool kPairs = PER % 2 == 0;
+    bool pv2[PER > 1 ? PER / 2 : 1];
     double y[PER], pv[PER], g[PER], pre0[PER], pre1[PER];
     R hv[PER], dv[PER][3];
     int eo[PER], ei[PER], ej[PER], ec[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = min(lane + TPK * m, NP - 1);
+        const int e = kPairs ? min(2 * (lane + TPK * (m >> 1)) + (m & 1), NP - 1) : min(lane + TPK * m, NP - 1);
         eo[m] = e;
         const int code = c.eidx[e];
         ei[m] = code & 255;
         ej[m] = (code >> 8) & 255;
         ec[m] = code >> 16;
         g[m] = pre0[m] = pre1[m] = 0.0;
-        y[m] = (double)Yin[base + e];
-        hv[m] = ldnt(Hop + base + e);
+    }
+    if constexpr (kPairs) {
 #pragma unroll
-        for (int x = 0; x < 3; x++) dv[m][x] = (a.dmask >> x & 1) ? ldnt(Dop + (size_t)x * c.nk * NP + base + e) : (R)0;
-        if constexpr (MODE == RK2 || MODE == RK3) {
-            pre0[m] = ldnt(a.ACC + base + e);
-            pre1[m] = ldnt(a.P0 + base + e);
-        } else if constexpr (MODE == RK4S) {
-            pre0[m] = ldnt(a.ACC + base + e);
+        for (int h = 0; h < PER / 2; h++) {
+            const int m = 2 * h;
+            pv2[h] = (NP % 2 == 0) && 2 * (lane + TPK * h) + 1 < NP;
+            const bool v2 = pv2[h];
+            const size_t o0 = base + eo[m], o1 = base + eo[m + 1];
+            YT y0, y1;
+            ld2<false>(Yin, o0, o1, v2, y0, y1);
+            y[m] = (double)y0;
+            y[m + 1] = (double)y1;
+            ld2<true>(Hop, o0, o1, v2, hv[m], hv[m + 1]);
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                if (a.dmask >> x & 1) ld2<true>(Dop + (size_t)x * c.nk * NP, o0, o1, v2, dv[m][x], dv[m + 1][x]);
+                else dv[m][x] = dv[m + 1][x] = (R)0;
+            }
+            if constexpr (MODE == RK2 || MODE == RK3) {
+                ld2<true>(a.ACC, o0, o1, v2, pre0[m], pre0[m + 1]);
+                ld2<true>(a.P0, o0, o1, v2, pre1[m], pre1[m + 1]);
+            } else if constexpr (MODE == RK4S) {
+                ld2<true>(a.ACC, o0, o1, v2, pre0[m], pre0[m + 1]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = eo[m];
+            y[m] = (double)Yin[base + e];
+            hv[m] = ldnt(Hop + base + e);
+#pragma unroll
+            for (int x = 0; x < 3; x++) dv[m][x] = (a.dmask >> x & 1) ? ldnt(Dop + (size_t)x * c.nk * NP + base + e) : (R)0;
+            if constexpr (MODE == RK2 || MODE == RK3) {
+                pre0[m] = ldnt(a.ACC + base + e);
+                pre1[m] = ldnt(a.P0 + base + e);
+            } else if constexpr (MODE == RK4S) {
+                pre0[m] = ldnt(a.ACC + base + e);
+            }
         }
     }
     // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).  The
@@ -1143,9 +1211,20 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
             for (int j = 0; j < GB; j++) kb[j] = (size_t)c.nbr[(size_t)min(n0 + j, nn - 1) * c.nkp + k] * NP;
             double v[GB][PER];
 #pragma unroll
-            for (int j = 0; j < GB; j++)
+            for (int j = 0; j < GB; j++) {
+                if constexpr (kPairs) {
 #pragma unroll
-                for (int m = 0; m < PER; m++) v[j][m] = (double)Yin[kb[j] + eo[m]];
+                    for (int h = 0; h < PER / 2; h++) {
+                        YT v0, v1;
+                        ld2<false>(Yin, kb[j] + eo[2 * h], kb[j] + eo[2 * h + 1], pv2[h], v0, v1);
+                        v[j][2 * h] = (double)v0;
+                        v[j][2 * h + 1] = (double)v1;
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < PER; m++) v[j][m] = (double)Yin[kb[j] + eo[m]];
+                }
+            }
 #pragma unroll
             for (int j = 0; j < GB; j++)
                 if (n0 + j < nn) {
@@ -1288,6 +1367,26 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         }
     }
     if (!store) return;  // after the last barrier
+    if constexpr (kPairs && (MODE == RK1 || MODE == RK2 || MODE == RK3 || MODE == RK4S)) {
+        S *const outs = reinterpret_cast<S *>(a.OUT);  // RK stages 1-3 write the next stage input
+#pragma unroll
+        for (int h = 0; h < PER / 2; h++) {
+            const int m = 2 * h;
+            const double k0 = pv[m] + g[m], k1 = pv[m + 1] + g[m + 1];
+            const size_t o0 = base + eo[m], o1 = base + eo[m + 1];
+            const bool v2 = pv2[h];
+            if constexpr (MODE == RK1) {
+                st2(a.ACC, o0, o1, v2, y[m] + k0 * a.ca, y[m + 1] + k1 * a.ca);
+                st2(outs, o0, o1, v2, (S)(y[m] + k0 * a.cb), (S)(y[m + 1] + k1 * a.cb));
+            } else if constexpr (MODE == RK2 || MODE == RK3) {
+                st2(a.ACC, o0, o1, v2, pre0[m] + k0 * a.ca, pre0[m + 1] + k1 * a.ca);
+                st2(outs, o0, o1, v2, (S)(pre1[m] + k0 * a.cb), (S)(pre1[m + 1] + k1 * a.cb));
+            } else {
+                st2(a.OUT, o0, o1, v2, pre0[m] + k0 * a.ca, pre0[m + 1] + k1 * a.ca);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const double kv = pv[m] + g[m];
