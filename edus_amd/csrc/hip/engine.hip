@@ -1238,11 +1238,31 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         for (int t = lane; t < nmat * MAT; t += TPK) A[t] = CT{0, 0};
     __syncthreads();
     // A <- P (full Hermitian), B <- M = H + E.D.  Element (i, j, comp) writes its
-    // upper value and its mirror; a diagonal element writes (y, 0).
+    // upper value and its mirror; a diagonal element writes (y, 0).  A lane's pair
+    // (re, im) of one upper element writes both as one complex, upper and mirror.
     {
         R *Ad = reinterpret_cast<R *>(A), *Bd = reinterpret_cast<R *>(B);
 #pragma unroll
         for (int m = 0; m < PER; m++) {
+            if (kPairs && (m & 1) == 0 && pv2[m >> 1] && eo[m] >= nb) {
+                const int i = ei[m], j = ej[m];
+                const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
+                const R E[3] = {w ? Eq[0] : Ep[0], w ? Eq[1] : Ep[1], w ? Eq[2] : Ep[2]};
+                R dr = hv[m], di = hv[m + 1], ddi = 0;
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    dr += E[x] * dv[m][x];
+                    di += E[x] * dv[m + 1][x];
+                    ddi += E[x] * dv[m + 1][x];
+                }
+                const R yr = (R)y[m], yi = (R)y[m + 1];
+                A[i * LD + j] = CT{yr, yi};
+                A[j * LD + i] = CT{yr, -yi};
+                B[i * LD + j] = CT{dr, di};
+                B[j * LD + i] = CT{dr, c.dherm ? -di : -hv[m + 1] + ddi};
+                continue;
+            }
+            if (kPairs && (m & 1) == 1 && pv2[m >> 1] && eo[m - 1] >= nb) continue;  // written with m - 1
             const int i = ei[m], j = ej[m], comp = ec[m];
             const bool w = (i >= c.nch && j < c.nch) || (i < c.nch && j >= c.nch);
             const R E[3] = {w ? Eq[0] : Ep[0], w ? Eq[1] : Ep[1], w ? Eq[2] : Ep[2]};  // selects, no pointer
@@ -1281,12 +1301,18 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     // the diagonal this is 2 Im X_ii.
 #pragma unroll
     for (int m = 0; m < PER; m++) {
+        if (kPairs && (m & 1) == 1 && pv2[m >> 1] && eo[m - 1] >= nb) continue;  // formed with m - 1
         const int i = ei[m], j = ej[m];
         const CT xij = C[i * LD + j], xji = C[j * LD + i], pij = A[i * LD + j];
         const double bre = (double)xij.x - (double)xji.x, bim = (double)xij.y + (double)xji.y;
         const double wr = (i < c.nch && j >= c.nch) ? c.wx2 : 0.0;
         const double vr = bim + -wr * (double)pij.y, vi = -bre + wr * (double)pij.x;
-        pv[m] = ec[m] ? vi : vr;
+        if (kPairs && (m & 1) == 0 && pv2[m >> 1] && eo[m] >= nb) {
+            pv[m] = vr;
+            pv[m + 1] = vi;
+        } else {
+            pv[m] = ec[m] ? vi : vr;
+        }
     }
     if (c.diss) {
         __syncthreads();
@@ -1353,6 +1379,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         }
 #pragma unroll
         for (int m = 0; m < PER; m++) {
+            if (kPairs && (m & 1) == 1 && pv2[m >> 1] && eo[m - 1] >= nb) continue;  // formed with m - 1
             const int i = ei[m], j = ej[m];
             const CT w = A[i * LD + j];
             CT hole = CT{0, 0};
@@ -1363,7 +1390,12 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
                 hole.x += (uj.x * ui.x + uj.y * ui.y) * tv;
                 hole.y += (uj.x * ui.y - uj.y * ui.x) * tv;
             }
-            pv[m] = ec[m] ? pv[m] - (double)w.y + (double)hole.y : pv[m] - (double)w.x + (double)hole.x;
+            if (kPairs && (m & 1) == 0 && pv2[m >> 1] && eo[m] >= nb) {
+                pv[m] = pv[m] - (double)w.x + (double)hole.x;
+                pv[m + 1] = pv[m + 1] - (double)w.y + (double)hole.y;
+            } else {
+                pv[m] = ec[m] ? pv[m] - (double)w.y + (double)hole.y : pv[m] - (double)w.x + (double)hole.x;
+            }
         }
     }
     if (!store) return;  // after the last barrier
