@@ -265,6 +265,38 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, size_t o, size_
     }
 }
 
+// stage_update of the Adams-Bashforth mode for two adjacent packed elements
+// (o1 = o0 + 1 when vec): the same operations in the same order, with the
+// history and state accessed as pairs.  (The Taylor mode keeps the single form:
+// its runtime-order loops over two elements' differences cost 60 registers.)
+template <typename S>
+__device__ __forceinline__ void stage_update2_abm(const StageArgs &a, size_t o0, size_t o1, bool vec, size_t h, double kv0,
+                                                  double kv1, double y0, double y1) {
+    S *const hist = reinterpret_cast<S *>(a.HIST);
+    const size_t so = (size_t)a.slot_out * h;
+    st2(hist, so + o0, so + o1, vec, (S)kv0, (S)kv1);
+    double h0[5], h1[5];
+#pragma unroll
+    for (int s = 1; s < 5; s++) {
+        h0[s] = h1[s] = 0.0;
+        if (s < a.order) {
+            const size_t sl = (size_t)a.slots[s] * h;
+            S u0, u1;
+            ld2<true>(hist, sl + o0, sl + o1, vec, u0, u1);
+            h0[s] = (double)u0;
+            h1[s] = (double)u1;
+        }
+    }
+    double d0 = a.coef[0] * kv0, d1 = a.coef[0] * kv1;
+#pragma unroll
+    for (int s = 1; s < 5; s++)
+        if (s < a.order) {
+            d0 += a.coef[s] * h0[s];
+            d1 += a.coef[s] * h1[s];
+        }
+    st2(a.OUT, o0, o1, vec, y0 + a.dt * d0, y1 + a.dt * d1);
+}
+
 // ================================================================ nb <= 4: thread per k
 // DISS: the context has dephasing terms (c.diss); without them the U sandwich
 // and its registers are compiled out.
@@ -1399,7 +1431,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         }
     }
     if (!store) return;  // after the last barrier
-    if constexpr (kPairs && (MODE == RK1 || MODE == RK2 || MODE == RK3 || MODE == RK4S)) {
+    if constexpr (kPairs && MODE != RHSO && MODE != TAY) {
         S *const outs = reinterpret_cast<S *>(a.OUT);  // RK stages 1-3 write the next stage input
 #pragma unroll
         for (int h = 0; h < PER / 2; h++) {
@@ -1407,7 +1439,9 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
             const double k0 = pv[m] + g[m], k1 = pv[m + 1] + g[m + 1];
             const size_t o0 = base + eo[m], o1 = base + eo[m + 1];
             const bool v2 = pv2[h];
-            if constexpr (MODE == RK1) {
+            if constexpr (MODE == ABM) {
+                stage_update2_abm<S>(a, o0, o1, v2, (size_t)c.nk * NP, k0, k1, y[m], y[m + 1]);
+            } else if constexpr (MODE == RK1) {
                 st2(a.ACC, o0, o1, v2, y[m] + k0 * a.ca, y[m + 1] + k1 * a.ca);
                 st2(outs, o0, o1, v2, (S)(y[m] + k0 * a.cb), (S)(y[m + 1] + k1 * a.cb));
             } else if constexpr (MODE == RK2 || MODE == RK3) {
