@@ -297,6 +297,43 @@ __device__ __forceinline__ void stage_update2_abm(const StageArgs &a, size_t o0,
     st2(a.OUT, o0, o1, vec, y0 + a.dt * d0, y1 + a.dt * d1);
 }
 
+// The Taylor tail for two adjacent packed elements: the history is read as
+// pairs, then each element's update is formed on its own (one set of
+// differences live at a time), in the operation order of stage_update.
+template <typename S>
+__device__ __forceinline__ void stage_update2_tay(const StageArgs &a, size_t o0, size_t o1, bool vec, size_t h, double kv0,
+                                                  double kv1, double y0, double y1) {
+    S *const hist = reinterpret_cast<S *>(a.HIST);
+    const size_t so = (size_t)a.slot_out * h;
+    st2(hist, so + o0, so + o1, vec, (S)kv0, (S)kv1);
+    const int n = a.order - 1;
+    S u0[EDUS_MAX_ORDER - 1], u1[EDUS_MAX_ORDER - 1];
+#pragma unroll
+    for (int i = 0; i < EDUS_MAX_ORDER - 1; i++) {
+        u0[i] = u1[i] = (S)0;
+        if (i < n) {
+            const size_t sl = (size_t)a.slots[i + 1] * h;
+            ld2<true>(hist, sl + o0, sl + o1, vec, u0[i], u1[i]);
+        }
+    }
+    double out[2];
+#pragma unroll
+    for (int w = 0; w < 2; w++) {
+        const double kv = w ? kv1 : kv0;
+        double dp[EDUS_MAX_ORDER];
+#pragma unroll
+        for (int i = 0; i < EDUS_MAX_ORDER - 1; i++) dp[i] = (i < n) ? kv - (double)(w ? u1[i] : u0[i]) : 0.0;
+        double delta = kv * a.dt;
+        for (int i = 1; i <= n; i++) {
+            double al = 0.0;
+            for (int q = 0; q < n; q++) al += a.tinv[(i - 1) * n + q] * dp[q];
+            delta += (a.dtp[i] * al) / (double)(i + 1);
+        }
+        out[w] = (w ? y1 : y0) + delta;
+    }
+    st2(a.OUT, o0, o1, vec, out[0], out[1]);
+}
+
 // ================================================================ nb <= 4: thread per k
 // DISS: the context has dephasing terms (c.diss); without them the U sandwich
 // and its registers are compiled out.
@@ -1431,7 +1468,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         }
     }
     if (!store) return;  // after the last barrier
-    if constexpr (kPairs && MODE != RHSO && MODE != TAY) {
+    if constexpr (kPairs && MODE != RHSO) {
         S *const outs = reinterpret_cast<S *>(a.OUT);  // RK stages 1-3 write the next stage input
 #pragma unroll
         for (int h = 0; h < PER / 2; h++) {
@@ -1441,6 +1478,8 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
             const bool v2 = pv2[h];
             if constexpr (MODE == ABM) {
                 stage_update2_abm<S>(a, o0, o1, v2, (size_t)c.nk * NP, k0, k1, y[m], y[m + 1]);
+            } else if constexpr (MODE == TAY) {
+                stage_update2_tay<S>(a, o0, o1, v2, (size_t)c.nk * NP, k0, k1, y[m], y[m + 1]);
             } else if constexpr (MODE == RK1) {
                 st2(a.ACC, o0, o1, v2, y[m] + k0 * a.ca, y[m + 1] + k1 * a.ca);
                 st2(outs, o0, o1, v2, (S)(y[m] + k0 * a.cb), (S)(y[m + 1] + k1 * a.cb));

@@ -190,7 +190,10 @@ int edus_eigen_element(edus_ctx *ctx, int32_t ic, int32_t jc, double *out);
  * files (Observables_MPI.h:286-304, 440-477).  Coulomb contexts only; out [nk][nb]. */
 int edus_renorm_energies(edus_ctx *ctx, double *out);
 
-/* Copies the current density matrix to/from the host, reference layout [nk][nb][nb]. */
+/* Copies the current density matrix to/from the host, reference layout [nk][nb][nb].
+ * Setting the state leaves an Adams-Bashforth or Taylor context's derivative
+ * history as it was: a new trajectory of those steppers starts from a fresh
+ * context, whose history is zero as at the reference's t_start. */
 int edus_get_state(edus_ctx *ctx, double *rho);
 int edus_set_state(edus_ctx *ctx, const double *rho);
 
