@@ -397,7 +397,11 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
 #pragma unroll
         for (int i = 0; i < NB; i++)
 #pragma unroll
-            for (int j = 0; j < NB; j++) U[i][j] = mk(uk[(i * NB + j) * 2], uk[(i * NB + j) * 2 + 1]);
+            for (int j = 0; j < NB; j++) {
+                double re, im;
+                ld2<false>(uk, (i * NB + j) * 2, (i * NB + j) * 2 + 1, true, re, im);
+                U[i][j] = mk(re, im);
+            }
         const cd scr0 = mk(c.screen[0], c.screen[1]), scr1c = mk(c.screen[2], -c.screen[3]);
         int u = 0;
 #pragma unroll
@@ -539,7 +543,11 @@ __device__ __forceinline__ void stage_small(const ConstArgs &c, const StageArgs 
 #pragma unroll
         for (int i = 0; i < NB; i++)
 #pragma unroll
-            for (int j = 0; j < NB; j++) U[i][j] = mk(ldnt(u + (i * NB + j) * 2), ldnt(u + (i * NB + j) * 2 + 1));
+            for (int j = 0; j < NB; j++) {  // one 16-byte load per complex entry (the k's U is 16-byte aligned)
+                double re, im;
+                ld2<true>(u, (i * NB + j) * 2, (i * NB + j) * 2 + 1, true, re, im);
+                U[i][j] = mk(re, im);
+            }
         cd G[NB][NB];
 #pragma unroll
         for (int ia = 0; ia < NB; ia++)
