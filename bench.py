@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_MATRIX_PEAK_TFLOPS = 78.6
 FP32_MATRIX_PEAK_TFLOPS = 157.3
+BF16_MATRIX_PEAK_TFLOPS = 2500.0  # dense bf16 matrix peak (no sparsity)
 METRIC = "k-point·steps/s, FP64 vs mixed/batched, at stated J(t) error; HBM GB/s, MFMA %"
 
 
@@ -56,7 +57,7 @@ def main():
                     help="stage-kernel forms to time, the first one is `value`: fp64 (the drop-in), mixed (FP32 "
                          "operators and products, FP64 state), mixed32 (mixed plus FP32 stage inputs and histories, "
                          "FP64 master state and accumulator); the mixed forms need 5 bands or more")
-    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed32"], default=None,
+    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed32", "bf16x3"], default=None,
                     help="time one form only (same as --forms <form>)")
     ap.add_argument("--stepper", choices=["rk4", "taylor4"], default="rk4",
                     help="wannier workloads: classic RK4 (four stage launches a step) or the Taylor-4 multistep "
@@ -170,7 +171,8 @@ def time_form(s, form, f, g, args):
     clock around a synchronised call; kernel time from the engine's HIP events)."""
     from edus_amd import native as N
 
-    prec = {"fp64": N.PRECISION_FP64, "mixed": N.PRECISION_MIXED, "mixed32": N.PRECISION_MIXED32}[form]
+    prec = {"fp64": N.PRECISION_FP64, "mixed": N.PRECISION_MIXED, "mixed32": N.PRECISION_MIXED32,
+            "bf16x3": N.PRECISION_BF16X3}[form]
     t0 = time.perf_counter()
     eng = N.Engine(s.problem(), precision=prec)
     t_create = time.perf_counter() - t0
@@ -246,8 +248,9 @@ def bench_single(args):
         achieved = bytes_launch / avg_launch_s / 1e9
         if achieved > HBM_PEAK_GBS:
             raise SystemExit(f"bench.py: {form} stage kernel at {achieved:.0f} GB/s above the HBM peak: the byte model is wrong")
-        peak_mx = FP64_MATRIX_PEAK_TFLOPS if form == "fp64" else FP32_MATRIX_PEAK_TFLOPS
-        tflops = m["flop_k"] * nk / avg_launch_s / 1e12
+        peak_mx = {"fp64": FP64_MATRIX_PEAK_TFLOPS, "bf16x3": BF16_MATRIX_PEAK_TFLOPS}.get(form, FP32_MATRIX_PEAK_TFLOPS)
+        # BF16x3: three bf16 matrix products per FP32 one
+        tflops = m["flop_k"] * (3 if form == "bf16x3" else 1) * nk / avg_launch_s / 1e12
         legs[form] = {
             "value": nk * args.steps / t["wall_s"], "ms_per_step": t["wall_s"] * 1e3 / args.steps,
             "device_ms_per_step": t["dev_ms"] / args.steps, "avg_launch_us": round(avg_launch_s * 1e6, 3),
@@ -256,10 +259,11 @@ def bench_single(args):
                          {("strip_width" if nb > 4 else "workgroup_threads"): t["schedule"], "picked_by": "edus_autotune"}),
             "kernel": m["kernel"] + (" (RK4 stages)" if m["stepper"] == "RK4" else f" ({m['stepper']} steps)")
                       + {"fp64": "", "mixed": ", FP32 operators and products",
-                         "mixed32": ", FP32 operators, products and stage storage"}[form],
+                         "mixed32": ", FP32 operators, products and stage storage",
+                         "bf16x3": ", FP32 operators, BF16x3 split products"}[form],
             "bytes_per_launch": bytes_launch, "achieved_gbs": round(achieved, 2),
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "mfma": ({"flop_per_launch": m["flop_k"] * nk, "achieved_tflops": round(tflops, 2),
+            "mfma": ({"flop_per_launch": m["flop_k"] * (3 if form == "bf16x3" else 1) * nk, "achieved_tflops": round(tflops, 2),
                       "peak_tflops": peak_mx, "frac": round(tflops / peak_mx, 4)} if m["flop_k"] else None),
             "survey_bytes_per_launch": m["survey_bytes_k"] * nk,
         }
@@ -290,7 +294,7 @@ def bench_single(args):
         leg = dict(legs[form])
         if "fp64" in legs:
             leg["speedup_vs_fp64"] = round(legs[form]["value"] / legs["fp64"]["value"], 4)
-        prec_of = {"mixed": N.PRECISION_MIXED, "mixed32": N.PRECISION_MIXED32}[form]
+        prec_of = {"mixed": N.PRECISION_MIXED, "mixed32": N.PRECISION_MIXED32, "bf16x3": N.PRECISION_BF16X3}[form]
         leg["error_vs_fp64"] = (mixed_j_error(s, engines[form], it0, args.acc_steps, ref=engines.get("fp64"),
                                               ref_setup=ref_setup, eng_precision=prec_of) if args.acc_steps > 0 else None)
         mixed_legs[form] = leg
@@ -321,7 +325,9 @@ def bench_single(args):
         "forms": {"fp64": "the drop-in: FP64 throughout",
                   "mixed": "FP32 operators and commutator/dephasing products on the matrix cores, FP64 state",
                   "mixed32": "mixed plus FP32 stage inputs and derivative histories; FP64 state at t_n, FP64 "
-                             "accumulator and updates"},
+                             "accumulator and updates",
+                  "bf16x3": "mixed with the products as BF16x3 split products (lo*hi + hi*lo + hi*hi) on the bf16 "
+                            "matrix cores, FP32 accumulation"},
         "data": "synthetic",
         "config": {"workload": wname, "nk": nk, "bands": nb, "stepper": stepper_name(info),
                    "form": head, "launch": "captured step graph" if args.graph else "stream launches per call",

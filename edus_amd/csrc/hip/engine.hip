@@ -1036,21 +1036,26 @@ __global__ __launch_bounds__(256) void k_coul_obs(const double *Hr, const double
 }
 
 // ================================================================ nb in 5..32: one workgroup per k
-// Threads per k: one wavefront for nb <= 16, four for nb <= 32.  The LDS GEMM
-// runs on a GRID x GRID thread grid, each thread owning an RB x RB block of the
-// NBT x NBT result.  OPA: conj(A) if 1.  OPB: 0 -> B[j][l] (result A B^T),
-// 1 -> B[l][j], 2 -> conj(B[l][j]).
+// Threads per k: 32 for nb <= 8 (eight k per workgroup), one wavefront for
+// nb <= 16, four for nb <= 32.  The LDS GEMM of the VALU form (nb <= 8) runs on a
+// GR x GC thread grid, each thread owning an RBR x RBC block of the NBT x NBT
+// result.  OPA: conj(A) if 1.  OPB: 0 -> B[j][l] (result A B^T), 1 -> B[l][j],
+// 2 -> conj(B[l][j]).
 template <int NBT>
 struct WaveGeom {
-    static constexpr int TPK = NBT >= 32 ? 256 : 64;
-    static constexpr int GRID = TPK == 256 ? 16 : 8;
-    static constexpr int RB = NBT / GRID;
+    // threads per k: 256 at 32 bands, 64 at 16, 32 at 8 (two packed elements
+    // each, moved as one 16-byte pair)
+    static constexpr int TPK = NBT >= 32 ? 256 : NBT >= 16 ? 64 : 32;
+    // VALU product lane grid GR x GC, each thread an RBR x RBC block (interleaved)
+    static constexpr int GR = TPK == 256 ? 16 : TPK == 64 ? 8 : 4;
+    static constexpr int GC = TPK == 256 ? 16 : 8;
+    static constexpr int RBR = NBT / GR, RBC = NBT / GC;
     static constexpr int PER = (NBT * NBT + TPK - 1) / TPK;  // packed doubles per thread
     static constexpr int LD = NBT + 1;  // LDS row length in complex elements (one pad)
-    // k per workgroup: at nb <= 8 one k is little work for a wave, so four waves
-    // (four k) share a workgroup and the dispatcher handles a quarter as many
-    static constexpr int KPB = NBT == 8 ? 4 : 1;
-    static constexpr int GATHER = PER <= 1 ? 12 : NBT <= 16 ? 6 : 4;  // stencil neighbours per load batch
+    // k per workgroup: at nb <= 8 one k is little work, so eight k (256 threads)
+    // share a workgroup and the dispatcher handles an eighth as many
+    static constexpr int KPB = NBT == 8 ? 8 : 1;
+    static constexpr int GATHER = NBT <= 8 ? 12 : NBT <= 16 ? 6 : 4;  // stencil neighbours per load batch
 };
 
 // The matrices are zero beyond nb (filled before use, and every product of
@@ -1058,34 +1063,36 @@ struct WaveGeom {
 // trip count NBT; unrolled by 4 (a full unroll hoists every LDS read and
 // costs twice the registers).
 template <int NBT, int OPA, int OPB, typename T>
-__device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
-    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = WaveGeom<NBT>::LD;
-    // Lane (r0, c0) owns rows r0 + i GRID and columns c0 + j GRID (interleaved,
-    // not blocked): the GRID rows one LDS read instruction touches are then
-    // consecutive, and rows NBT+1 complex apart start 4 banks apart, so the
-    // reads are free of bank conflicts.
-    const int r0 = t / GRID, c0 = t % GRID;
+__device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t,
+                                         T (&acc)[WaveGeom<NBT>::RBR][WaveGeom<NBT>::RBC]) {
+    constexpr int RBR = WaveGeom<NBT>::RBR, RBC = WaveGeom<NBT>::RBC, GR = WaveGeom<NBT>::GR, GC = WaveGeom<NBT>::GC;
+    constexpr int LD = WaveGeom<NBT>::LD;
+    // Lane (r0, c0) owns rows r0 + i GR and columns c0 + j GC (interleaved, not
+    // blocked): the rows one LDS read instruction touches are then consecutive,
+    // and rows NBT+1 complex apart start 4 banks apart, so the reads are free of
+    // bank conflicts.
+    const int r0 = t / GC, c0 = t % GC;
 #pragma unroll
-    for (int i = 0; i < RB; i++)
+    for (int i = 0; i < RBR; i++)
 #pragma unroll
-        for (int j = 0; j < RB; j++) acc[i][j] = T{0, 0};
+        for (int j = 0; j < RBC; j++) acc[i][j] = T{0, 0};
 #pragma unroll 4
     for (int l = 0; l < NBT; l++) {
-        T av[RB], bv[RB];
+        T av[RBR], bv[RBC];
 #pragma unroll
-        for (int i = 0; i < RB; i++) {
-            T v = A[(r0 + i * GRID) * LD + l];
+        for (int i = 0; i < RBR; i++) {
+            T v = A[(r0 + i * GR) * LD + l];
             av[i] = OPA ? T{v.x, -v.y} : v;
         }
 #pragma unroll
-        for (int j = 0; j < RB; j++) {
-            T v = (OPB == 0) ? B[(c0 + j * GRID) * LD + l] : B[l * LD + c0 + j * GRID];
+        for (int j = 0; j < RBC; j++) {
+            T v = (OPB == 0) ? B[(c0 + j * GC) * LD + l] : B[l * LD + c0 + j * GC];
             bv[j] = (OPB == 2) ? T{v.x, -v.y} : v;
         }
 #pragma unroll
-        for (int i = 0; i < RB; i++)
+        for (int i = 0; i < RBR; i++)
 #pragma unroll
-            for (int j = 0; j < RB; j++) {
+            for (int j = 0; j < RBC; j++) {
                 acc[i][j].x += av[i].x * bv[j].x - av[i].y * bv[j].y;
                 acc[i][j].y += av[i].x * bv[j].y + av[i].y * bv[j].x;
             }
@@ -1093,13 +1100,14 @@ __device__ __forceinline__ void gemm_lds(const T *A, const T *B, int t, T (&acc)
 }
 
 template <int NBT, typename T>
-__device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom<NBT>::RB][WaveGeom<NBT>::RB]) {
-    constexpr int RB = WaveGeom<NBT>::RB, GRID = WaveGeom<NBT>::GRID, LD = WaveGeom<NBT>::LD;
-    const int r0 = t / GRID, c0 = t % GRID;  // rows r0 + i GRID, columns c0 + j GRID
+__device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom<NBT>::RBR][WaveGeom<NBT>::RBC]) {
+    constexpr int RBR = WaveGeom<NBT>::RBR, RBC = WaveGeom<NBT>::RBC, GR = WaveGeom<NBT>::GR, GC = WaveGeom<NBT>::GC;
+    constexpr int LD = WaveGeom<NBT>::LD;
+    const int r0 = t / GC, c0 = t % GC;  // rows r0 + i GR, columns c0 + j GC
 #pragma unroll
-    for (int i = 0; i < RB; i++)
+    for (int i = 0; i < RBR; i++)
 #pragma unroll
-        for (int j = 0; j < RB; j++) C[(r0 + i * GRID) * LD + c0 + j * GRID] = acc[i][j];
+        for (int j = 0; j < RBC; j++) C[(r0 + i * GR) * LD + c0 + j * GC] = acc[i][j];
 }
 
 // The same products on the matrix cores (NBT = 16: the k's one wave computes the
@@ -1160,6 +1168,74 @@ __device__ __forceinline__ void store_mfma(T *C, int t, const V &cre, const V &c
     for (int i = 0; i < 4; i++) C[(r0 + mfma_row<R>(lane, i)) * LD + c0 + (lane & 15)] = T{cre[i], cim[i]};
 }
 
+// BF16x3 split products on the bf16 matrix cores: each FP32 operand x is split
+// into hi = bf16(x) and lo = bf16(x - hi) and every real product is taken as
+// lo*hi + hi*lo + hi*hi with FP32 accumulation (v_mfma_f32_16x16x32_bf16: lane l
+// feeds op(A)[row l%16][8 (l/16) + q] and op(B)[8 (l/16) + q][col l%16], q < 8;
+// the accumulator layout is the FP32 16x16x4 one).  The same operands, result
+// and tile assignment as gemm_mfma<..., float>; K runs in slices of 32 (zero
+// beyond NBT).
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void bf16_split(float x, __bf16 &hi, __bf16 &lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+__device__ __forceinline__ v4f prod3(const v8bf &xh, const v8bf &xl, const v8bf &yh, const v8bf &yl, v4f acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, yh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, yl, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, yh, acc, 0, 0, 0);
+}
+
+template <int NBT, int OPA, int OPB>
+__device__ __forceinline__ void gemm_bf16x3(const cf *A, const cf *B, int t, v4f &cre, v4f &cim) {
+    constexpr int LD = WaveGeom<NBT>::LD;
+    int r0, c0;
+    mfma_tile<NBT>(t, r0, c0);
+    const int lane = t & 63, ra = r0 + (lane & 15), cb = c0 + (lane & 15), kq = 8 * (lane >> 4);
+    cre = v4f{0.f, 0.f, 0.f, 0.f};
+    cim = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < NBT; k0 += 32) {
+        v8bf arh, arl, aih, ail, anh, anl, brh, brl, bih, bil;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int l = k0 + kq + q;
+            cf a = cf{0.f, 0.f}, b = cf{0.f, 0.f};
+            if (l < NBT) {
+                a = A[ra * LD + l];
+                b = (OPB == 0) ? B[cb * LD + l] : B[l * LD + cb];
+            }
+            if (OPA) a.y = -a.y;
+            if (OPB == 2) b.y = -b.y;
+            __bf16 h, lo;
+            bf16_split(a.x, h, lo); arh[q] = h; arl[q] = lo;
+            bf16_split(a.y, h, lo); aih[q] = h; ail[q] = lo;
+            bf16_split(-a.y, h, lo); anh[q] = h; anl[q] = lo;
+            bf16_split(b.x, h, lo); brh[q] = h; brl[q] = lo;
+            bf16_split(b.y, h, lo); bih[q] = h; bil[q] = lo;
+        }
+        // Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br
+        cre = prod3(arh, arl, brh, brl, cre);
+        cre = prod3(anh, anl, bih, bil, cre);
+        cim = prod3(arh, arl, bih, bil, cim);
+        cim = prod3(aih, ail, brh, brl, cim);
+    }
+}
+
+// The stage kernels' products: native MFMA of R, or BF16x3 (B3, R = float).
+template <int NBT, int OPA, int OPB, typename R, bool B3>
+__device__ __forceinline__ void gemm_prod(const typename CplxOf<R>::T *A, const typename CplxOf<R>::T *B, int t,
+                                          typename MfmaAcc<R>::T &cre, typename MfmaAcc<R>::T &cim) {
+    if constexpr (B3) {
+        static_assert(std::is_same<R, float>::value, "BF16x3 products take FP32 operands");
+        gemm_bf16x3<NBT, OPA, OPB>(A, B, t, cre, cim);
+    } else {
+        gemm_mfma<NBT, OPA, OPB, R>(A, B, t, cre, cim);
+    }
+}
+
 // Row and column of accumulator element i of thread t (for the dephasing rates).
 template <int NBT, typename R>
 __device__ __forceinline__ void mfma_elem(int t, int i, int &row, int &col) {
@@ -1193,7 +1269,7 @@ __host__ __device__ constexpr int wave_lds_doubles(int diss) {
 // S: storage type of the RK stage inputs/outputs (YA, YB) and of the multistep
 // histories: double, or float in the mixed form with FP32 stage storage, where
 // the state at t_n (P0), the RK accumulator and every update stay FP64.
-template <int NBT, int MODE, typename R, typename S = double>
+template <int NBT, int MODE, typename R, typename S = double, bool B3 = false>
 __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &a, const int k, const int lane, double *smem,
                                            const bool store) {
     using CT = typename CplxOf<R>::T;
@@ -1201,7 +1277,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     constexpr bool kInS = std::is_same<S, float>::value && (MODE == RK2 || MODE == RK3 || MODE == RK4S);
     using YT = typename std::conditional<kInS, S, double>::type;
     const YT *const Yin = reinterpret_cast<const YT *>(a.Y);
-    constexpr int TPK = WaveGeom<NBT>::TPK, RB = WaveGeom<NBT>::RB, PER = WaveGeom<NBT>::PER;
+    constexpr int TPK = WaveGeom<NBT>::TPK, RBR = WaveGeom<NBT>::RBR, RBC = WaveGeom<NBT>::RBC, PER = WaveGeom<NBT>::PER;
     constexpr int LD = WaveGeom<NBT>::LD, MAT = NBT * LD;
     const int nb = c.nb, NP = nb * nb, nmat = wave_mats(c.diss);
     constexpr bool kMfma = NBT >= 16;  // 16x16 matrix-core tiles; NBT = 8 stays on the VALU
@@ -1364,11 +1440,11 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     // X = P M^T; -i[M,rho] (transposed storage) = -i (X - X^+)
     if constexpr (kMfma) {
         typename MfmaAcc<R>::T cre, cim;
-        gemm_mfma<NBT, 0, 0, R>(A, B, lane, cre, cim);
+        gemm_prod<NBT, 0, 0, R, B3>(A, B, lane, cre, cim);
         if (C == B) __syncthreads();  // every lane has finished reading M
         store_mfma<NBT, R>(C, lane, cre, cim);
     } else {
-        CT acc[RB][RB];
+        CT acc[RBR][RBC];
         gemm_lds<NBT, 0, 0>(A, B, lane, acc);
         if (C == B) __syncthreads();  // every lane has finished reading M
         store_block<NBT>(C, lane, acc);
@@ -1399,11 +1475,11 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         __syncthreads();
         if constexpr (kMfma) {
             typename MfmaAcc<R>::T cre, cim;
-            gemm_mfma<NBT, 0, 0, R>(A, C, lane, cre, cim);     // T1 = P U^T
+            gemm_prod<NBT, 0, 0, R, B3>(A, C, lane, cre, cim);     // T1 = P U^T
             __syncthreads();
             store_mfma<NBT, R>(B, lane, cre, cim);
             __syncthreads();
-            gemm_mfma<NBT, 1, 1, R>(C, B, lane, cre, cim);     // G = conj(U) T1
+            gemm_prod<NBT, 1, 1, R, B3>(C, B, lane, cre, cim);     // G = conj(U) T1
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 int ia, ib;
@@ -1415,29 +1491,29 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
             __syncthreads();
             store_mfma<NBT, R>(A, lane, cre, cim);               // A <- K = T o G
             __syncthreads();
-            gemm_mfma<NBT, 0, 2, R>(A, C, lane, cre, cim);     // T2 = K conj(U)
+            gemm_prod<NBT, 0, 2, R, B3>(A, C, lane, cre, cim);     // T2 = K conj(U)
             __syncthreads();
             store_mfma<NBT, R>(B, lane, cre, cim);
             __syncthreads();
-            gemm_mfma<NBT, 1, 1, R>(C, B, lane, cre, cim);     // W = conj(U) T2
+            gemm_prod<NBT, 1, 1, R, B3>(C, B, lane, cre, cim);     // W = conj(U) T2
             __syncthreads();
             store_mfma<NBT, R>(A, lane, cre, cim);
             __syncthreads();
         } else {
-            CT acc[RB][RB];
+            CT acc[RBR][RBC];
             gemm_lds<NBT, 0, 0>(A, C, lane, acc);     // T1 = P U^T
             __syncthreads();
             store_block<NBT>(B, lane, acc);
             __syncthreads();
             gemm_lds<NBT, 1, 1>(C, B, lane, acc);     // G = conj(U) T1
             {
-                constexpr int GRID = WaveGeom<NBT>::GRID;
-                const int r0 = lane / GRID, c0 = lane % GRID;
+                constexpr int GR = WaveGeom<NBT>::GR, GC = WaveGeom<NBT>::GC;
+                const int r0 = lane / GC, c0 = lane % GC;
     #pragma unroll
-                for (int i = 0; i < RB; i++)
+                for (int i = 0; i < RBR; i++)
     #pragma unroll
-                    for (int j = 0; j < RB; j++) {
-                        const int ia = r0 + i * GRID, ib = c0 + j * GRID;
+                    for (int j = 0; j < RBC; j++) {
+                        const int ia = r0 + i * GR, ib = c0 + j * GC;
                         const R t = (ia < nb && ib < nb) ? (R)c.T[ia * nb + ib] : (R)0;
                         acc[i][j] = CT{acc[i][j].x * t, acc[i][j].y * t};
                     }
@@ -1519,17 +1595,17 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     }
 }
 
-template <int NBT, int MODE, typename R, typename S = double>
+template <int NBT, int MODE, typename R, typename S = double, bool B3 = false>
 __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_stage_wave(ConstArgs c, StageArgs a) {
     extern __shared__ double smem_d[];
     constexpr int TPK = WaveGeom<NBT>::TPK;
     if constexpr (WaveGeom<NBT>::KPB == 1) {
         // xcd_block and strip_k: bijections of [0, nk)
-        stage_wave<NBT, MODE, R, S>(c, a, strip_k(xcd_block(), c.nk, c.sw, c.scol), threadIdx.x, smem_d, true);
+        stage_wave<NBT, MODE, R, S, B3>(c, a, strip_k(xcd_block(), c.nk, c.sw, c.scol), threadIdx.x, smem_d, true);
     } else {
         const int sub = threadIdx.x / TPK;
         const int kk = xcd_block() * WaveGeom<NBT>::KPB + sub;
-        stage_wave<NBT, MODE, R, S>(c, a, strip_k(min(kk, c.nk - 1), c.nk, c.sw, c.scol), threadIdx.x % TPK,
+        stage_wave<NBT, MODE, R, S, B3>(c, a, strip_k(min(kk, c.nk - 1), c.nk, c.sw, c.scol), threadIdx.x % TPK,
                                  smem_d + sub * wave_lds_doubles<NBT, R>(c.diss), kk < c.nk);
     }
 }
@@ -2067,13 +2143,7 @@ __global__ __launch_bounds__(256) void k_wannier_mfma32(const cd *__restrict__ P
 // k_wannier_mfma32 with R consumed 32 at a time (one K slice of the
 // instruction); lane l supplies A[row l%16][8 (l/16) .. +7] and B[same k][col l%16]
 // and holds D[4 (l/16) + i][l%16].
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 constexpr int WBR = 32;
-
-__device__ __forceinline__ void bf16_split(float x, __bf16 &hi, __bf16 &lo) {
-    hi = (__bf16)x;
-    lo = (__bf16)(x - (float)hi);
-}
 
 __global__ __launch_bounds__(256) void k_wannier_mfma_bf16x3(const cd *__restrict__ PT, const cd *__restrict__ Val, int N0, int N1,
                                                              int N2, int nR, int nm, int nk, int kbase, cd *__restrict__ Out) {
@@ -2541,6 +2611,12 @@ int launch_stage(edus_ctx *c, StageArgs a) {
             const dim3 grid((c->nk + KPB - 1) / KPB), block(WaveGeom<NBT>::TPK * KPB);
             const size_t lds_f = (size_t)KPB * wave_lds_doubles<NBT, float>(c->diss) * 8;
             const size_t lds_d = (size_t)KPB * wave_lds_doubles<NBT, double>(c->diss) * 8;
+            if constexpr (NBT >= 16) {
+                if (c->precision == EDUS_PRECISION_BF16X3) {
+                    hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float, double, true>), grid, block, lds_f, c->stream, ca, a);
+                    return;
+                }
+            }
             if (c->precision == EDUS_PRECISION_MIXED32)
                 hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float, float>), grid, block, lds_f, c->stream, ca, a);
             else if (c->precision == EDUS_PRECISION_MIXED)
@@ -2679,10 +2755,13 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
     c->stepper = p->stepper; c->order = (p->stepper == EDUS_STEPPER_RK4) ? 4 : p->order;
     c->soa = nb <= 4;
     c->precision = p->precision;
-    if (c->precision != EDUS_PRECISION_FP64 && c->precision != EDUS_PRECISION_MIXED && c->precision != EDUS_PRECISION_MIXED32)
-        return fail(set_err(EDUS_ERR_INVALID, "precision must be EDUS_PRECISION_FP64, _MIXED or _MIXED32"));
+    if (c->precision != EDUS_PRECISION_FP64 && c->precision != EDUS_PRECISION_MIXED && c->precision != EDUS_PRECISION_MIXED32 &&
+        c->precision != EDUS_PRECISION_BF16X3)
+        return fail(set_err(EDUS_ERR_INVALID, "precision must be EDUS_PRECISION_FP64, _MIXED, _MIXED32 or _BF16X3"));
     if (c->precision != EDUS_PRECISION_FP64 && c->soa)
         return fail(set_err(EDUS_ERR_UNSUPPORTED, "mixed precision is implemented for nb >= 5"));
+    if (c->precision == EDUS_PRECISION_BF16X3 && nb <= 8)
+        return fail(set_err(EDUS_ERR_UNSUPPORTED, "EDUS_PRECISION_BF16X3 runs on the matrix-core tiles of nb >= 9"));
     c->nkp = c->soa ? ((p->nk + 255) / 256) * 256 : p->nk;
     // Wave kernels: walk the grid in strips (strip_k) when a strip's two previous
     // rows of packed state, the stencil's reach, fit in kStripL2Bytes while whole
@@ -3458,7 +3537,7 @@ int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch
         if (!problems[i]) return set_err(EDUS_ERR_INVALID, "null problem in batch");
         if (problems[i]->stepper != EDUS_STEPPER_RK4) return set_err(EDUS_ERR_UNSUPPORTED, "batches run RK4 problems");
         if (problems[i]->coulomb) return set_err(EDUS_ERR_UNSUPPORTED, "batches do not carry the Coulomb term");
-        if (problems[i]->precision == EDUS_PRECISION_MIXED32)
+        if (problems[i]->precision == EDUS_PRECISION_MIXED32 || problems[i]->precision == EDUS_PRECISION_BF16X3)
             return set_err(EDUS_ERR_UNSUPPORTED, "batches run FP64 or EDUS_PRECISION_MIXED members");
     }
     auto *b = new edus_batch();

@@ -14,7 +14,7 @@ import numpy as np
 LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 
 STEPPER_RK4, STEPPER_AB, STEPPER_TAYLOR = 0, 1, 2
-PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED32 = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED32, PRECISION_BF16X3 = 0, 1, 2, 3
 FIELDS_PER_EVAL = 6
 
 _dp = C.POINTER(C.c_double)

@@ -51,6 +51,14 @@ extern "C" {
  * t_n, the RK4 accumulator and every update stay FP64 (the master copy).  Band
  * counts 5..32; not for batches.  No reference counterpart. */
 #define EDUS_PRECISION_MIXED32 2
+/* BF16X3: MIXED with the commutator and dephasing products taken as BF16x3 split
+ * products on the bf16 matrix cores: each FP32 operand x is split into
+ * hi = bf16(x) and lo = bf16(x - hi), every real product is lo*hi + hi*lo + hi*hi
+ * with FP32 accumulation (a product carries at most about 3 2^-16 of relative
+ * error, against 2^-24 per FP32 multiply).  State, stencil drift and increments
+ * stay FP64.  Band counts 9..32 (the matrix-core tiles); not for batches.  No
+ * reference counterpart. */
+#define EDUS_PRECISION_BF16X3 3
 
 #define EDUS_MAX_NB          32  /* largest band count with a kernel in this build */
 #define EDUS_MAX_ORDER        8  /* largest Adams-Bashforth / Taylor order */
