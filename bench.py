@@ -14,7 +14,7 @@ inputs and histories, FP64 state at t_n and accumulator), each mixed form with
 its J(t), final conduction population and harmonic-intensity errors against the
 FP64 path over a window of steps at the pulse peak.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload ...] [--forms fp64,mixed,mixed32] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload ...] [--forms fp64,mixed,mixed32,bf16x3] [--no-cpu]
 
 Prints one JSON line.
 """
@@ -53,10 +53,11 @@ def main():
     ap.add_argument("--nw", type=int, default=16)
     ap.add_argument("--rmax", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,mixed,mixed32",
+    ap.add_argument("--forms", default="fp64,mixed,mixed32,bf16x3",
                     help="stage-kernel forms to time, the first one is `value`: fp64 (the drop-in), mixed (FP32 "
                          "operators and products, FP64 state), mixed32 (mixed plus FP32 stage inputs and histories, "
-                         "FP64 master state and accumulator); the mixed forms need 5 bands or more")
+                         "FP64 master state and accumulator), bf16x3 (mixed with BF16x3 split products); the mixed "
+                         "forms need 5 bands or more, bf16x3 9 or more")
     ap.add_argument("--precision", choices=["fp64", "mixed", "mixed32", "bf16x3"], default=None,
                     help="time one form only (same as --forms <form>)")
     ap.add_argument("--stepper", choices=["rk4", "taylor4"], default="rk4",
@@ -213,6 +214,8 @@ def bench_single(args):
     nk, nb = info["nk"], info["nb"]
     if nb < 5:
         forms = [x for x in forms if x == "fp64"] or ["fp64"]
+    elif nb <= 8:  # the BF16x3 products run on the matrix-core tiles of nb >= 9
+        forms = [x for x in forms if x != "bf16x3"] or ["fp64"]
     if args.end_to_end:
         s.run(os.path.join(wd, "warm"), max_steps=args.warmup)
         t0 = time.perf_counter()
