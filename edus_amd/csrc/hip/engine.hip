@@ -1691,48 +1691,62 @@ __device__ __forceinline__ cd load_p(const ObsArgs &o, int k, int i, int j) {
     return (i < j) ? v : cconj(v);
 }
 
-// One wave per workgroup; the wave walks its k points and stages each k's full
-// P (from the packed state) and U in its LDS, so the population quadratic form
-// and the current / absorption sums read their matrix operands from LDS.
+// One workgroup of T threads walks its k points and stages each k's full P (from
+// the packed state) and U in its LDS, so the population quadratic form and the
+// current / absorption sums read their matrix operands from LDS.  T = 256 for
+// large band counts: the k's loads and sums are spread over four waves, and the
+// population form of band c over S = T / 32 lanes (terms ii = s, s + S, ...,
+// summed by shuffles), so no lane walks a long chain of dependent loads.
 constexpr int kObsMaxBlocks = 8192;
 
-__global__ __launch_bounds__(64) void k_observables(ObsArgs o) {
+template <int T>
+__global__ __launch_bounds__(T) void k_observables(ObsArgs o) {
     extern __shared__ double obs_lds[];
-    const int lane = threadIdx.x;
+    constexpr int S = T / 32, NW = T / 64;
+    const int lane = threadIdx.x, wave = lane >> 6;
     const int nb = o.nb, NP = nb * nb, NN = nb * nb, nch = o.nch;
-    // U rows padded to nb + 1: lane c reads row c, so consecutive lanes start
-    // 4 banks apart instead of on the same bank
+    // U rows padded to nb + 1: lanes of consecutive bands start 4 banks apart
     const int LU = nb + 1;
     cd *Pl = reinterpret_cast<cd *>(obs_lds), *Ul = Pl + NN;
+    double *red = reinterpret_cast<double *>(Ul + nb * LU);  // [NW][17] cross-wave partials, then [32] populations
     Kahan pop, j1[3], j2[6], ta[6];
     double pmin = 1.0, pmax = 0.0;
+    const int pc = lane / S, ps = lane % S;  // band and term slice of the population form
     for (int k = blockIdx.x; k < o.nk; k += gridDim.x) {
         __syncthreads();  // the previous k's LDS reads are done
-        for (int t = lane; t < NN; t += 64) {
+        for (int t = lane; t < NN; t += T) {
             Pl[t] = load_p(o, k, t / nb, t % nb);
-            const double *u = o.U + ((size_t)k * NN + t) * 2;
-            Ul[(t / nb) * LU + t % nb] = mk(u[0], u[1]);
+            double ur, ui;
+            ld2<false>(o.U, ((size_t)k * NN + t) * 2, ((size_t)k * NN + t) * 2 + 1, true, ur, ui);
+            Ul[(t / nb) * LU + t % nb] = mk(ur, ui);
         }
         __syncthreads();
         // population of band c = Re (conj(U) P_sym U^T)_cc (InitParameters.h:264-335)
-        if (lane < nb) {
-            const cd *uc = Ul + lane * LU;
-            cd s = mk(0.0, 0.0);
-            for (int ii = 0; ii < nb; ii++) {
-                cd t = mk(0.0, 0.0);
+        {
+            double sre = 0.0;
+            if (pc < nb) {
+                const cd *uc = Ul + pc * LU;
+                cd sacc = mk(0.0, 0.0);
+                for (int ii = ps; ii < nb; ii += S) {
+                    cd t = mk(0.0, 0.0);
 #pragma unroll 4
-                for (int jj = 0; jj < nb; jj++) t = cadd(t, cmul(Pl[ii * nb + jj], uc[jj]));
-                s = cadd(s, cmul(cconj(uc[ii]), t));
+                    for (int jj = 0; jj < nb; jj++) t = cadd(t, cmul(Pl[ii * nb + jj], uc[jj]));
+                    sacc = cadd(sacc, cmul(cconj(uc[ii]), t));
+                }
+                sre = sacc.x;
             }
-            const double pik = s.x;
-            pop.add(pik);
-            const double hv = (lane < o.ncc) ? 1. - pik : pik;
-            pmax = fmax(pmax, fabs(hv));
-            pmin = fmin(pmin, hv);
+#pragma unroll
+            for (int off = S / 2; off > 0; off >>= 1) sre += __shfl_xor(sre, off, 64);
+            if (pc < nb && ps == 0) {
+                pop.add(sre);
+                const double hv = (pc < o.ncc) ? 1. - sre : sre;
+                pmax = fmax(pmax, fabs(hv));
+                pmin = fmin(pmin, hv);
+            }
         }
         // Current1: sum_ij P_ij gradH_ij (real: P and gradH are Hermitian)
         if (o.GE) {
-            for (int e = lane; e < NP; e += 64) {
+            for (int e = lane; e < NP; e += T) {
                 const double pe = o.P[k * o.sk + e * o.se];
                 const double wgt = (e < nb) ? 1.0 : 2.0;   // a pair carries (i,j) and (j,i)
                 // Re(P_ij G_ij): re*re - im*im over the (re, im) doubles of the pair
@@ -1741,18 +1755,19 @@ __global__ __launch_bounds__(64) void k_observables(ObsArgs o) {
             }
         }
         // Current2: sum_ij P_ij (H D^+)_ij
-        for (int t = lane; t < NN; t += 64) {
+        for (int t = lane; t < NN; t += T) {
             const cd pij = Pl[t];
             for (int x = 0; x < 3; x++) {
-                const double *q = o.Q + (((size_t)k * 3 + x) * NN + t) * 2;
-                cd v = cmul(pij, mk(q[0], q[1]));
+                double qr, qi;
+                ld2<false>(o.Q, (((size_t)k * 3 + x) * NN + t) * 2, (((size_t)k * 3 + x) * NN + t) * 2 + 1, true, qr, qi);
+                cd v = cmul(pij, mk(qr, qi));
                 j2[2 * x].add(v.x);
                 j2[2 * x + 1].add(v.y);
             }
         }
         // Transient absorption: -2 sum_{i<nch<=j} D_ij P_ij (Observables_MPI.h:559-569)
         const int nblk = nch * (nb - nch);
-        for (int t = lane; t < nblk; t += 64) {
+        for (int t = lane; t < nblk; t += T) {
             const int i = t / (nb - nch), j = nch + t % (nb - nch);
             const cd pij = Pl[i * nb + j];
             for (int x = 0; x < 3; x++) {
@@ -1763,23 +1778,33 @@ __global__ __launch_bounds__(64) void k_observables(ObsArgs o) {
             }
         }
     }
-    // Lane sums by shuffle, in a fixed order.
-    const int nv = nb + 15;
-    double *out = o.part + (size_t)blockIdx.x * (nv + 2);
-    for (int c = 0; c < nb; c++) {
-        const double v = wave_sum((lane == c) ? pop.s : 0.0);
-        if (lane == 0) out[c] = v;
-    }
+    // Lane sums by shuffle, then across the waves in wave order, in a fixed order.
+    __syncthreads();
+    double *pw = red + wave * 17;
     for (int x = 0; x < 3; x++) {
         const double v = wave_sum(j1[x].s);
-        if (lane == 0) out[nb + x] = v;
+        if ((lane & 63) == 0) pw[x] = v;
     }
     for (int x = 0; x < 6; x++) {
         const double v = wave_sum(j2[x].s), w = wave_sum(ta[x].s);
-        if (lane == 0) { out[nb + 3 + x] = v; out[nb + 9 + x] = w; }
+        if ((lane & 63) == 0) { pw[3 + x] = v; pw[9 + x] = w; }
     }
     const double mn = wave_min(pmin), mx = wave_max(pmax);
-    if (lane == 0) { out[nv] = mn; out[nv + 1] = mx; }
+    if ((lane & 63) == 0) { pw[15] = mn; pw[16] = mx; }
+    double *pops = red + NW * 17;
+    if (pc < nb && ps == 0) pops[pc] = pop.s;
+    __syncthreads();
+    const int nv = nb + 15;
+    double *out = o.part + (size_t)blockIdx.x * (nv + 2);
+    if (lane < nb) out[lane] = pops[lane];
+    if (lane < 17) {
+        double v = red[lane];
+        for (int w = 1; w < NW; w++) {
+            const double u = red[w * 17 + lane];
+            v = lane == 15 ? fmin(v, u) : lane == 16 ? fmax(v, u) : v + u;
+        }
+        out[nb + lane] = v;  // nb+0..2 J1, +3..8 J2, +9..14 TAbs, nv and nv+1 min and max
+    }
 }
 
 // Final reduction of the per-block partials: one workgroup per value.  Each
@@ -3298,7 +3323,11 @@ int edus_observables(edus_ctx *c, double *pop, double *j1, double *j2, double *t
     o.nk = c->nk; o.nb = c->nb; o.nch = c->nch; o.ncc = c->ncc;
     o.sk = c->soa ? 1 : c->np;
     o.se = c->soa ? c->nkp : 1;
-    hipLaunchKernelGGL(k_observables, dim3(c->obs_blocks), dim3(64), (size_t)c->nb * (2 * c->nb + 1) * sizeof(cd), c->stream, o);
+    const size_t obs_lds = (size_t)c->nb * (2 * c->nb + 1) * sizeof(cd) + (4 * 17 + 32) * sizeof(double);
+    if (c->nb > 8)
+        hipLaunchKernelGGL(k_observables<256>, dim3(c->obs_blocks), dim3(256), obs_lds, c->stream, o);
+    else
+        hipLaunchKernelGGL(k_observables<64>, dim3(c->obs_blocks), dim3(64), obs_lds, c->stream, o);
     HIP_TRY(hipGetLastError());
     const int nv = c->nb + 15;
     hipLaunchKernelGGL(k_obs_final, dim3(nv + 2), dim3(256), 0, c->stream, (const double *)c->part, c->obs_blocks, nv, c->obs);
