@@ -68,6 +68,14 @@ struct ConstArgs {
     const int *eidx;     // packed element e -> i | j << 8 | comp << 16 (wave layout)
     int nk, nkp, nb, nch, ncc, n_nbr, diss, dherm;  // dherm: D_ji = conj(D_ij) (else D_ji = D_ij)
     int sw, scol;        // wave kernels: strip width and row length of the workgroup order (sw = 0: k order)
+    // Wave kernels of one k per workgroup: pair = 1 when every pair of k (2q, 2q+1)
+    // has the same stencil overlap (a uniform grid of even length along the
+    // fastest axis); a workgroup then serves both and loads each shared
+    // neighbour once.  pat[n] (n < n_nbr): the slot of 2q+1's stencil that is
+    // 2q's slot n, or -1; pat[kMaxNbr + t] (t < npnew): 2q+1's slots found in
+    // neither 2q's stencil nor 2q itself; pself: 2q+1's slot holding 2q, or -1.
+    int pair, pself, npnew;
+    const int *pat;
     double wx2;
     // Coulomb mean field (small-nb kernel only): the mean-field matrix per k,
     // peak correction against PW0, renormalised H out.
@@ -1266,12 +1274,17 @@ __host__ __device__ constexpr int wave_lds_doubles(int diss) {
 // the group; smem: the group's LDS; store: false for a group past the end of the
 // grid, which computes its clamped k again but writes nothing (some stage modes
 // update their accumulator in place).
+// ROLE 0: k alone.  A stencil pair (k, kb = k + 1, see ConstArgs::pair) is served
+// by two calls: ROLE 1 steps k and forms kb's stencil sum in gb, ROLE 2 steps kb
+// with that sum.  The sum of k runs in its own slot order (as alone); the one of
+// kb adds the neighbours it shares with k in k's slot order, then k itself, then
+// its other neighbours in its own slot order.
 // S: storage type of the RK stage inputs/outputs (YA, YB) and of the multistep
 // histories: double, or float in the mixed form with FP32 stage storage, where
 // the state at t_n (P0), the RK accumulator and every update stay FP64.
-template <int NBT, int MODE, typename R, typename S = double, bool B3 = false>
-__device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &a, const int k, const int lane, double *smem,
-                                           const bool store) {
+template <int NBT, int MODE, typename R, typename S = double, bool B3 = false, int ROLE = 0>
+__device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &a, const int k, const int kb, const int lane,
+                                           double *smem, const bool store, double (&gb)[WaveGeom<NBT>::PER]) {
     using CT = typename CplxOf<R>::T;
     // stage input in S for the RK stages 2-4 (they read YA / YB); the FP64 state otherwise
     constexpr bool kInS = std::is_same<S, float>::value && (MODE == RK2 || MODE == RK3 || MODE == RK4S);
@@ -1287,7 +1300,6 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     const R *Uop = reinterpret_cast<const R *>(sizeof(R) == 8 ? (const void *)c.U : (const void *)c.Uf);
     const double *tab = stage_row(a);
     const R Ep[3] = {(R)tab[0], (R)tab[1], (R)tab[2]}, Eq[3] = {(R)tab[3], (R)tab[4], (R)tab[5]};
-    const size_t base = (size_t)k * NP;
     const int nn = c.n_nbr;
     const bool grad = tab[6 + nn] != 0.0;
 
@@ -1299,10 +1311,15 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     // when nb is even (NP even keeps every pair inside one k and aligned); the
     // pair is a real diagonal pair or the (re, im) of one upper element.
     constexpr bool kPairs = PER % 2 == 0;
+    // A stencil pair's calls load the RK2/RK3 accumulator and state at t_n after
+    // the products, not with the other inputs: the registers then hold kb's
+    // stencil sum instead through the gather and the products.
+    constexpr bool kLatePre = ROLE != 0 && kPairs;
     bool pv2[PER > 1 ? PER / 2 : 1];
+    int eo[PER], ei[PER], ej[PER], ec[PER];
     double y[PER], pv[PER], g[PER], pre0[PER], pre1[PER];
     R hv[PER], dv[PER][3];
-    int eo[PER], ei[PER], ej[PER], ec[PER];
+    const size_t base = (size_t)k * NP;
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const int e = kPairs ? min(2 * (lane + TPK * (m >> 1)) + (m & 1), NP - 1) : min(lane + TPK * m, NP - 1);
@@ -1312,12 +1329,37 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         ej[m] = (code >> 8) & 255;
         ec[m] = code >> 16;
         g[m] = pre0[m] = pre1[m] = 0.0;
+        if constexpr (ROLE == 1) gb[m] = 0.0;
     }
+#pragma unroll
+    for (int h = 0; h < (PER > 1 ? PER / 2 : 1); h++) pv2[h] = kPairs && (NP % 2 == 0) && 2 * (lane + TPK * h) + 1 < NP;
+    // Neighbour data of the stage input, GB neighbours per call, their loads all
+    // issued before any is used (k's or kb's stencil slots, by `slot`).
+    constexpr int GB = WaveGeom<NBT>::GATHER;
+    auto gather = [&](const int kk, auto slot, const int cnt, double (&v)[GB][PER]) {
+        size_t kn[GB];
+#pragma unroll
+        for (int j = 0; j < GB; j++) kn[j] = (size_t)c.nbr[(size_t)slot(min(j, cnt - 1)) * c.nkp + kk] * NP;
+#pragma unroll
+        for (int j = 0; j < GB; j++) {
+            if constexpr (kPairs) {
+#pragma unroll
+                for (int h = 0; h < PER / 2; h++) {
+                    YT v0, v1;
+                    ld2<false>(Yin, kn[j] + eo[2 * h], kn[j] + eo[2 * h + 1], pv2[h], v0, v1);
+                    v[j][2 * h] = (double)v0;
+                    v[j][2 * h + 1] = (double)v1;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < PER; m++) v[j][m] = (double)Yin[kn[j] + eo[m]];
+            }
+        }
+    };
     if constexpr (kPairs) {
 #pragma unroll
         for (int h = 0; h < PER / 2; h++) {
             const int m = 2 * h;
-            pv2[h] = (NP % 2 == 0) && 2 * (lane + TPK * h) + 1 < NP;
             const bool v2 = pv2[h];
             const size_t o0 = base + eo[m], o1 = base + eo[m + 1];
             YT y0, y1;
@@ -1330,7 +1372,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
                 if (a.dmask >> x & 1) ld2<true>(Dop + (size_t)x * c.nk * NP, o0, o1, v2, dv[m][x], dv[m + 1][x]);
                 else dv[m][x] = dv[m + 1][x] = (R)0;
             }
-            if constexpr (MODE == RK2 || MODE == RK3) {
+            if constexpr ((MODE == RK2 || MODE == RK3) && !kLatePre) {
                 ld2<true>(a.ACC, o0, o1, v2, pre0[m], pre0[m + 1]);
                 ld2<true>(a.P0, o0, o1, v2, pre1[m], pre1[m + 1]);
             } else if constexpr (MODE == RK4S) {
@@ -1355,36 +1397,50 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     }
     // Field drift E.grad_k rho on the shell stencil (RungeKutta.h:258-276).  The
     // neighbours are fetched GB at a time: their index and data loads are all in
-    // flight together, and the sum keeps the stencil order.
-    if (grad) {
-        constexpr int GB = WaveGeom<NBT>::GATHER;
+    // flight together, and the sum keeps the stencil order.  First of a pair
+    // (ROLE 1): every neighbour of k that is also one of kb's adds to kb's sum gb
+    // as well, and kb's remaining neighbours are fetched after k's.  Second of a
+    // pair (ROLE 2): the sum is gb.
+    if constexpr (ROLE == 2) {
+#pragma unroll
+        for (int m = 0; m < PER; m++) g[m] = gb[m];
+    } else if (grad) {
         for (int n0 = 0; n0 < nn; n0 += GB) {
-            size_t kb[GB];
-#pragma unroll
-            for (int j = 0; j < GB; j++) kb[j] = (size_t)c.nbr[(size_t)min(n0 + j, nn - 1) * c.nkp + k] * NP;
             double v[GB][PER];
-#pragma unroll
-            for (int j = 0; j < GB; j++) {
-                if constexpr (kPairs) {
-#pragma unroll
-                    for (int h = 0; h < PER / 2; h++) {
-                        YT v0, v1;
-                        ld2<false>(Yin, kb[j] + eo[2 * h], kb[j] + eo[2 * h + 1], pv2[h], v0, v1);
-                        v[j][2 * h] = (double)v0;
-                        v[j][2 * h + 1] = (double)v1;
-                    }
-                } else {
-#pragma unroll
-                    for (int m = 0; m < PER; m++) v[j][m] = (double)Yin[kb[j] + eo[m]];
-                }
-            }
+            gather(k, [&](int j) { return n0 + j; }, nn - n0, v);
 #pragma unroll
             for (int j = 0; j < GB; j++)
                 if (n0 + j < nn) {
                     const double cn = tab[6 + n0 + j];
 #pragma unroll
                     for (int m = 0; m < PER; m++) g[m] += cn * v[j][m];
+                    if constexpr (ROLE == 1) {
+                        const int pj = c.pat[n0 + j];
+                        if (pj >= 0) {
+                            const double cb = tab[6 + pj];
+#pragma unroll
+                            for (int m = 0; m < PER; m++) gb[m] += cb * v[j][m];
+                        }
+                    }
                 }
+        }
+        if constexpr (ROLE == 1) {
+            if (c.pself >= 0) {
+                const double cb = tab[6 + c.pself];
+#pragma unroll
+                for (int m = 0; m < PER; m++) gb[m] += cb * y[m];
+            }
+            for (int t0 = 0; t0 < c.npnew; t0 += GB) {
+                double v[GB][PER];
+                gather(kb, [&](int j) { return c.pat[kMaxNbr + t0 + j]; }, c.npnew - t0, v);
+#pragma unroll
+                for (int j = 0; j < GB; j++)
+                    if (t0 + j < c.npnew) {
+                        const double cb = tab[6 + c.pat[kMaxNbr + t0 + j]];
+#pragma unroll
+                        for (int m = 0; m < PER; m++) gb[m] += cb * v[j][m];
+                    }
+            }
         }
     }
     if (nb < NBT)
@@ -1450,6 +1506,15 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
         store_block<NBT>(C, lane, acc);
     }
     __syncthreads();
+    if constexpr (kLatePre && (MODE == RK2 || MODE == RK3)) {
+#pragma unroll
+        for (int h = 0; h < PER / 2; h++) {
+            const int m = 2 * h;
+            const size_t o0 = base + eo[m], o1 = base + eo[m + 1];
+            ld2<true>(a.ACC, o0, o1, pv2[h], pre0[m], pre0[m + 1]);
+            ld2<true>(a.P0, o0, o1, pv2[h], pre1[m], pre1[m + 1]);
+        }
+    }
     // pv = -i (X_ij - conj(X_ji)) plus the RWA term on core -> non-core pairs; on
     // the diagonal this is 2 Im X_ii.
 #pragma unroll
@@ -1551,7 +1616,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
             }
         }
     }
-    if (!store) return;  // after the last barrier
+    if (!store) return;  // after the last barrier (a group past the grid serves no pair)
     if constexpr (kPairs && MODE != RHSO) {
         S *const outs = reinterpret_cast<S *>(a.OUT);  // RK stages 1-3 write the next stage input
 #pragma unroll
@@ -1595,18 +1660,33 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     }
 }
 
-template <int NBT, int MODE, typename R, typename S = double, bool B3 = false>
+// One k per workgroup, or with PAIR (ConstArgs::pair, one k per workgroup only)
+// the stencil pair (2q, 2q + 1).
+template <int NBT, int MODE, typename R, typename S = double, bool B3 = false, bool PAIR = false>
 __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_stage_wave(ConstArgs c, StageArgs a) {
     extern __shared__ double smem_d[];
     constexpr int TPK = WaveGeom<NBT>::TPK;
-    if constexpr (WaveGeom<NBT>::KPB == 1) {
+    double gb[WaveGeom<NBT>::PER];
+    if constexpr (PAIR) {
+        static_assert(WaveGeom<NBT>::KPB == 1, "stencil pairs: one k per workgroup");
+        // xcd_block and strip_k: bijections of the nk / 2 pairs (the pair grid has
+        // half as many columns: strips of sw / 2 pairs)
+        const int k = 2 * strip_k(xcd_block(), c.nk >> 1, c.sw >> 1, c.scol >> 1);
+        stage_wave<NBT, MODE, R, S, B3, 1>(c, a, k, k + 1, threadIdx.x, smem_d, true, gb);
+        __syncthreads();  // every lane is done with k's matrices
+        // the second call forms its lane-derived indices and addresses anew
+        // rather than keeping the first call's live across both
+        int lane2 = threadIdx.x;
+        asm volatile("" : "+v"(lane2));
+        stage_wave<NBT, MODE, R, S, B3, 2>(c, a, k + 1, -1, lane2, smem_d, true, gb);
+    } else if constexpr (WaveGeom<NBT>::KPB == 1) {
         // xcd_block and strip_k: bijections of [0, nk)
-        stage_wave<NBT, MODE, R, S, B3>(c, a, strip_k(xcd_block(), c.nk, c.sw, c.scol), threadIdx.x, smem_d, true);
+        stage_wave<NBT, MODE, R, S, B3>(c, a, strip_k(xcd_block(), c.nk, c.sw, c.scol), -1, threadIdx.x, smem_d, true, gb);
     } else {
         const int sub = threadIdx.x / TPK;
         const int kk = xcd_block() * WaveGeom<NBT>::KPB + sub;
-        stage_wave<NBT, MODE, R, S, B3>(c, a, strip_k(min(kk, c.nk - 1), c.nk, c.sw, c.scol), threadIdx.x % TPK,
-                                 smem_d + sub * wave_lds_doubles<NBT, R>(c.diss), kk < c.nk);
+        stage_wave<NBT, MODE, R, S, B3>(c, a, strip_k(min(kk, c.nk - 1), c.nk, c.sw, c.scol), -1, threadIdx.x % TPK,
+                                 smem_d + sub * wave_lds_doubles<NBT, R>(c.diss), kk < c.nk, gb);
     }
 }
 
@@ -1639,7 +1719,7 @@ __global__ __launch_bounds__(256) void k_stage_small_batch(const BatchEntry *E, 
     stage_small<NB, MODE, DISS>(be.c, a, w.y * blockDim.x + threadIdx.x);
 }
 
-template <int NBT, int MODE, typename R>
+template <int NBT, int MODE, typename R, bool PAIR = false>
 __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_stage_wave_batch(const BatchEntry *E, const int2 *blk,
                                                                                          const int *dmask, size_t tab_step) {
     extern __shared__ double smem_d[];
@@ -1649,12 +1729,18 @@ __global__ __launch_bounds__(WaveGeom<NBT>::TPK * WaveGeom<NBT>::KPB) void k_sta
     StageArgs a = be.a[rk4_stage_of<MODE>()];
     a.tab += tab_step;
     a.dmask = dmask[w.x];
-    if constexpr (WaveGeom<NBT>::KPB == 1) {
-        stage_wave<NBT, MODE, R>(be.c, a, w.y, threadIdx.x, smem_d, true);
+    double gb[WaveGeom<NBT>::PER];
+    if constexpr (PAIR) {  // every instance of the class is paired: block w.y serves pair w.y
+        static_assert(WaveGeom<NBT>::KPB == 1, "stencil pairs: one k per workgroup");
+        stage_wave<NBT, MODE, R, double, false, 1>(be.c, a, 2 * w.y, 2 * w.y + 1, threadIdx.x, smem_d, true, gb);
+        __syncthreads();  // every lane is done with the first k's matrices
+        stage_wave<NBT, MODE, R, double, false, 2>(be.c, a, 2 * w.y + 1, -1, threadIdx.x, smem_d, true, gb);
+    } else if constexpr (WaveGeom<NBT>::KPB == 1) {
+        stage_wave<NBT, MODE, R>(be.c, a, w.y, -1, threadIdx.x, smem_d, true, gb);
     } else {
         const int sub = threadIdx.x / TPK, kk = w.y * WaveGeom<NBT>::KPB + sub;
-        stage_wave<NBT, MODE, R>(be.c, a, min(kk, be.c.nk - 1), threadIdx.x % TPK,
-                                 smem_d + sub * wave_lds_doubles<NBT, R>(be.c.diss), kk < be.c.nk);
+        stage_wave<NBT, MODE, R>(be.c, a, min(kk, be.c.nk - 1), -1, threadIdx.x % TPK,
+                                 smem_d + sub * wave_lds_doubles<NBT, R>(be.c.diss), kk < be.c.nk, gb);
     }
 }
 
@@ -2422,6 +2508,7 @@ struct edus_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nk = 0, nkp = 0, nb = 0, np = 0, nch = 0, ncc = 0, n_nbr = 0, stepper = 0, order = 1, diss = 0, dherm = 0;
     int sw = 0, scol = 0;  // strip order of the wave kernels (strip_k)
+    int pair = 0, pself = -1, npnew = 0;  // stencil pairs of the wave kernels (ConstArgs::pair)
     int tpb = 0;           // threads per workgroup of the nb <= 4 kernel (0: by grid size)
     int precision = EDUS_PRECISION_FP64;
     float *Hf = nullptr, *Df = nullptr, *Uf = nullptr;  // FP32 operator copies (mixed precision)
@@ -2597,6 +2684,7 @@ ConstArgs const_args(const edus_ctx *c) {
     ca.Hf = c->Hf; ca.Df = c->Df; ca.Uf = c->Uf; ca.eidx = c->eidx;
     ca.nk = c->nk; ca.nkp = c->nkp; ca.nb = c->nb; ca.nch = c->nch; ca.ncc = c->ncc; ca.n_nbr = c->n_nbr;
     ca.diss = c->diss; ca.dherm = c->dherm; ca.wx2 = c->wx2; ca.sw = c->sw; ca.scol = c->scol;
+    ca.pair = c->pair; ca.pself = c->pself; ca.npnew = c->npnew; ca.pat = c->nbr + (size_t)std::max(c->n_nbr, 1) * c->nkp;
     ca.coul = c->coul; ca.N2 = c->N2; ca.ncut = c->ncut; ca.nu = c->nu; ca.n_cond = c->n_cond;
     ca.cxm = c->cxm; ca.sxm = c->sxm; ca.Tt = c->cT; ca.PW0 = c->cPW0; ca.EH = c->cEH; ca.screen = c->cscreen; ca.Hr = c->cHr;
     ca.Xw = c->cXw;
@@ -2631,27 +2719,37 @@ int launch_stage(edus_ctx *c, StageArgs a) {
         if (c->diss) go(std::true_type{});
         else go(std::false_type{});
     } else {
-        auto go = [&](auto tag) {
+        auto go = [&](auto tag, auto ptag) {
             constexpr int NBT = decltype(tag)::value, KPB = WaveGeom<NBT>::KPB;
-            const dim3 grid((c->nk + KPB - 1) / KPB), block(WaveGeom<NBT>::TPK * KPB);
+            constexpr bool PAIR = decltype(ptag)::value;
+            const dim3 grid(PAIR ? c->nk / 2 : (c->nk + KPB - 1) / KPB), block(WaveGeom<NBT>::TPK * KPB);
             const size_t lds_f = (size_t)KPB * wave_lds_doubles<NBT, float>(c->diss) * 8;
             const size_t lds_d = (size_t)KPB * wave_lds_doubles<NBT, double>(c->diss) * 8;
-            if constexpr (NBT >= 16) {
+            if constexpr (NBT >= 16 && !PAIR) {
                 if (c->precision == EDUS_PRECISION_BF16X3) {
                     hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float, double, true>), grid, block, lds_f, c->stream, ca, a);
                     return;
                 }
             }
             if (c->precision == EDUS_PRECISION_MIXED32)
-                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float, float>), grid, block, lds_f, c->stream, ca, a);
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float, float, false, PAIR>), grid, block, lds_f, c->stream, ca, a);
             else if (c->precision == EDUS_PRECISION_MIXED)
-                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float>), grid, block, lds_f, c->stream, ca, a);
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, float, double, false, PAIR>), grid, block, lds_f, c->stream, ca, a);
             else
-                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double>), grid, block, lds_d, c->stream, ca, a);
+                hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double, double, false, PAIR>), grid, block, lds_d, c->stream, ca, a);
         };
-        if (c->nb <= 8) go(std::integral_constant<int, 8>{});
-        else if (c->nb <= 16) go(std::integral_constant<int, 16>{});
-        else go(std::integral_constant<int, 32>{});
+        // stencil pairs only where a workgroup serves one k (nb > 8; see
+        // stencil_pairs), and where they measured faster: not in the BF16x3 form,
+        // whose split products leave no registers for the second k's stencil
+        // sum, nor at 32 bands with FP32 stage storage (2.24 ms per stage launch
+        // paired, 2.21 unpaired, at 512 x 512 k)
+        const bool pair = c->pair && c->precision != EDUS_PRECISION_BF16X3 &&
+                          !(c->precision == EDUS_PRECISION_MIXED32 && c->nb > 16);
+        if (c->nb <= 8) go(std::integral_constant<int, 8>{}, std::false_type{});
+        else if (c->nb <= 16 && pair) go(std::integral_constant<int, 16>{}, std::true_type{});
+        else if (c->nb <= 16) go(std::integral_constant<int, 16>{}, std::false_type{});
+        else if (pair) go(std::integral_constant<int, 32>{}, std::true_type{});
+        else go(std::integral_constant<int, 32>{}, std::false_type{});
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -2694,6 +2792,52 @@ int check_ctx(edus_ctx *c) {
     hipError_t e = hipSetDevice(c->dev);
     if (e != hipSuccess) return set_err(EDUS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
     return 0;
+}
+
+// Stencil overlap of the pairs (ka, kb) = (2q, 2q + 1) of a neighbour table
+// nbr[k][nn]: each slot j of kb's stencil is matched to the first unmatched slot
+// of ka's holding the same point (pat[i] = j for ka's slot i), else to ka itself
+// (pself = j), else listed as new (pat[kMaxNbr + t] = j, t < npnew).  True when
+// every pair has the same pattern, as on a uniform grid of even length along the
+// fastest axis; the wave kernels then serve a pair per workgroup.
+bool stencil_pairs(const int32_t *nbr, int nk, int nn, int *pat, int &pself, int &npnew) {
+    if (nn > kMaxNbr || nk < 2 || nk % 2) return false;
+    struct Pattern {
+        int pm[kMaxNbr], pn[kMaxNbr], self, cnt;
+    };
+    auto pattern = [&](int q, Pattern &P) {
+        const int32_t *A = nbr + (size_t)(2 * q) * nn, *B = A + nn;
+        uint64_t used = 0;
+        P.self = -1;
+        P.cnt = 0;
+        for (int i = 0; i < nn; i++) P.pm[i] = -1;
+        for (int j = 0; j < nn; j++) {
+            int hit = -1;
+            for (int i = 0; i < nn && hit < 0; i++)
+                if (!(used >> i & 1) && A[i] == B[j]) hit = i;
+            if (hit >= 0) {
+                P.pm[hit] = j;
+                used |= 1ull << hit;
+            } else if (B[j] == 2 * q && P.self < 0) {
+                P.self = j;
+            } else {
+                P.pn[P.cnt++] = j;
+            }
+        }
+    };
+    Pattern P0, P;
+    pattern(0, P0);
+    for (int q = 1; q < nk / 2; q++) {
+        pattern(q, P);
+        if (P.self != P0.self || P.cnt != P0.cnt || std::memcmp(P.pm, P0.pm, nn * sizeof(int)) ||
+            std::memcmp(P.pn, P0.pn, P.cnt * sizeof(int)))
+            return false;
+    }
+    for (int i = 0; i < nn; i++) pat[i] = P0.pm[i];
+    for (int t = 0; t < P0.cnt; t++) pat[kMaxNbr + t] = P0.pn[t];
+    pself = P0.self;
+    npnew = P0.cnt;
+    return true;
 }
 
 }  // namespace
@@ -2909,9 +3053,14 @@ int edus_create(const edus_problem *p, edus_ctx **out) {
         if (rc) return fail(rc);
     }
     {
-        std::vector<int> nbr((size_t)std::max(p->n_nbr, 1) * c->nkp, 0);
+        // [n_nbr][nkp] table, then the stencil-pair pattern (2 kMaxNbr ints)
+        const size_t ntab = (size_t)std::max(p->n_nbr, 1) * c->nkp;
+        std::vector<int> nbr(ntab + 2 * kMaxNbr, -1);
         for (int k = 0; k < p->nk; k++)
             for (int n = 0; n < p->n_nbr; n++) nbr[(size_t)n * c->nkp + k] = p->nbr[(size_t)k * p->n_nbr + n];
+        if (!c->soa && nb > 8 && p->n_nbr > 0 && p->nk % 2 == 0 && c->scol % 2 == 0 &&
+            stencil_pairs(p->nbr, p->nk, p->n_nbr, nbr.data() + ntab, c->pself, c->npnew))
+            c->pair = 1;
         if ((rc = dalloc(c, &c->nbr, nbr.size()))) return fail(rc);
         if (hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return fail(set_err(EDUS_ERR_HIP, "upload nbr"));
@@ -3460,7 +3609,7 @@ struct edus_batch {
     std::vector<edus_ctx *> inst;
     struct Cls {
         bool soa = true;
-        int nb_small = 0, nbt = 0, precision = EDUS_PRECISION_FP64, diss = 0;
+        int nb_small = 0, nbt = 0, precision = EDUS_PRECISION_FP64, diss = 0, pair = 0;
         std::vector<int> members;
         int2 *blk = nullptr;
         int nblocks = 0;
@@ -3522,19 +3671,22 @@ int batch_launch(edus_batch *b, const edus_batch::Cls &k, size_t tab_step) {
         if (k.diss) go(std::true_type{});
         else go(std::false_type{});
     } else {
-        auto go = [&](auto tag) {
+        auto go = [&](auto tag, auto ptag) {
             constexpr int NBT = decltype(tag)::value, KPB = WaveGeom<NBT>::KPB;
+            constexpr bool PAIR = decltype(ptag)::value;
             const dim3 block(WaveGeom<NBT>::TPK * KPB);
             const size_t lds_f = (size_t)KPB * wave_lds_doubles<NBT, float>(k.diss) * 8;
             const size_t lds_d = (size_t)KPB * wave_lds_doubles<NBT, double>(k.diss) * 8;
             if (k.precision == EDUS_PRECISION_MIXED)
-                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, float>), grid, block, lds_f, b->stream, b->entries, k.blk, b->masks, tab_step);
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, float, PAIR>), grid, block, lds_f, b->stream, b->entries, k.blk, b->masks, tab_step);
             else
-                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, double>), grid, block, lds_d, b->stream, b->entries, k.blk, b->masks, tab_step);
+                hipLaunchKernelGGL((k_stage_wave_batch<NBT, MODE, double, PAIR>), grid, block, lds_d, b->stream, b->entries, k.blk, b->masks, tab_step);
         };
-        if (k.nbt == 8) go(std::integral_constant<int, 8>{});
-        else if (k.nbt == 16) go(std::integral_constant<int, 16>{});
-        else go(std::integral_constant<int, 32>{});
+        if (k.nbt == 8) go(std::integral_constant<int, 8>{}, std::false_type{});
+        else if (k.nbt == 16 && k.pair) go(std::integral_constant<int, 16>{}, std::true_type{});
+        else if (k.nbt == 16) go(std::integral_constant<int, 16>{}, std::false_type{});
+        else if (k.pair) go(std::integral_constant<int, 32>{}, std::true_type{});
+        else go(std::integral_constant<int, 32>{}, std::false_type{});
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -3588,20 +3740,20 @@ int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch
         b->R_tot += 7 + c->n_nbr;
     }
     // kernel classes: small kernels by band count, wave kernels by tile and
-    // precision, each split by the presence of dephasing
+    // precision, each split by the presence of dephasing and by stencil pairing
     for (int i = 0; i < n; i++) {
         const edus_ctx *c = b->inst[i];
         const int nbt = c->soa ? 0 : (c->nb <= 8 ? 8 : c->nb <= 16 ? 16 : 32);
         edus_batch::Cls *hit = nullptr;
         for (auto &k : b->cls)
             if (k.soa == c->soa && k.nb_small == (c->soa ? c->nb : 0) && k.nbt == nbt && k.precision == c->precision &&
-                k.diss == c->diss)
+                k.diss == c->diss && k.pair == c->pair)
                 hit = &k;
         if (!hit) {
             b->cls.emplace_back();
             hit = &b->cls.back();
             hit->soa = c->soa; hit->nb_small = c->soa ? c->nb : 0; hit->nbt = nbt; hit->precision = c->precision;
-            hit->diss = c->diss;
+            hit->diss = c->diss; hit->pair = c->pair;
         }
         hit->members.push_back(i);
     }
@@ -3610,7 +3762,9 @@ int edus_batch_create(const edus_problem *const *problems, int32_t n, edus_batch
         for (int i : k.members) {
             const edus_ctx *c = b->inst[i];
             const int kpb = c->nb <= 8 ? WaveGeom<8>::KPB : c->nb <= 16 ? WaveGeom<16>::KPB : WaveGeom<32>::KPB;
-            const int nblk = c->soa ? (c->nk + kSmallBatchThreads - 1) / kSmallBatchThreads : (c->nk + kpb - 1) / kpb;
+            const int nblk = c->soa    ? (c->nk + kSmallBatchThreads - 1) / kSmallBatchThreads
+                             : c->pair ? c->nk / 2
+                                       : (c->nk + kpb - 1) / kpb;
             for (int q = 0; q < nblk; q++) map.push_back(make_int2(i, q));
         }
         k.nblocks = (int)map.size();

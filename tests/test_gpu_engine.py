@@ -221,6 +221,31 @@ def test_rhs_wave_kernel_matches_oracle(tmp_path, nw, t2):
         assert np.max(np.abs(pv - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("nw", [16, 32])
+@pytest.mark.parametrize("listed", [False, True])
+def test_rhs_stencil_pairs_and_single_k_match_oracle(tmp_path, nw, listed):
+    """Above 8 bands a workgroup serves the stencil pair (2q, 2q+1) when every
+    pair's stencils overlap alike (a uniform grid, whose rows have even length)
+    and one k otherwise (the same points as an explicit shuffled list, nkPT
+    read).  Both match the oracle, with the drift on, in all three field
+    components; the list's neighbours come from the oracle's own search."""
+    import test_kpt_read as KR
+    s = _wannier_setup(str(tmp_path), nw, nk=12, rmax=1)
+    if listed:
+        text, _ = KR._listed(str(tmp_path / "input.txt"), np.random.default_rng(11).permutation(s.info["nk"]))
+        (tmp_path / "input_list.txt").write_text(text)
+        s = N.Setup(str(tmp_path / "input_list.txt"))
+    osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=listed)
+    eng = N.Engine(s.problem())
+    P = random_hermitian_state(s.array("rho0", True).reshape(-1, nw, nw), 0.05, nw)
+    eng.set_state(P)
+    field6 = np.array([1e-3, -2e-3, 3e-3, 4e-4, 0, 0])
+    pv = eng.rhs(field6, True)
+    ref = osys["sys"].rhs(P, field6, True)
+    print(f"nw={nw} listed={listed}: max rel err {np.max(np.abs(pv - ref)) / np.max(np.abs(ref)):.3e}", flush=True)
+    assert np.max(np.abs(pv - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
 @pytest.mark.parametrize("nw,nk", [(32, 80), (20, 150)])
 def test_rhs_wave_kernel_strip_order_matches_oracle(tmp_path, nw, nk):
     """Grids whose rows are longer than the engine's strip width (64 columns at
