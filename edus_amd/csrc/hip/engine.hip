@@ -1125,6 +1125,7 @@ __device__ __forceinline__ void store_block(T *C, int t, const T (&acc)[WaveGeom
 // and op(B)[l/16][col l%16]; accumulator element i of lane l is C[row][l%16] with
 // row = l/16 + 4 i for the FP64 instruction and 4 (l/16) + i for the FP32 one.
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 template <typename R> struct MfmaAcc;
 template <> struct MfmaAcc<double> { using T = v4d; };
@@ -1144,6 +1145,24 @@ __device__ __forceinline__ void mfma_tile(int t, int &r0, int &c0) {
     c0 = NBT == 32 ? (w & 1) * 16 : 0;
 }
 
+// LDS complex element access: an FP64 complex moves as one 16-byte access
+// (ds_read_b128 / ds_write_b128; the matrices start 16-byte aligned and every
+// element is 16 bytes) rather than two 8-byte halves, whose 32-lane passes put
+// rows r and r + 1 of neighbouring K slices on one bank.
+template <typename T>
+__device__ __forceinline__ T lds_get(const T *p) { return *p; }
+template <typename T>
+__device__ __forceinline__ void lds_put(T *p, T v) { *p = v; }
+template <>
+__device__ __forceinline__ cd lds_get<cd>(const cd *p) {
+    const v2d v = *reinterpret_cast<const v2d *>(p);
+    return cd{v.x, v.y};
+}
+template <>
+__device__ __forceinline__ void lds_put<cd>(cd *p, cd v) {
+    *reinterpret_cast<v2d *>(p) = v2d{v.x, v.y};
+}
+
 template <int NBT, int OPA, int OPB, typename R, typename T = typename CplxOf<R>::T, typename V = typename MfmaAcc<R>::T>
 __device__ __forceinline__ void gemm_mfma(const T *A, const T *B, int t, V &cre, V &cim) {
     constexpr int LD = WaveGeom<NBT>::LD;
@@ -1155,9 +1174,9 @@ __device__ __forceinline__ void gemm_mfma(const T *A, const T *B, int t, V &cre,
 #pragma unroll
     for (int k0 = 0; k0 < NBT; k0 += 4) {
         const int l = k0 + kq;
-        T a = A[ra * LD + l];
+        T a = lds_get(A + ra * LD + l);
         if (OPA) a.y = -a.y;
-        T b = (OPB == 0) ? B[cb * LD + l] : B[l * LD + cb];
+        T b = lds_get(B + ((OPB == 0) ? cb * LD + l : l * LD + cb));
         if (OPB == 2) b.y = -b.y;
         cre = mfma16(a.x, b.x, cre);
         cre = mfma16(-a.y, b.y, cre);
@@ -1173,7 +1192,7 @@ __device__ __forceinline__ void store_mfma(T *C, int t, const V &cre, const V &c
     mfma_tile<NBT>(t, r0, c0);
     const int lane = t & 63;
 #pragma unroll
-    for (int i = 0; i < 4; i++) C[(r0 + mfma_row<R>(lane, i)) * LD + c0 + (lane & 15)] = T{cre[i], cim[i]};
+    for (int i = 0; i < 4; i++) lds_put(C + (r0 + mfma_row<R>(lane, i)) * LD + c0 + (lane & 15), T{cre[i], cim[i]});
 }
 
 // BF16x3 split products on the bf16 matrix cores: each FP32 operand x is split
@@ -1336,12 +1355,13 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     // Neighbour data of the stage input, GB neighbours per call, their loads all
     // issued before any is used (k's or kb's stencil slots, by `slot`).
     constexpr int GB = WaveGeom<NBT>::GATHER;
-    auto gather = [&](const int kk, auto slot, const int cnt, double (&v)[GB][PER]) {
-        size_t kn[GB];
+    auto gather = [&](const int kk, auto slot, const int cnt, auto &v) {
+        constexpr int NV = std::extent<std::remove_reference_t<decltype(v)>>::value;
+        size_t kn[NV];
 #pragma unroll
-        for (int j = 0; j < GB; j++) kn[j] = (size_t)c.nbr[(size_t)slot(min(j, cnt - 1)) * c.nkp + kk] * NP;
+        for (int j = 0; j < NV; j++) kn[j] = (size_t)c.nbr[(size_t)slot(min(j, cnt - 1)) * c.nkp + kk] * NP;
 #pragma unroll
-        for (int j = 0; j < GB; j++) {
+        for (int j = 0; j < NV; j++) {
             if constexpr (kPairs) {
 #pragma unroll
                 for (int h = 0; h < PER / 2; h++) {
@@ -1430,11 +1450,13 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
 #pragma unroll
                 for (int m = 0; m < PER; m++) gb[m] += cb * y[m];
             }
-            for (int t0 = 0; t0 < c.npnew; t0 += GB) {
-                double v[GB][PER];
+            // kb's own rows (5 on the hexagonal two-shell stencil) in one batch
+            constexpr int GBN = GB < 5 ? 5 : GB;
+            for (int t0 = 0; t0 < c.npnew; t0 += GBN) {
+                double v[GBN][PER];
                 gather(kb, [&](int j) { return c.pat[kMaxNbr + t0 + j]; }, c.npnew - t0, v);
 #pragma unroll
-                for (int j = 0; j < GB; j++)
+                for (int j = 0; j < GBN; j++)
                     if (t0 + j < c.npnew) {
                         const double cb = tab[6 + c.pat[kMaxNbr + t0 + j]];
 #pragma unroll
@@ -1465,10 +1487,10 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
                     ddi += E[x] * dv[m + 1][x];
                 }
                 const R yr = (R)y[m], yi = (R)y[m + 1];
-                A[i * LD + j] = CT{yr, yi};
-                A[j * LD + i] = CT{yr, -yi};
-                B[i * LD + j] = CT{dr, di};
-                B[j * LD + i] = CT{dr, c.dherm ? -di : -hv[m + 1] + ddi};
+                lds_put(A + i * LD + j, CT{yr, yi});
+                lds_put(A + j * LD + i, CT{yr, -yi});
+                lds_put(B + i * LD + j, CT{dr, di});
+                lds_put(B + j * LD + i, CT{dr, c.dherm ? -di : -hv[m + 1] + ddi});
                 continue;
             }
             if (kPairs && (m & 1) == 1 && pv2[m >> 1] && eo[m - 1] >= nb) continue;  // written with m - 1
@@ -1521,7 +1543,7 @@ __device__ __forceinline__ void stage_wave(const ConstArgs &c, const StageArgs &
     for (int m = 0; m < PER; m++) {
         if (kPairs && (m & 1) == 1 && pv2[m >> 1] && eo[m - 1] >= nb) continue;  // formed with m - 1
         const int i = ei[m], j = ej[m];
-        const CT xij = C[i * LD + j], xji = C[j * LD + i], pij = A[i * LD + j];
+        const CT xij = lds_get(C + i * LD + j), xji = lds_get(C + j * LD + i), pij = lds_get(A + i * LD + j);
         const double bre = (double)xij.x - (double)xji.x, bim = (double)xij.y + (double)xji.y;
         const double wr = (i < c.nch && j >= c.nch) ? c.wx2 : 0.0;
         const double vr = bim + -wr * (double)pij.y, vi = -bre + wr * (double)pij.x;
