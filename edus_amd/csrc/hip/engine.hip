@@ -772,9 +772,16 @@ __global__ __launch_bounds__(64) void k_coul_s1_mfma(CoulArgs a) {
 
 // F_xy[mm][e] = w (1 + corr_mn) sum_i1 x_m(i1) S_y[i1][n][e]: one workgroup per
 // mm; thread t takes element e = t % NP and every (256/NP)-th term of the sum,
-// and the parts are added in a fixed order per element.
+// and the parts are added in a fixed order per element.  The same workgroup then
+// forms X_mn from its F (coul_x_elem) and, for mm = 0, the Hartree shift: X_mn
+// needs no other mode's F, so no launch of its own.
+constexpr int kCoulMaxNP = 16;  // the mean field runs at nb <= 4
+
+__device__ __forceinline__ void coul_x_elem(const CoulArgs &a, const double (*Fs)[kCoulMaxNP], int mm, int u);
+
 __global__ __launch_bounds__(kCoulThreads) void k_coul_s2(CoulArgs a) {
     __shared__ double red[4][kCoulThreads];
+    __shared__ double Fs[4][kCoulMaxNP];
     const int nmm = a.ncut * a.ncut, mm = blockIdx.x, m = mm / a.ncut, n = mm % a.ncut, t = threadIdx.x;
     const int NP = a.NP, nparts = kCoulThreads / NP, e = t % NP, part = t / NP;
     double cc = 0, cs = 0, sc = 0, ss = 0;
@@ -801,27 +808,28 @@ __global__ __launch_bounds__(kCoulThreads) void k_coul_s2(CoulArgs a) {
             double v = 0.0;
             for (int q = 0; q < nparts; q++) v += red[qd][q * NP + t];
             a.F[((size_t)qd * nmm + mm) * NP + t] = v * f;
+            Fs[qd][t] = v * f;
         }
     }
-}
-
-__global__ void k_coul_x(CoulArgs a) {
-    const int nb = a.nb, nmm = a.ncut * a.ncut, nb2 = nb * nb;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nb) {  // Hartree shift from mode 0 (Coulomb.h:385-410)
+    __syncthreads();
+    if (mm == 0 && t < a.nb) {  // Hartree shift from mode 0 (Coulomb.h:385-410)
         double e = 0.;
-        for (int jc = 0; jc < nb; jc++) e += a.F[jc] * a.vh[2 * (t * nb + jc)];
+        for (int jc = 0; jc < a.nb; jc++) e += Fs[0][jc] * a.vh[2 * (t * a.nb + jc)];
         a.EH[t] = e - a.EH0[t];
     }
-    if (t >= nmm * a.nu) return;
-    const int mm = t / a.nu, u = t % a.nu;
+    if (t < a.nu) coul_x_elem(a, Fs, mm, t);
+}
+
+// X_mn for upper element u of mode mm from the mode's F (Coulomb.h:413-470).
+__device__ __forceinline__ void coul_x_elem(const CoulArgs &a, const double (*Fs)[kCoulMaxNP], int mm, int u) {
+    const int nb = a.nb, nmm = a.ncut * a.ncut, nb2 = nb * nb;
     int i = 0, j = 0;
     for (int q = u, r = 0; ; r++) {
         if (q < nb - r) { i = r; j = r + q; break; }
         q -= nb - r;
     }
     auto fget = [&](int qd) -> cd {
-        const double *F = a.F + ((size_t)qd * nmm + mm) * a.NP;
+        const double *F = Fs[qd];
         if (i == j) return mk(F[i], 0.0);
         const int p = pair_index(i, j, nb);
         return mk(F[nb + 2 * p], F[nb + 2 * p + 1]);
@@ -2671,12 +2679,11 @@ CoulArgs coul_args(edus_ctx *c, const double *Y) {
 // Mean-field coefficients of state Y: reduction, X_mn, column sums (4 launches).
 int launch_coulomb(edus_ctx *c, const double *Y) {
     CoulArgs a = coul_args(c, Y);
-    const int tx = std::max(c->ncut * c->ncut * c->nu, c->nb), t3 = c->N2 * c->ncut * c->nu;
+    const int t3 = c->N2 * c->ncut * c->nu;
     // 64-thread workgroups: the passes have a few 10^4 outputs each, and small
     // workgroups spread them over all CUs.  Pass 1 runs a wave per 16 x 16 tile.
     hipLaunchKernelGGL(k_coul_s1_mfma, dim3((c->np * c->N1 + 15) / 16, (2 * c->ncut + 15) / 16), dim3(64), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_s2, dim3(c->ncut * c->ncut), dim3(kCoulThreads), 0, c->stream, a);
-    hipLaunchKernelGGL(k_coul_x, dim3((tx + 63) / 64), dim3(64), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_s3, dim3((t3 + 63) / 64), dim3(64), 0, c->stream, a);
     hipLaunchKernelGGL(k_coul_w_mfma, dim3((c->N1 + 15) / 16, (2 * c->nu * c->N2 + 15) / 16), dim3(64), 0, c->stream, a,
                        c->N1);
