@@ -2768,12 +2768,9 @@ int launch_stage(edus_ctx *c, StageArgs a) {
                 hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double, double, false, PAIR>), grid, block, lds_d, c->stream, ca, a);
         };
         // stencil pairs only where a workgroup serves one k (nb > 8; see
-        // stencil_pairs), and where they measured faster: not in the BF16x3 form,
-        // whose split products leave no registers for the second k's stencil
-        // sum, nor at 32 bands with FP32 stage storage (2.24 ms per stage launch
-        // paired, 2.21 unpaired, at 512 x 512 k)
-        const bool pair = c->pair && c->precision != EDUS_PRECISION_BF16X3 &&
-                          !(c->precision == EDUS_PRECISION_MIXED32 && c->nb > 16);
+        // stencil_pairs), and not in the BF16x3 form, whose split products leave
+        // no registers for the second k's stencil sum
+        const bool pair = c->pair && c->precision != EDUS_PRECISION_BF16X3;
         if (c->nb <= 8) go(std::integral_constant<int, 8>{}, std::false_type{});
         else if (c->nb <= 16 && pair) go(std::integral_constant<int, 16>{}, std::true_type{});
         else if (c->nb <= 16) go(std::integral_constant<int, 16>{}, std::false_type{});
