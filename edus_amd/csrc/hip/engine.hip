@@ -2768,9 +2768,12 @@ int launch_stage(edus_ctx *c, StageArgs a) {
                 hipLaunchKernelGGL((k_stage_wave<NBT, MODE, double, double, false, PAIR>), grid, block, lds_d, c->stream, ca, a);
         };
         // stencil pairs only where a workgroup serves one k (nb > 8; see
-        // stencil_pairs), and not in the BF16x3 form, whose split products leave
-        // no registers for the second k's stencil sum
-        const bool pair = c->pair && c->precision != EDUS_PRECISION_BF16X3;
+        // stencil_pairs), and where they measured faster: not in the BF16x3 form,
+        // whose split products leave no registers for the second k's stencil
+        // sum, nor in the 32-band Taylor step with FP32 storage (3.73 ms paired,
+        // 3.53 one k per workgroup; its RK4 stages gain: 2.21 against 2.26 ms)
+        const bool pair = c->pair && c->precision != EDUS_PRECISION_BF16X3 &&
+                          !(MODE == TAY && c->nb > 16 && c->precision == EDUS_PRECISION_MIXED32);
         if (c->nb <= 8) go(std::integral_constant<int, 8>{}, std::false_type{});
         else if (c->nb <= 16 && pair) go(std::integral_constant<int, 16>{}, std::true_type{});
         else if (c->nb <= 16) go(std::integral_constant<int, 16>{}, std::false_type{});
