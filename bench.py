@@ -85,8 +85,8 @@ def main():
                          "known-size kernel of the HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the k-grid of one problem lives in one HBM); "
-              "multi-GPU k-slab sharding is a design in DESIGN.md, not built.", file=sys.stderr)
+        print("bench.py: runs on one GPU only (the k-grid of one problem lives in one HBM); "
+              "sharding the grid in k-slabs over several GPUs is not built.", file=sys.stderr)
         sys.exit(2)
     if args.precision:
         args.forms = args.precision
@@ -141,8 +141,9 @@ def stage_model(info, nd, form):
     order s: s + 2 (read y_n, write f_n, read s - 1 old f, write y_n+1), +1 for
     the Coulomb pass.  Flops: one complex NBT^3 product for the commutator
     (NBT the padded tile, 16 or 32 on the matrix cores), four more with
-    dephasing.  Also the survey's figure S (4 + n_op) per RK4 launch
-    (S = 16 nb^2, every operator full complex, all three dipole components)."""
+    dephasing.  Also the full-complex figure S (4 + n_op) per RK4 launch
+    (S = 16 nb^2, every operator full complex, all three dipole components),
+    reported as survey_bytes_*."""
     from edus_amd import native as N
 
     nb = info["nb"]

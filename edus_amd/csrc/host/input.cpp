@@ -437,7 +437,7 @@ Input parse_input(const std::string &path) {
                             set_a(6.3303100000, ax, ay, ax, -ay);
                             in.Nb[0] = 0; in.Nb[1] = 1; in.Nb[2] = 1;
                         } else {
-                            throw Error("TightBinding type " + tb + " is not supported by this build");
+                            throw Error("TightBinding type " + tb + " is not supported by this library");
                         }
                         reciprocal_from_direct(in.a, in.b);
                         in.iMode = "TB";

@@ -59,7 +59,7 @@ void TBModel::init(const std::string &nm, int nch, int ncv, const std::vector<do
     name = nm;
     Nch = nch;
     Ncv = ncv;
-    if (!supported()) throw Error("TightBinding type " + nm + " is not supported by this build");
+    if (!supported()) throw Error("TightBinding type " + nm + " is not supported by this library");
     if (b_rows)
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) kcart[i][j] = b_rows[j][i];

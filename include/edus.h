@@ -31,7 +31,7 @@ extern "C" {
 #define EDUS_ERR_INVALID     -1  /* bad argument, shape or table */
 #define EDUS_ERR_HIP         -2  /* HIP runtime failure (message has the hipError_t) */
 #define EDUS_ERR_NOMEM       -3  /* device allocation failed */
-#define EDUS_ERR_UNSUPPORTED -4  /* configuration this build does not implement */
+#define EDUS_ERR_UNSUPPORTED -4  /* configuration this library does not implement */
 #define EDUS_ERR_NODEVICE    -5  /* no gfx950 device visible */
 
 #define EDUS_STEPPER_RK4     0
@@ -60,7 +60,7 @@ extern "C" {
  * reference counterpart. */
 #define EDUS_PRECISION_BF16X3 3
 
-#define EDUS_MAX_NB          32  /* largest band count with a kernel in this build */
+#define EDUS_MAX_NB          32  /* largest band count with a kernel in this library */
 #define EDUS_MAX_ORDER        8  /* largest Adams-Bashforth / Taylor order */
 
 typedef struct edus_ctx edus_ctx;

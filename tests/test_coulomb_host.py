@@ -1,6 +1,6 @@
 """Coulomb mean-field input path on the CPU: the host reads the reference's
 coefficient files (Fourier_Coulomb.cpp:505-622) exactly as numpy parses them,
-and rejects the Coulomb variants this build does not carry.  (Without
+and rejects the Coulomb variants this library does not carry.  (Without
 Read_Coulomb_from_files the host computes the coefficients on the GPU; that path
 is tested in test_gpu_coulomb_gen.py.)"""
 import os

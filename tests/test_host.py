@@ -42,7 +42,7 @@ BASE = open(os.path.join(Hh.AB5, "input.txt")).read()
     (lambda s: s.replace("it_resolution 5", "it_resolution 5\nSpectrum"), "option Spectrum not available"),
     (lambda s: s.replace("1 70 70", "1 71 71"), "must be even"),
     (lambda s: s.replace("nkPT{", "nkPT read {"), "Cannot read the k points of nkPT read"),
-    (lambda s: s.replace("Haldane_CoreBN_Nedge", "GaSe_kp2"), "not supported by this build"),
+    (lambda s: s.replace("Haldane_CoreBN_Nedge", "GaSe_kp2"), "not supported by this library"),
 ])
 def test_input_errors(tmp_path, mutate, message):
     with pytest.raises(N.EdusError, match=re.escape(message)):
