@@ -3630,6 +3630,16 @@ int edus_last_step_time(edus_ctx *c, double *ms, int32_t *nlaunch) {
     return 0;
 }
 
+int edus_launch_schedule(edus_ctx *c, int32_t *out) {
+    if (!c || !out) return set_err(EDUS_ERR_INVALID, "null argument");
+    out[0] = c->soa ? 0 : (c->nb <= 8 ? 8 : c->nb <= 16 ? 16 : 32);
+    out[1] = c->sw;
+    out[2] = c->pair;
+    out[3] = c->pair ? c->npnew : 0;
+    out[4] = c->soa ? (c->tpb ? c->tpb : (c->nk < 256 * 256 ? 64 : 256)) : 0;
+    return 0;
+}
+
 // ---------------------------------------------------------------- batched instances
 struct edus_batch {
     int dev = 0;

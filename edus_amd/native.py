@@ -100,6 +100,7 @@ hip.edus_rhs.argtypes = [C.c_void_p, _dp, C.c_uint8, _dp]
 hip.edus_get_state.argtypes = [C.c_void_p, _dp]
 hip.edus_set_state.argtypes = [C.c_void_p, _dp]
 hip.edus_last_step_time.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
+hip.edus_launch_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
 hip.edus_autotune.argtypes = [C.c_void_p, _dp, C.c_uint8, C.c_int32, C.POINTER(C.c_int32)]
 hip.edus_wannier_hk.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
 hip.edus_wannier_hk_fp32.argtypes = [_ip, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, _dp]
@@ -403,6 +404,12 @@ class Engine:
         n = C.c_int32()
         hip.edus_last_step_time(self._h, C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+    def launch_schedule(self):
+        """edus_launch_schedule: {tile, strip_width, stencil_pairs, pair_own_rows, small_threads}."""
+        out = (C.c_int32 * 5)()
+        _check_hip(hip.edus_launch_schedule(self._h, out), "edus_launch_schedule")
+        return dict(zip(("tile", "strip_width", "stencil_pairs", "pair_own_rows", "small_threads"), list(out)))
 
     def autotune(self, field6, grad_on=True, reps=5):
         """Launch schedule picked by timing (edus_autotune); returns the kept strip

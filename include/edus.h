@@ -209,6 +209,16 @@ int edus_set_state(edus_ctx *ctx, const double *rho);
  * with HIP events on the context's stream; *nlaunch receives the kernel count. */
 int edus_last_step_time(edus_ctx *ctx, double *ms, int32_t *nlaunch);
 
+/* The stage kernels' launch schedule of the context, out[5]: the padded tile of
+ * the wave kernels (8, 16, 32; 0 for the thread-per-k kernel of 1..4 bands), the
+ * strip width of their workgroup walk (0 = k order), 1 when a workgroup serves a
+ * stencil pair of k (2q, 2q + 1: every pair's 12-point stencils overlap alike,
+ * as on a uniform grid; the BF16x3 form and the FP32-storage Taylor step at 32
+ * bands still run one k per workgroup), the stencil rows the second k of a pair
+ * fetches itself, and the threads per workgroup of the thread-per-k kernel.  No
+ * reference counterpart (a launch diagnostic). */
+int edus_launch_schedule(edus_ctx *ctx, int32_t *out);
+
 /* Picks the stage kernels' launch schedule by measurement on the context's own
  * data and keeps it for every later launch: for 5..32 bands the strip width of
  * the workgroup walk over the grid (0 = k order, or 16..256 columns narrower than

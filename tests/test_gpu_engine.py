@@ -237,6 +237,9 @@ def test_rhs_stencil_pairs_and_single_k_match_oracle(tmp_path, nw, listed):
         s = N.Setup(str(tmp_path / "input_list.txt"))
     osys = Hh.oracle_system(s, use_oracle_tables=False, bruteforce_neighbours=listed)
     eng = N.Engine(s.problem())
+    sched = eng.launch_schedule()
+    # the uniform grid pairs, its second k fetching 5 of its 12 rows itself; the list does not
+    assert (sched["stencil_pairs"], sched["pair_own_rows"]) == ((0, 0) if listed else (1, 5)), sched
     P = random_hermitian_state(s.array("rho0", True).reshape(-1, nw, nw), 0.05, nw)
     eng.set_state(P)
     field6 = np.array([1e-3, -2e-3, 3e-3, 4e-4, 0, 0])
